@@ -1,0 +1,25 @@
+"""fem_2d_amd: MI355X-native Galerkin GEP assembly for fem_2d domains.
+
+The prelude mirrors the reference crate's (src/lib.rs:14-39) names for the
+assembly path: Mesh / HRef / PRef / Orders build the input, Domain numbers the
+DoFs, and galerkin_sample_gep_hcurl assembles A and B with HIP kernels on the
+GPU.
+"""
+
+from .domain import BasisDir, BasisSpec, ContinuityCondition, Domain
+from .errors import (EmptyDOFSet, Fem2dError, GalerkinSamplingError, HRefError, InvalidGLQSettings, MeshAccessError,
+                     MeshPanic, NativeError, PRefError, WrongContinuityCondition)
+from .galerkin import (GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace, HierCurlIntegral,
+                       HierPoly, KOLShapeFn, L2Inner, ShapeFn, default_context, galerkin_sample_gep_hcurl,
+                       gauss_quadrature_points)
+from .mesh import (MAX_POLYNOMIAL_ORDER, MIN_EDGE_LENGTH, Edge, Elem, HLevels, HRef, Mesh, Node, Orders, PolyOrders,
+                   PRef)
+
+__all__ = [
+    "BasisDir", "BasisSpec", "ContinuityCondition", "Domain", "EmptyDOFSet", "Fem2dError", "GalerkinSamplingError",
+    "HRefError", "InvalidGLQSettings", "MeshAccessError", "MeshPanic", "NativeError", "PRefError",
+    "WrongContinuityCondition", "GEP", "MIN_GLQ_ORDER", "Context", "CurlCurl", "DeviceDomain", "HierCurlBasisFnSpace",
+    "HierCurlIntegral", "HierPoly", "KOLShapeFn", "L2Inner", "ShapeFn", "default_context",
+    "galerkin_sample_gep_hcurl", "gauss_quadrature_points", "MAX_POLYNOMIAL_ORDER", "MIN_EDGE_LENGTH", "Edge", "Elem",
+    "HLevels", "HRef", "Mesh", "Node", "Orders", "PolyOrders", "PRef",
+]

@@ -1,0 +1,184 @@
+"""ctypes declarations of the C ABI in include/fem2d.h (libfem2d_amd.so)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfem2d_amd.so")
+
+# status codes (enum fem2d_status)
+OK = 0
+ERR_WRONG_CC = 1
+ERR_EMPTY_DOFS = 2
+ERR_INVALID_GLQ = 3
+ERR_HREF = 10
+ERR_PREF = 11
+ERR_MESH_ACCESS = 12
+ERR_PANIC = 13
+ERR_IO = 14
+ERR_INVALID_ARG = 100
+ERR_INVALID_VIEW = 101
+ERR_UNSUPPORTED = 102
+ERR_HIP = 110
+ERR_OOM = 111
+
+
+class DomainView(C.Structure):
+    _fields_ = [
+        ("cc", C.c_int32),
+        ("n_elems", C.c_int32),
+        ("n_dofs", C.c_int64),
+        ("parent", C.POINTER(C.c_int32)),
+        ("loc", C.POINTER(C.c_uint8)),
+        ("element_box", C.POINTER(C.c_double)),
+        ("materials", C.POINTER(C.c_double)),
+        ("orders", C.POINTER(C.c_uint8)),
+        ("bs_offsets", C.POINTER(C.c_int64)),
+        ("bs_dir", C.POINTER(C.c_uint8)),
+        ("bs_i", C.POINTER(C.c_uint8)),
+        ("bs_j", C.POINTER(C.c_uint8)),
+        ("bs_dof", C.POINTER(C.c_int32)),
+    ]
+
+
+class GepCsr(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("nnz", C.c_int64),
+        ("row_ptr", C.POINTER(C.c_int64)),
+        ("col", C.POINTER(C.c_int32)),
+        ("a_val", C.POINTER(C.c_double)),
+        ("b_val", C.POINTER(C.c_double)),
+    ]
+
+
+class GepDevice(C.Structure):
+    _fields_ = GepCsr._fields_
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double),
+        ("prep_ms", C.c_double),
+        ("symbolic_ms", C.c_double),
+        ("fill_ms", C.c_double),
+        ("n_groups", C.c_int64),
+    ]
+
+
+class ElemInfo(C.Structure):
+    _fields_ = [
+        ("id", C.c_int32),
+        ("nodes", C.c_int32 * 4),
+        ("edges", C.c_int32 * 4),
+        ("element", C.c_int32),
+        ("parent", C.c_int32),
+        ("loc", C.c_int32),
+        ("h_levels", C.c_int32 * 2),
+        ("orders", C.c_int32 * 2),
+        ("n_children", C.c_int32),
+        ("children", C.c_int32 * 4),
+        ("depth", C.c_int32),
+    ]
+
+
+class EdgeInfo(C.Structure):
+    _fields_ = [
+        ("id", C.c_int32),
+        ("nodes", C.c_int32 * 2),
+        ("boundary", C.c_int32),
+        ("dir", C.c_int32),
+        ("length", C.c_double),
+        ("parent", C.c_int32),
+        ("children", C.c_int32 * 2),
+        ("child_node", C.c_int32),
+        ("active", C.c_int32 * 2),
+    ]
+
+
+P = C.POINTER
+vp = C.c_void_p
+i32, i64, dbl = C.c_int32, C.c_int64, C.c_double
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "fem2d_ctx_create": (C.c_int, [C.c_int, P(vp)]),
+    "fem2d_ctx_destroy": (None, [vp]),
+    "fem2d_ctx_stream": (vp, [vp]),
+    "fem2d_galerkin_sample_gep_hcurl": (
+        C.c_int, [vp, P(DomainView), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepCsr)]),
+    "fem2d_gep_csr_free": (None, [P(GepCsr)]),
+    "fem2d_device_domain_upload": (C.c_int, [vp, P(DomainView), P(vp)]),
+    "fem2d_device_domain_free": (None, [vp]),
+    "fem2d_assemble_device": (
+        C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepDevice)]),
+    "fem2d_gep_device_download": (C.c_int, [vp, P(GepDevice), P(GepCsr)]),
+    "fem2d_ctx_last_timing": (C.c_int, [vp, P(Timing)]),
+    "fem2d_last_error": (C.c_char_p, []),
+    "fem2d_last_error_detail": (C.c_int, []),
+    "fem2d_mesh_from_file": (C.c_int, [C.c_char_p, P(vp)]),
+    "fem2d_mesh_from_json": (C.c_int, [C.c_char_p, P(vp)]),
+    "fem2d_mesh_from_parts": (C.c_int, [i64, P(dbl), i64, P(i32), P(dbl), P(vp)]),
+    "fem2d_mesh_unit": (C.c_int, [P(vp)]),
+    "fem2d_mesh_blank": (C.c_int, [P(vp)]),
+    "fem2d_mesh_clone": (C.c_int, [vp, P(vp)]),
+    "fem2d_mesh_free": (None, [vp]),
+    "fem2d_mesh_counts": (C.c_int, [vp, P(i64), P(i64), P(i64), P(i64)]),
+    "fem2d_mesh_elem_info": (C.c_int, [vp, i32, P(ElemInfo)]),
+    "fem2d_mesh_edge_info": (C.c_int, [vp, i32, P(EdgeInfo)]),
+    "fem2d_mesh_node_info": (C.c_int, [vp, i32, P(dbl), P(dbl), P(i32)]),
+    "fem2d_mesh_element_info": (C.c_int, [vp, i32, P(dbl), P(dbl)]),
+    "fem2d_mesh_descendant_elems": (C.c_int, [vp, i32, C.c_int, P(i32), i64, P(i64)]),
+    "fem2d_mesh_ancestor_elems": (C.c_int, [vp, i32, C.c_int, P(i32), i64, P(i64)]),
+    "fem2d_mesh_descendant_edges": (C.c_int, [vp, i32, C.c_int, P(i32), i64, P(i64)]),
+    "fem2d_mesh_max_expansion_orders": (C.c_int, [vp, P(i32)]),
+    "fem2d_mesh_elem_is_h_refineable": (C.c_int, [vp, i32, P(i32)]),
+    "fem2d_mesh_elem_p_refinement_window": (C.c_int, [vp, i32, P(i32)]),
+    "fem2d_mesh_elem_parametric_range": (C.c_int, [vp, i32, i32, P(dbl)]),
+    "fem2d_mesh_global_h_refinement": (C.c_int, [vp, i32, i32]),
+    "fem2d_mesh_h_refine_elems": (C.c_int, [vp, P(i32), i64, i32, i32]),
+    "fem2d_mesh_execute_h_refinements": (C.c_int, [vp, P(i32), P(i32), P(i32), i64]),
+    "fem2d_mesh_global_p_refinement": (C.c_int, [vp, i32, i32]),
+    "fem2d_mesh_p_refine_elems": (C.c_int, [vp, P(i32), i64, i32, i32]),
+    "fem2d_mesh_execute_p_refinements": (C.c_int, [vp, P(i32), P(i32), P(i32), i64]),
+    "fem2d_mesh_set_global_expansion_orders": (C.c_int, [vp, i32, i32]),
+    "fem2d_mesh_set_expansion_orders": (C.c_int, [vp, P(i32), P(i32), P(i32), i64]),
+    "fem2d_domain_from_mesh": (C.c_int, [vp, i32, P(vp)]),
+    "fem2d_domain_blank": (C.c_int, [i32, P(vp)]),
+    "fem2d_domain_free": (None, [vp]),
+    "fem2d_domain_num_dofs": (C.c_int, [vp, P(i64)]),
+    "fem2d_domain_mesh": (C.c_int, [vp, P(vp)]),
+    "fem2d_domain_get_view": (C.c_int, [vp, P(DomainView)]),
+    "fem2d_domain_dof_info": (C.c_int, [vp, i64, P(i32), P(i32), P(i32)]),
+    "fem2d_gauss_legendre": (C.c_int, [i32, P(dbl), P(dbl)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libfem2d_amd.so; raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m fem_2d_amd.build_native` "
+            "(the assembly has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().fem2d_last_error()
+    return msg.decode() if msg else ""
+
+
+def last_error_detail() -> int:
+    return int(load().fem2d_last_error_detail())

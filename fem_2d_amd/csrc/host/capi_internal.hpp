@@ -1,0 +1,17 @@
+// Helpers shared by the host and device halves of the C ABI.
+#pragma once
+
+#include <string>
+
+namespace fem2d {
+
+// Gauss-Legendre nodes (ascending) and weights on (-1, 1).
+void gauss_legendre_nodes(int n, double* x, double* w);
+
+namespace capi {
+void set_error(const std::string& msg, int detail);
+// Maps the in-flight exception to a status code and records its message.
+int translate_exception();
+}  // namespace capi
+
+}  // namespace fem2d

@@ -1,0 +1,236 @@
+"""Galerkin GEP assembly on the GPU: the drop-in for
+``galerkin_sample_gep_hcurl::<BSpace, AI, BI>(&domain, glq_grid_dim)``
+(reference src/fem_problem/galerkin.rs:33-187).
+
+The assembly always runs in libfem2d_amd.so's HIP kernels; there is no CPU
+fallback. Without a usable GPU, :class:`Context` raises NativeError."""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .domain import Domain
+from .errors import check
+
+MIN_GLQ_ORDER = 4  # galerkin.rs:13
+
+
+# -- generic arguments of galerkin_sample_gep_hcurl ------------------------
+class HierCurlBasisFnSpace:
+    """Marker base of the built-in shape-function spaces (basis.rs:19-44)."""
+
+    code = -1
+
+
+ShapeFn = HierCurlBasisFnSpace
+
+
+class HierPoly(HierCurlBasisFnSpace):
+    """Monomial N and "bubble" T functions (hierarchical_basis_fns.rs:15-198)."""
+
+    code = 0
+
+
+class KOLShapeFn(HierCurlBasisFnSpace):
+    """Legendre-based maximally-orthogonal space (hierarchical_basis_fns.rs:201-647), with the
+    order-10 coefficient row corrected so that T_n(+-1) = 0 for every n >= 2."""
+
+    code = 1
+
+
+class HierCurlIntegral:
+    """Marker base of the built-in integrals (integration.rs:59-90)."""
+
+    code = -1
+
+
+class CurlCurl(HierCurlIntegral):
+    """<curl u, curl v> / mu_r (integrals.rs:13-92)"""
+
+    code = 0
+
+
+class L2Inner(HierCurlIntegral):
+    """eps_r <u, v> (integrals.rs:279-354)"""
+
+    code = 1
+
+
+class Context:
+    """A GPU context (HIP stream, work buffers, timers) on one device."""
+
+    def __init__(self, device: int = 0):
+        self._lib = N.load()
+        h = C.c_void_p()
+        check(self._lib.fem2d_ctx_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.fem2d_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return int(self._lib.fem2d_ctx_stream(self._h) or 0)
+
+    def last_timing(self) -> dict:
+        t = N.Timing()
+        check(self._lib.fem2d_ctx_last_timing(self._h, C.byref(t)))
+        return dict(total_ms=t.total_ms, prep_ms=t.prep_ms, symbolic_ms=t.symbolic_ms, fill_ms=t.fill_ms,
+                    n_groups=t.n_groups)
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class GEP:
+    """Generalized eigenproblem A u = lambda B u (linalg.rs:29-57) as upper-triangular CSR.
+
+    ``row_ptr``, ``col``, ``a_val``, ``b_val`` hold the stored upper triangle (diagonal
+    included, explicit zeros kept); A and B share the pattern."""
+
+    def __init__(self, n: int, row_ptr, col, a_val, b_val):
+        self.n = int(n)
+        self.row_ptr = row_ptr
+        self.col = col
+        self.a_val = a_val
+        self.b_val = b_val
+
+    @classmethod
+    def _from_csr(cls, g: N.GepCsr) -> "GEP":
+        n, nnz = g.n, g.nnz
+
+        def take(ptr, count, dt):
+            if count == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dt, copy=True)
+
+        return cls(n, take(g.row_ptr, n + 1, np.int64), take(g.col, nnz, np.int32), take(g.a_val, nnz, np.float64),
+                   take(g.b_val, nnz, np.float64))
+
+    @property
+    def nnz(self) -> int:
+        """Stored upper-triangular entries (len of the reference's BTreeMap)."""
+        return int(self.col.size)
+
+    def rows(self) -> np.ndarray:
+        return np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.row_ptr))
+
+    def num_entries(self) -> int:
+        """SparseMatrix::num_entries (sparse_matrix.rs:32-35): full symmetric count."""
+        return 2 * self.nnz - int(np.count_nonzero(self.rows() == self.col))
+
+    def iter_upper_tri(self, which: str = "a"):
+        vals = self.a_val if which == "a" else self.b_val
+        rows = self.rows()
+        for r, c, v in zip(rows, self.col, vals):
+            yield (int(r), int(c)), float(v)
+
+    def to_dense(self):
+        """Full symmetric dense (A, B) (sparse_matrix.rs:168-182)."""
+        A = np.zeros((self.n, self.n))
+        B = np.zeros((self.n, self.n))
+        rows = self.rows()
+        A[rows, self.col] = self.a_val
+        A[self.col, rows] = self.a_val
+        B[rows, self.col] = self.b_val
+        B[self.col, rows] = self.b_val
+        return A, B
+
+    def to_scipy(self):
+        """Full symmetric scipy.sparse CSR matrices (A, B)."""
+        import scipy.sparse as sp
+
+        rows = self.rows()
+        off = rows != self.col
+        r = np.concatenate([rows, self.col[off]])
+        c = np.concatenate([self.col, rows[off]])
+        A = sp.csr_matrix((np.concatenate([self.a_val, self.a_val[off]]), (r, c)), shape=(self.n, self.n))
+        B = sp.csr_matrix((np.concatenate([self.b_val, self.b_val[off]]), (r, c)), shape=(self.n, self.n))
+        return A, B
+
+
+def _glq(glq_grid_dim: Optional[Sequence[int]]):
+    if glq_grid_dim is None:
+        return 0, 0
+    u, v = glq_grid_dim
+    return int(u), int(v)
+
+
+def galerkin_sample_gep_hcurl(domain: Domain, glq_grid_dim: Optional[Sequence[int]] = None,
+                              BSpace=HierPoly, AI=CurlCurl, BI=L2Inner, ctx: Optional[Context] = None) -> GEP:
+    """Fill A (``AI``) and B (``BI``) over the Domain's basis space on the GPU (galerkin.rs:33-187).
+
+    Raises WrongContinuityCondition, EmptyDOFSet or InvalidGLQSettings in the reference's order."""
+    ctx = ctx or default_context()
+    gu, gv = _glq(glq_grid_dim)
+    out = N.GepCsr()
+    lib = N.load()
+    check(lib.fem2d_galerkin_sample_gep_hcurl(ctx._h, C.byref(domain.view()), BSpace.code, AI.code, BI.code, gu, gv,
+                                              0, C.byref(out)))
+    try:
+        return GEP._from_csr(out)
+    finally:
+        lib.fem2d_gep_csr_free(C.byref(out))
+
+
+class DeviceDomain:
+    """A Domain uploaded to HBM once, for repeated device-resident assembly."""
+
+    def __init__(self, domain: Domain, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self._lib = N.load()
+        h = C.c_void_p()
+        check(self._lib.fem2d_device_domain_upload(self.ctx._h, C.byref(domain.view()), C.byref(h)))
+        self._h = h
+        self.n_dofs = domain.num_dofs
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.fem2d_device_domain_free(self._h)
+        except Exception:
+            pass
+
+    def assemble(self, glq_grid_dim=None, BSpace=HierPoly, AI=CurlCurl, BI=L2Inner) -> N.GepDevice:
+        """Assemble into context-owned device buffers (valid until the next assembly)."""
+        gu, gv = _glq(glq_grid_dim)
+        out = N.GepDevice()
+        check(self._lib.fem2d_assemble_device(self.ctx._h, self._h, BSpace.code, AI.code, BI.code, gu, gv, 0,
+                                              C.byref(out)))
+        return out
+
+    def download(self, dev: N.GepDevice) -> GEP:
+        out = N.GepCsr()
+        check(self._lib.fem2d_gep_device_download(self.ctx._h, C.byref(dev), C.byref(out)))
+        try:
+            return GEP._from_csr(out)
+        finally:
+            self._lib.fem2d_gep_csr_free(C.byref(out))
+
+
+def gauss_quadrature_points(n: int):
+    """Gauss-Legendre nodes (ascending) and weights on (-1, 1) used by the library."""
+    x = (C.c_double * n)()
+    w = (C.c_double * n)()
+    check(N.load().fem2d_gauss_legendre(int(n), x, w))
+    return np.array(x[:]), np.array(w[:])

@@ -1,0 +1,277 @@
+/*
+ * fem2d.h - C ABI of the MI355X Galerkin GEP assembly library (libfem2d_amd.so).
+ *
+ * The drop-in boundary is fem2d_galerkin_sample_gep_hcurl(), which replaces
+ *   galerkin_sample_gep_hcurl::<BSpace, AI, BI>(&domain, glq_grid_dim)
+ * of the reference (src/fem_problem/galerkin.rs:33-187). The reference's
+ * Domain crosses the boundary as a flattened, caller-owned fem2d_domain_view
+ * (plain arrays, no ownership transfer); the GEP comes back as upper-triangular
+ * CSR holding A and B on one shared sparsity pattern.
+ *
+ * The remaining entry points are the host-side input builder (Mesh and Domain,
+ * reference src/fem_domain/domain.rs and domain/mesh.rs), the device-resident
+ * variant of the assembly used by callers that keep the Domain in HBM across
+ * calls, and error/timing queries.
+ *
+ * All functions return FEM2D_OK (0) or a status code; they never abort the
+ * process. fem2d_last_error() returns the message of the calling thread's last
+ * failure.
+ */
+#ifndef FEM2D_H
+#define FEM2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Status codes
+ * ------------------------------------------------------------------------- */
+enum fem2d_status {
+  FEM2D_OK = 0,
+  /* GalerkinSamplingError (galerkin.rs:191-195), checked before any work */
+  FEM2D_ERR_WRONG_CC = 1,    /* WrongContinuityCondition(HCurl, got) */
+  FEM2D_ERR_EMPTY_DOFS = 2,  /* EmptyDOFSet */
+  FEM2D_ERR_INVALID_GLQ = 3, /* InvalidGLQSettings: fewer than FEM2D_MIN_GLQ_ORDER points */
+  /* input-builder errors; fem2d_last_error_detail() gives the variant */
+  FEM2D_ERR_HREF = 10,        /* HRefError (h_refinement.rs:284-296) */
+  FEM2D_ERR_PREF = 11,        /* PRefError (p_refinement.rs:305-317) */
+  FEM2D_ERR_MESH_ACCESS = 12, /* MeshAccessError (mesh.rs:1799-1804) */
+  FEM2D_ERR_PANIC = 13,       /* a condition on which the reference panics */
+  FEM2D_ERR_IO = 14,          /* mesh file missing or unreadable */
+  /* library errors */
+  FEM2D_ERR_INVALID_ARG = 100,
+  FEM2D_ERR_INVALID_VIEW = 101, /* domain view violates the Domain invariants */
+  FEM2D_ERR_UNSUPPORTED = 102,  /* a size or option beyond this build's limits */
+  FEM2D_ERR_HIP = 110,          /* HIP runtime failure (no device, launch failure, ...) */
+  FEM2D_ERR_OOM = 111
+};
+
+/* galerkin.rs:13 */
+#define FEM2D_MIN_GLQ_ORDER 4
+
+/* ContinuityCondition (domain.rs:18-23) */
+enum fem2d_cc { FEM2D_CC_HCURL = 0, FEM2D_CC_HDIV = 1, FEM2D_CC_DISCONTINUOUS = 2 };
+
+/* HierCurlBasisFnSpace implementations: HierPoly (hierarchical_basis_fns.rs:15-198)
+ * and KOLShapeFn, the Legendre-based maximally-orthogonal space modelled on
+ * HierMaxOrtho (hierarchical_basis_fns.rs:201-647) with the order-10 coefficient
+ * row corrected so every T_n (n >= 2) vanishes at both endpoints. */
+enum fem2d_basis { FEM2D_BASIS_HIERPOLY = 0, FEM2D_BASIS_KOL = 1 };
+
+/* HierCurlIntegral implementations (integrals.rs): CurlCurl and L2Inner */
+enum fem2d_integral { FEM2D_INTEGRAL_CURLCURL = 0, FEM2D_INTEGRAL_L2INNER = 1 };
+
+/* Arithmetic of the assembly. Only FP64 is provided: the separable (sum-factorised)
+ * contraction leaves the FP64 path bound by HBM writes of the FP64 output. */
+enum fem2d_precision { FEM2D_PRECISION_FP64 = 0 };
+
+/* BasisDir (basis_spec.rs:259-268) */
+enum fem2d_basis_dir { FEM2D_DIR_U = 0, FEM2D_DIR_V = 1, FEM2D_DIR_W = 2 };
+
+/* HRefLoc (h_refinement.rs:210-229) */
+enum fem2d_href_loc {
+  FEM2D_LOC_SW = 0, FEM2D_LOC_SE = 1, FEM2D_LOC_NW = 2, FEM2D_LOC_NE = 3,
+  FEM2D_LOC_W = 4, FEM2D_LOC_E = 5, FEM2D_LOC_S = 6, FEM2D_LOC_N = 7
+};
+
+/* HRef variants (h_refinement.rs:60-77): kind T/U/V plus the optional index of
+ * the child that is refined again in the other direction (-1 for none). */
+enum fem2d_href_kind { FEM2D_HREF_T = 0, FEM2D_HREF_U = 1, FEM2D_HREF_V = 2 };
+
+/* ---------------------------------------------------------------------------
+ * Drop-in boundary: flattened Domain in, upper-triangular CSR GEP out
+ * ------------------------------------------------------------------------- */
+
+/* Flattened view of a reference Domain (domain.rs:42-50). Arrays are owned by
+ * the caller and only read during the call. Elem ids are array indices; the
+ * children of an Elem must have consecutive ids, as every h-refinement of the
+ * reference assigns them (h_refinement.rs:119-134). */
+typedef struct fem2d_domain_view {
+  int32_t cc;        /* Domain::cc */
+  int32_t n_elems;   /* mesh.elems.len() */
+  int64_t n_dofs;    /* dofs.len() */
+  const int32_t* parent;      /* [n_elems] Elem::parent_id(), -1 on the base layer */
+  const uint8_t* loc;         /* [n_elems] HRefLoc of the Elem inside its parent */
+  const double* element_box;  /* [4*n_elems] x0, y0, x3, y3: points[0] and points[3] of the Elem's Element */
+  const double* materials;    /* [2*n_elems] eps_rel.re, mu_rel.re of the Elem's Element */
+  const uint8_t* orders;      /* [2*n_elems] poly_orders ni, nj */
+  const int64_t* bs_offsets;  /* [n_elems+1] basis_specs[e] occupies [bs_offsets[e], bs_offsets[e+1]) */
+  const uint8_t* bs_dir;      /* [n_bs] BasisSpec::dir */
+  const uint8_t* bs_i;        /* [n_bs] BasisSpec::i */
+  const uint8_t* bs_j;        /* [n_bs] BasisSpec::j */
+  const int32_t* bs_dof;      /* [n_bs] BasisSpec::dof_id */
+} fem2d_domain_view;
+
+/* GEP { a, b } (linalg.rs:29-34) as upper-triangular CSR: one pattern shared by
+ * A and B, columns ascending within a row and >= the row index, explicit zeros
+ * kept. nnz counts the stored upper-triangular entries (the reference's
+ * SparseMatrix::num_entries() is 2*nnz - number of diagonal entries). */
+typedef struct fem2d_gep_csr {
+  int64_t n;
+  int64_t nnz;
+  int64_t* row_ptr; /* [n+1] */
+  int32_t* col;     /* [nnz] */
+  double* a_val;    /* [nnz] */
+  double* b_val;    /* [nnz] */
+} fem2d_gep_csr;
+
+/* GPU context: owns a HIP stream, device work buffers and per-stage timers.
+ * One context per host thread; calls on one context are not re-entrant. */
+typedef struct fem2d_ctx fem2d_ctx;
+
+int fem2d_ctx_create(int device, fem2d_ctx** out);
+void fem2d_ctx_destroy(fem2d_ctx* ctx);
+/* hipStream_t the context launches on */
+void* fem2d_ctx_stream(fem2d_ctx* ctx);
+
+/* Replaces galerkin_sample_gep_hcurl (galerkin.rs:33-187).
+ * basis: fem2d_basis; a_integral / b_integral: fem2d_integral (the AI / BI
+ * generic arguments); glq_u, glq_v: Gauss-Legendre points per direction, both 0
+ * for the default of 4*max_order rounded up to a power of two (basis.rs:172-177);
+ * precision: fem2d_precision. Errors are checked in the reference's order:
+ * continuity condition, empty DoF set, GLQ settings. On success *out holds
+ * library-allocated host arrays; release them with fem2d_gep_csr_free. */
+int fem2d_galerkin_sample_gep_hcurl(fem2d_ctx* ctx, const fem2d_domain_view* domain, int basis, int a_integral,
+                                    int b_integral, int glq_u, int glq_v, int precision, fem2d_gep_csr* out);
+void fem2d_gep_csr_free(fem2d_gep_csr* gep);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident assembly (same computation, Domain and GEP stay in HBM)
+ * ------------------------------------------------------------------------- */
+
+typedef struct fem2d_device_domain fem2d_device_domain;
+
+/* Device pointers owned by the context; valid until the next assembly on that
+ * context or its destruction. */
+typedef struct fem2d_gep_device {
+  int64_t n;
+  int64_t nnz;
+  int64_t* row_ptr;
+  int32_t* col;
+  double* a_val;
+  double* b_val;
+} fem2d_gep_device;
+
+int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* domain, fem2d_device_domain** out);
+void fem2d_device_domain_free(fem2d_device_domain* dd);
+int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int a_integral, int b_integral,
+                          int glq_u, int glq_v, int precision, fem2d_gep_device* out);
+int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d_gep_csr* out);
+
+/* Per-stage device time of the last assembly on this context, measured with
+ * HIP events recorded on the context's stream (milliseconds). */
+typedef struct fem2d_timing {
+  double total_ms;    /* first to last event of the assembly */
+  double prep_ms;     /* elem geometry, DoF signatures, row groups, Gram tables */
+  double symbolic_ms; /* per-group column sets and row lengths, row_ptr scan */
+  double fill_ms;     /* column, A and B values of every stored entry */
+  int64_t n_groups;
+} fem2d_timing;
+int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out);
+
+/* ---------------------------------------------------------------------------
+ * Errors
+ * ------------------------------------------------------------------------- */
+const char* fem2d_last_error(void);
+/* variant of the last HRefError / PRefError (1-based, in the reference's
+ * declaration order), 0 otherwise */
+int fem2d_last_error_detail(void);
+
+/* ---------------------------------------------------------------------------
+ * Host input builder: Mesh (mesh.rs) and Domain (domain.rs)
+ * ------------------------------------------------------------------------- */
+
+typedef struct fem2d_mesh fem2d_mesh;
+typedef struct fem2d_domain fem2d_domain;
+
+typedef struct fem2d_elem_info {
+  int32_t id;
+  int32_t nodes[4];
+  int32_t edges[4];
+  int32_t element;
+  int32_t parent;     /* -1 on the base layer */
+  int32_t loc;        /* HRefLoc inside the parent, -1 on the base layer */
+  int32_t h_levels[2];
+  int32_t orders[2];
+  int32_t n_children;
+  int32_t children[4];
+  int32_t depth;      /* number of ancestors */
+} fem2d_elem_info;
+
+typedef struct fem2d_edge_info {
+  int32_t id;
+  int32_t nodes[2];
+  int32_t boundary;
+  int32_t dir;        /* ParaDir: 0 U, 1 V */
+  double length;
+  int32_t parent;
+  int32_t children[2]; /* -1 when not refined */
+  int32_t child_node;
+  int32_t active[2];   /* active Elem pair, -1 when none */
+} fem2d_edge_info;
+
+int fem2d_mesh_from_file(const char* path, fem2d_mesh** out);       /* Mesh::from_file (mesh.rs:142) */
+int fem2d_mesh_from_json(const char* json_text, fem2d_mesh** out);  /* same parser on a string */
+/* Mesh from node coordinates ([2*n_nodes]) and element node ids ([4*n_elements],
+ * order SW, SE, NW, NE) with materials ([4*n_elements]: eps re/im, mu re/im);
+ * identical to reading the equivalent JSON file. */
+int fem2d_mesh_from_parts(int64_t n_nodes, const double* xy, int64_t n_elements, const int32_t* node_ids,
+                          const double* materials, fem2d_mesh** out);
+int fem2d_mesh_unit(fem2d_mesh** out);   /* Mesh::unit (mesh.rs:59) */
+int fem2d_mesh_blank(fem2d_mesh** out);  /* Mesh::blank (mesh.rs:91) */
+int fem2d_mesh_clone(const fem2d_mesh* m, fem2d_mesh** out);
+void fem2d_mesh_free(fem2d_mesh* m);
+
+int fem2d_mesh_counts(const fem2d_mesh* m, int64_t* n_elems, int64_t* n_nodes, int64_t* n_edges,
+                      int64_t* n_elements);
+int fem2d_mesh_elem_info(const fem2d_mesh* m, int32_t elem_id, fem2d_elem_info* out);
+int fem2d_mesh_edge_info(const fem2d_mesh* m, int32_t edge_id, fem2d_edge_info* out);
+int fem2d_mesh_node_info(const fem2d_mesh* m, int32_t node_id, double* x, double* y, int32_t* boundary);
+int fem2d_mesh_element_info(const fem2d_mesh* m, int32_t element_id, double* points8, double* materials4);
+/* id lists: writes at most cap ids to out, *n receives the full count */
+int fem2d_mesh_descendant_elems(const fem2d_mesh* m, int32_t elem_id, int include_start, int32_t* out, int64_t cap,
+                                int64_t* n);
+int fem2d_mesh_ancestor_elems(const fem2d_mesh* m, int32_t elem_id, int include_start, int32_t* out, int64_t cap,
+                              int64_t* n);
+int fem2d_mesh_descendant_edges(const fem2d_mesh* m, int32_t edge_id, int include_start, int32_t* out, int64_t cap,
+                                int64_t* n);
+int fem2d_mesh_max_expansion_orders(const fem2d_mesh* m, int32_t* out2);
+int fem2d_mesh_elem_is_h_refineable(const fem2d_mesh* m, int32_t elem_id, int32_t* out);
+int fem2d_mesh_elem_p_refinement_window(const fem2d_mesh* m, int32_t elem_id, int32_t* out4);
+int fem2d_mesh_elem_parametric_range(const fem2d_mesh* m, int32_t elem_id, int32_t from_ancestor, double* out4);
+
+int fem2d_mesh_global_h_refinement(fem2d_mesh* m, int32_t kind, int32_t ext);
+int fem2d_mesh_h_refine_elems(fem2d_mesh* m, const int32_t* ids, int64_t n, int32_t kind, int32_t ext);
+int fem2d_mesh_execute_h_refinements(fem2d_mesh* m, const int32_t* ids, const int32_t* kinds, const int32_t* exts,
+                                     int64_t n);
+int fem2d_mesh_global_p_refinement(fem2d_mesh* m, int32_t di, int32_t dj);
+int fem2d_mesh_p_refine_elems(fem2d_mesh* m, const int32_t* ids, int64_t n, int32_t di, int32_t dj);
+int fem2d_mesh_execute_p_refinements(fem2d_mesh* m, const int32_t* ids, const int32_t* dis, const int32_t* djs,
+                                     int64_t n);
+int fem2d_mesh_set_global_expansion_orders(fem2d_mesh* m, int32_t i, int32_t j);
+int fem2d_mesh_set_expansion_orders(fem2d_mesh* m, const int32_t* ids, const int32_t* is, const int32_t* js,
+                                    int64_t n);
+
+/* Domain::from_mesh (domain.rs:69-159); the mesh is copied */
+int fem2d_domain_from_mesh(const fem2d_mesh* m, int32_t cc, fem2d_domain** out);
+int fem2d_domain_blank(int32_t cc, fem2d_domain** out);
+void fem2d_domain_free(fem2d_domain* d);
+int fem2d_domain_num_dofs(const fem2d_domain* d, int64_t* n);
+/* borrowed pointer to the Domain's mesh */
+int fem2d_domain_mesh(const fem2d_domain* d, const fem2d_mesh** out);
+/* flattened view; arrays stay valid while the Domain lives */
+int fem2d_domain_get_view(const fem2d_domain* d, fem2d_domain_view* out);
+/* DoF -> (elem id, index in that elem's basis_specs) of its 1, 2 or 4 BasisSpecs */
+int fem2d_domain_dof_info(const fem2d_domain* d, int64_t dof, int32_t* n_bs, int32_t* elem_ids4, int32_t* elem_idx4);
+
+/* Gauss-Legendre nodes (ascending) and weights on (-1, 1), n >= 1 */
+int fem2d_gauss_legendre(int32_t n, double* points, double* weights);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEM2D_H */

@@ -1,0 +1,110 @@
+"""Small domains that exercise every branch of the assembly: uniform and mixed orders,
+T / U / V and extended refinements, n-irregular trees, materials, both bases."""
+
+from __future__ import annotations
+
+import fem_2d_amd as F
+from fem_2d_amd import workloads as W
+
+from conftest import golden
+
+
+def nalg_problem():
+    """lib.rs:45-80: mesh_a, orders (3,3), 8x8 Gauss."""
+    m = F.Mesh.from_file(golden("test_mesh_a.json"))
+    m.global_p_refinement(F.PRef(2, 2))
+    return F.Domain.from_mesh(m), (8, 8)
+
+
+def slepc_problem():
+    """lib.rs:82-119: mesh_b, orders (4,4), global T then T on 6, 9, 12; 8x8 Gauss."""
+    m = F.Mesh.from_file(golden("test_mesh_b.json"))
+    m.global_p_refinement(F.PRef(3, 3))
+    m.global_h_refinement(F.HRef.T)
+    m.h_refine_elems([6, 9, 12], F.HRef.T)
+    return F.Domain.from_mesh(m), (8, 8)
+
+
+def readme_example():
+    """README.md example: mesh_a, orders (4,4), global T, U on the leaves at the central node."""
+    m = F.Mesh.from_file(golden("test_mesh_a.json"))
+    m.set_global_expansion_orders(F.Orders(4, 4))
+    m.global_h_refinement(F.HRef.t())
+    centre = m.elem(0).nodes[3]
+    m.h_refine_with_filter(lambda e: F.HRef.u() if centre in e.nodes else None)
+    return F.Domain.from_mesh(m), (8, 8)
+
+
+def mixed_refinements():
+    """mesh.rs:1950-1981 (refined_mesh_to_file): per-elem orders, T + extended U/V, negative p-refinement."""
+    m = F.Mesh.from_file(golden("test_mesh_b.json"))
+    m.execute_p_refinements([(0, F.PRef(3, 3)), (1, F.PRef(2, 3)), (2, F.PRef(3, 2))])
+    m.execute_h_refinements([(0, F.HRef.T), (1, F.HRef.u_extened(0)), (2, F.HRef.v_extened(0))])
+    m.p_refine_elems([3, 4, 5], F.PRef(-1, -1))
+    m.h_refine_elems([6, 14, 12], F.HRef.T)
+    return F.Domain.from_mesh(m), (8, 8)
+
+
+def create_domain_case():
+    """domain.rs:398-412: mesh_a, (5,5), T, T on 4,5, U on 6,7, V on 8,9, p (+2,-1) on 10..13."""
+    m = F.Mesh.from_file(golden("test_mesh_a.json"))
+    m.set_global_expansion_orders(F.Orders.try_new(5, 5))
+    m.global_h_refinement(F.HRef.T)
+    m.h_refine_elems([4, 5], F.HRef.T)
+    m.h_refine_elems([6, 7], F.HRef.u())
+    m.h_refine_elems([8, 9], F.HRef.v())
+    m.p_refine_elems([10, 11, 12, 13], F.PRef(2, -1))
+    return F.Domain.from_mesh(m), (10, 6)
+
+
+def deep_unit():
+    """Unit mesh refined four levels deep in one corner with mixed T/U/V, orders (2,3)."""
+    m = F.Mesh.unit()
+    m.set_global_expansion_orders(F.Orders(2, 3))
+    m.h_refine_elems([0], F.HRef.T)
+    m.h_refine_elems([1], F.HRef.T)
+    m.h_refine_elems([5], F.HRef.u())
+    m.h_refine_elems([9], F.HRef.v())
+    m.h_refine_elems([2], F.HRef.U(1))
+    return F.Domain.from_mesh(m), (5, 7)
+
+
+def proper_edge_order_mesh():
+    """mesh.rs:1917-1948: T, U on boundary leaves, T/V by node 4, global U(Some(1))."""
+    m = F.Mesh.from_file(golden("test_mesh_b.json"))
+    m.global_h_refinement(F.HRef.T)
+    boundary = [e.boundary for e in m.edges]
+    m.h_refine_with_filter(lambda e: F.HRef.u() if any(boundary[k] for k in e.edges) else None)
+    m.h_refine_with_filter(lambda e: F.HRef.T if 4 in e.nodes else F.HRef.v())
+    m.global_h_refinement(F.HRef.U(1))
+    m.set_global_expansion_orders(F.Orders(3, 2))
+    return F.Domain.from_mesh(m), (6, 6)
+
+
+def waveguide_small():
+    w = W.waveguide_small()
+    return w.domain, w.glq
+
+
+def small_hp_irregular():
+    m = W.hp_irregular_mesh(12, 6, levels=3, seed=1)
+    return F.Domain.from_mesh(m), (10, 10)
+
+
+def small_many_waveguides():
+    mesh, inst = W.many_waveguides_mesh(6, seed=2)
+    return F.Domain.from_mesh(mesh), (8, 8)
+
+
+ALL = {
+    "nalg_problem": nalg_problem,
+    "slepc_problem": slepc_problem,
+    "readme_example": readme_example,
+    "mixed_refinements": mixed_refinements,
+    "create_domain": create_domain_case,
+    "deep_unit": deep_unit,
+    "proper_edge_order": proper_edge_order_mesh,
+    "waveguide_small": waveguide_small,
+    "small_hp_irregular": small_hp_irregular,
+    "small_many_waveguides": small_many_waveguides,
+}
