@@ -1,0 +1,204 @@
+"""Benchmark: Galerkin GEP assembly throughput on one MI355X.
+
+Workload (default): the 64x32 waveguide ([0,1]x[0,0.5], 64x32 quads, orders
+(6,6), two levels of T refinement on a seeded random 25 % of the leaves per
+level, HierPoly, 8x8 Gauss points) - 7784 Elems, 445380 DoFs.
+
+A step is one full assembly (galerkin_sample_gep_hcurl semantics) from the
+device-resident flattened Domain to the device-resident upper-triangular CSR
+of A and B: elem geometry, DoF signatures, row groups, Gram tables, row
+lengths, row_ptr scan and the entry fill. `value` is stored upper-triangular
+nonzeros per second (the length of the reference's BTreeMap per second).
+
+Prints one JSON line. `roofline` covers the entry-fill kernel (the dominant
+one): achieved = algorithmic bytes (20 B per stored entry: 4 B column + 8 B A
++ 8 B B) / its HIP-event duration. `cpu_baseline` times the oracle (the C++
+restatement of the reference algorithm) on a bounded sample of the same
+workload on the host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GEP assembly nonzeros/s: mixed-prec vs FP64 GPU at stated rel err; CPU ref"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ENTRY = 20  # col (4) + A (8) + B (8)
+
+
+def build_workload(name: str):
+    from fem_2d_amd import workloads as W
+
+    if name == "waveguide_64x32":
+        return W.waveguide_64x32()
+    if name == "waveguide_small":
+        return W.waveguide_small()
+    if name == "precision_study":
+        return W.precision_study()
+    if name == "hp_irregular":
+        return W.hp_irregular()
+    if name == "many_waveguides":
+        return W.many_waveguides()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(w, total_pairs: int, nnz: int, target_s: float):
+    """Oracle on the first elems of the same Domain (all host threads up to 16);
+    scaled to the full workload through its element-pair count."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n_elems = w.domain.mesh.num_elems
+    limit = 64
+    while True:
+        t0 = time.perf_counter()
+        rc, res = oracle.assemble(w.domain.view(), basis=w.basis.code, glq=w.glq, n_threads=threads,
+                                  elem_limit=limit)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"oracle failed: {rc}")
+        if dt >= target_s or limit >= n_elems:
+            break
+        limit = min(n_elems, max(limit * 2, int(limit * target_s / max(dt, 1e-3) * 1.1)))
+    pairs_per_s = res.pairs / dt
+    est_total_s = total_pairs / pairs_per_s
+    return {
+        "value": nnz / est_total_s,
+        "unit": "nonzeros/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle (C++ restatement of galerkin.rs) over elems [0, {limit}) of {n_elems}: "
+                   f"{res.pairs} element pairs in {dt:.2f} s on {threads} threads; scaled by the workload's "
+                   f"{total_pairs} element pairs"),
+        "pairs_per_s": pairs_per_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="waveguide_64x32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (for scaling HBM byte counters)")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = 1 << 27  # 2^27 doubles = 1 GiB
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}), flush=True)
+        return
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the assembly shards by base Elem "
+              "tree across GPUs by design but no multi-GPU path is built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import fem_2d_amd as F
+
+    w = build_workload(args.workload)
+    ctx = F.Context(0)
+    dd = F.DeviceDomain(w.domain, ctx)
+
+    def step():
+        return dd.assemble(w.glq, w.basis, F.CurlCurl, F.L2Inner)
+
+    for _ in range(args.warmup):
+        out = step()
+    # every assembly synchronises its stream before returning; the timed region is
+    # also bracketed by device-wide synchronisation
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    fill_ms, count_ms, total_ms = [], [], []
+    for _ in range(args.steps):
+        out = step()
+        t = ctx.last_timing()
+        fill_ms.append(t["fill_kernel_ms"])
+        count_ms.append(t["count_kernel_ms"])
+        total_ms.append(t["total_ms"])
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    nnz = int(out.nnz)
+    n = int(out.n)
+    fill_avg = float(np.mean(fill_ms))
+
+    # element pairs visited by the reference's loops (pattern-independent work count)
+    a = w.domain.arrays()
+    nbs = np.diff(a["bs_offsets"])
+    parent = a["parent"]
+    pairs = int(np.sum(nbs * (nbs + 1) // 2))
+    # ancestor-descendant pairs: sum over elems of |own specs| * |specs of every ancestor|
+    anc_specs = np.zeros(len(parent), dtype=np.int64)
+    for e in range(len(parent)):
+        p = parent[e]
+        s = 0
+        while p >= 0:
+            s += nbs[p]
+            p = parent[p]
+        anc_specs[e] = s
+    pairs += int(np.sum(nbs * anc_specs))
+
+    achieved = BYTES_PER_ENTRY * nnz / (fill_avg * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": nnz / (ms_per_step * 1e-3),
+        "unit": "nonzeros/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name,
+            "n_elems": int(a["n_elems"]),
+            "n_dofs": n,
+            "nnz_upper": nnz,
+            "element_pairs": pairs,
+            "glq": list(w.glq),
+            "basis": w.basis.__name__,
+            "precision": "fp64",
+        },
+        "element_pairs_per_s": pairs / (ms_per_step * 1e-3),
+        "device_ms": {"total": float(np.mean(total_ms)), "count_kernel": float(np.mean(count_ms)),
+                      "fill_kernel": fill_avg},
+        "roofline": {
+            "kernel": "k_group_fill",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": BYTES_PER_ENTRY * nnz,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, pairs, nnz, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

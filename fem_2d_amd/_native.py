@@ -64,6 +64,9 @@ class Timing(C.Structure):
         ("symbolic_ms", C.c_double),
         ("fill_ms", C.c_double),
         ("n_groups", C.c_int64),
+        ("count_kernel_ms", C.c_double),
+        ("fill_kernel_ms", C.c_double),
+        ("nnz", C.c_int64),
     ]
 
 
@@ -106,6 +109,7 @@ SIGNATURES = {
     "fem2d_ctx_create": (C.c_int, [C.c_int, P(vp)]),
     "fem2d_ctx_destroy": (None, [vp]),
     "fem2d_ctx_stream": (vp, [vp]),
+    "fem2d_ctx_synchronize": (C.c_int, [vp]),
     "fem2d_galerkin_sample_gep_hcurl": (
         C.c_int, [vp, P(DomainView), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepCsr)]),
     "fem2d_gep_csr_free": (None, [P(GepCsr)]),

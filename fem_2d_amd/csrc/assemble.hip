@@ -835,7 +835,7 @@ struct fem2d_device_domain {
 struct fem2d_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<DevBuf> bufs;  // grow-only work buffers, by slot
   DevBuf scan_tmp;
   DevBuf out_col, out_a, out_b;
@@ -992,6 +992,16 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
 }
 
 void* fem2d_ctx_stream(fem2d_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int fem2d_ctx_synchronize(fem2d_ctx* ctx) {
+  if (!ctx) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipDeviceSynchronize());
+  return FEM2D_OK;
+}
 
 int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out) {
   if (!ctx || !out) {
@@ -1188,8 +1198,10 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_group_fill),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+  HIP_TRY(hipEventRecord(ctx->ev[4], st));
   k_group_count<<<ggrid, kBlock, shmem, st>>>(v, w, n_groups);
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[5], st));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.row_len, w.row_ptr, int(nd + 1), st));
   HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 4, w.row_ptr + nd, sizeof(long long), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 2, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1203,17 +1215,24 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   if ((rc = ensure(ctx->out_col, size_t(nnz) * sizeof(int))) || (rc = ensure(ctx->out_a, size_t(nnz) * 8)) ||
       (rc = ensure(ctx->out_b, size_t(nnz) * 8)))
     return rc;
+  HIP_TRY(hipEventRecord(ctx->ev[6], st));
   k_group_fill<<<ggrid, kBlock, shmem, st>>>(v, w, n_groups, L, ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0,
                                               bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0,
                                               static_cast<int*>(ctx->out_col.p), static_cast<double*>(ctx->out_a.p),
                                               static_cast<double*>(ctx->out_b.p));
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[7], st));
   HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 2, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipEventRecord(ctx->ev[3], st));
   HIP_TRY(hipStreamSynchronize(st));
   if (ctx->h_pinned[2]) return err_to_status(ctx->h_pinned[2]);
 
-  float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+  float t01 = 0, t12 = 0, t23 = 0, t03 = 0, tc = 0, tf = 0;
+  (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
+  (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
+  ctx->timing.count_kernel_ms = tc;
+  ctx->timing.fill_kernel_ms = tf;
+  ctx->timing.nnz = nnz;
   (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
   (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
   (void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);

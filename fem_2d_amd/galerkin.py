@@ -81,6 +81,10 @@ class Context:
         except Exception:
             pass
 
+    def synchronize(self) -> None:
+        """Wait for all work on the context's device."""
+        check(self._lib.fem2d_ctx_synchronize(self._h))
+
     @property
     def stream(self) -> int:
         return int(self._lib.fem2d_ctx_stream(self._h) or 0)
@@ -89,7 +93,8 @@ class Context:
         t = N.Timing()
         check(self._lib.fem2d_ctx_last_timing(self._h, C.byref(t)))
         return dict(total_ms=t.total_ms, prep_ms=t.prep_ms, symbolic_ms=t.symbolic_ms, fill_ms=t.fill_ms,
-                    n_groups=t.n_groups)
+                    n_groups=t.n_groups, count_kernel_ms=t.count_kernel_ms, fill_kernel_ms=t.fill_kernel_ms,
+                    nnz=t.nnz)
 
 
 _default_ctx: Optional[Context] = None

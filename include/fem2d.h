@@ -126,6 +126,8 @@ int fem2d_ctx_create(int device, fem2d_ctx** out);
 void fem2d_ctx_destroy(fem2d_ctx* ctx);
 /* hipStream_t the context launches on */
 void* fem2d_ctx_stream(fem2d_ctx* ctx);
+/* device-wide synchronisation of the context's device */
+int fem2d_ctx_synchronize(fem2d_ctx* ctx);
 
 /* Replaces galerkin_sample_gep_hcurl (galerkin.rs:33-187).
  * basis: fem2d_basis; a_integral / b_integral: fem2d_integral (the AI / BI
@@ -168,7 +170,10 @@ typedef struct fem2d_timing {
   double prep_ms;     /* elem geometry, DoF signatures, row groups, Gram tables */
   double symbolic_ms; /* per-group column sets and row lengths, row_ptr scan */
   double fill_ms;     /* column, A and B values of every stored entry */
-  int64_t n_groups;
+  int64_t n_groups;   /* row groups (runs of DoFs whose basis specs share their Elems) */
+  double count_kernel_ms; /* the row-length kernel alone */
+  double fill_kernel_ms;  /* the entry-writing kernel alone */
+  int64_t nnz;
 } fem2d_timing;
 int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out);
 

@@ -1,0 +1,109 @@
+"""Pins of the oracle (CPU restatement of galerkin.rs) against the reference's known answers
+and against mathematics:
+
+* the SLEPc eigenpair of lib.rs:82-119 stored in test_eval.dat / test_evec.dat (pins DoF
+  numbering and every A/B value of that 624-DoF problem at once);
+* the nalgebra eigenvalue of lib.rs:45-80 (2.6479657, tolerance 1e-6), which the reference
+  obtains from SymmetricEigen of the non-symmetric B^-1 A reading its lower triangle;
+* exact rational integrals for HierPoly (exact_ref.py) on domains with T / U / V /
+  extended refinements, mixed orders and non-square Elems;
+* the analytic TE cutoff (pi/a)^2 of an air-filled rectangular guide;
+* structural invariants: symmetric keys, identical A/B patterns, B(U, V) = 0, B SPD."""
+
+import struct
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle
+import cases
+from conftest import golden
+from exact_ref import compare_to_exact
+from fem_2d_amd import workloads as W
+
+
+def _golden_eigenpair():
+    lam = struct.unpack(">d", open(golden("test_eval.dat"), "rb").read())[0]
+    raw = open(golden("test_evec.dat"), "rb").read()
+    classid, n = struct.unpack(">ii", raw[:8])
+    assert classid == 1211214  # PETSc Vec
+    v = np.array(struct.unpack(f">{n}d", raw[8:8 + 8 * n]))
+    return lam, v
+
+
+def test_slepc_golden_eigenpair():
+    """lib.rs:82-119 and test_input/test_{eval,evec}.dat."""
+    dom, glq = cases.slepc_problem()
+    lam, v = _golden_eigenpair()
+    assert dom.num_dofs == v.size == 624
+    rc, res = oracle.assemble(dom.view(), glq=glq)
+    assert rc == 0
+    A, B = res.to_dense()
+    resid = np.linalg.norm(A @ v - lam * (B @ v)) / np.linalg.norm(A @ v)
+    assert resid < 1e-10, resid
+    w = sl.eigh(A, B, eigvals_only=True)
+    assert abs(w[np.argmin(np.abs(w - 1.475))] - 1.4745880937) < 1e-9  # lib.rs:104
+
+
+def test_nalgebra_eigenvalue():
+    """lib.rs:45-80: nalgebra_solve_gep (nalgebra_solve.rs:14-50) near 2.64 gives 2.6479657 (tol 1e-6).
+    B^-1 A is not symmetric; nalgebra's SymmetricEigen reads its lower triangle."""
+    dom, glq = cases.nalg_problem()
+    rc, res = oracle.assemble(dom.view(), glq=glq)
+    assert rc == 0
+    A, B = res.to_dense()
+    C = np.linalg.inv(np.linalg.cholesky(B)).T @ np.linalg.inv(np.linalg.cholesky(B)) @ A  # B^-1 A
+    lower = np.tril(C) + np.tril(C, -1).T
+    w = np.linalg.eigvalsh(lower)
+    assert abs(w[np.argmin(np.abs(w - 2.64))] - 2.6479657) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["nalg_problem", "deep_unit", "mixed_refinements", "readme_example"])
+def test_exact_rational_integrals(name):
+    dom, glq = cases.ALL[name]()
+    rc, res = oracle.assemble(dom.view(), glq=glq)
+    assert rc == 0
+    ea, eb = compare_to_exact(res, dom.arrays())
+    assert ea < 1e-13 and eb < 1e-13, (ea, eb)
+
+
+def test_waveguide_te10_cutoff():
+    """Lowest non-gradient eigenvalue of the 1 x 0.5 PEC guide is the TE10 cutoff pi^2."""
+    w = W.waveguide_small()
+    rc, res = oracle.assemble(w.domain.view(), glq=w.glq)
+    assert rc == 0
+    A, B = res.to_dense()
+    ev = sl.eigh(A, B, eigvals_only=True)
+    physical = ev[ev > 1e-3 * np.pi ** 2]
+    te10 = np.pi ** 2
+    assert abs(physical[0] - te10) / te10 < 1e-6, physical[:3]
+    # next modes: TE20 = TE01 = 4 pi^2 (degenerate)
+    assert abs(physical[1] - 4 * te10) / (4 * te10) < 1e-4
+    assert abs(physical[2] - 4 * te10) / (4 * te10) < 1e-4
+
+
+def test_structure_invariants():
+    dom, glq = cases.mixed_refinements()
+    rc, res = oracle.assemble(dom.view(), glq=glq)
+    rows = res.rows()
+    assert np.all(res.col >= rows)
+    a = dom.arrays()
+    dof_dir = np.zeros(dom.num_dofs, dtype=int)
+    dof_dir[a["bs_dof"]] = a["bs_dir"]
+    mixed = dof_dir[rows] != dof_dir[res.col]
+    assert np.all(res.b_val[mixed] == 0.0)  # f_u . f_v = 0 for a diagonal Jacobian
+    A, B = res.to_dense()
+    assert np.all(np.linalg.eigvalsh(B) > 0)
+    assert np.linalg.eigvalsh(A).min() > -1e-9 * np.abs(A).max()
+
+
+def test_galerkin_error_order():
+    """galerkin.rs:41-59 via the oracle: cc, then empty DoFs, then GLQ."""
+    import fem_2d_amd as F
+
+    assert oracle.assemble(F.Domain.blank(F.ContinuityCondition.HDiv).view(), glq=(2, 2))[0] == 1
+    assert oracle.assemble(F.Domain.blank(F.ContinuityCondition.HCurl).view(), glq=(2, 2))[0] == 2
+    dom, _ = cases.nalg_problem()
+    assert oracle.assemble(dom.view(), glq=(3, 8))[0] == 3
+    assert oracle.assemble(dom.view(), glq=None)[0] == 0
