@@ -180,12 +180,13 @@ def main():
             "glq": list(w.glq),
             "basis": w.basis.__name__,
             "precision": "fp64",
+            "large_groups": int(t["n_large"]),
         },
         "element_pairs_per_s": pairs / (ms_per_step * 1e-3),
         "device_ms": {"total": float(np.mean(total_ms)), "count_kernel": float(np.mean(count_ms)),
                       "fill_kernel": fill_avg},
         "roofline": {
-            "kernel": "k_group_fill",
+            "kernel": "k_fill_wave",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -67,6 +67,7 @@ class Timing(C.Structure):
         ("count_kernel_ms", C.c_double),
         ("fill_kernel_ms", C.c_double),
         ("nnz", C.c_int64),
+        ("n_large", C.c_int64),
     ]
 
 

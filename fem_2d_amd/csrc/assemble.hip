@@ -20,24 +20,27 @@
 //    live on the same Elem(s) (the elem-type DoFs of one leaf, or the DoFs of
 //    one edge). Every row of a group has the same candidate column set: the
 //    DoFs on the Elems nested with (equal to, ancestor of or descendant of) the
-//    group's Elem(s). That set is a union of whole row groups, built per group
-//    in LDS (sources -> memberships -> bitonic sort -> dedupe). A row's columns
+//    group's Elem(s). That set is a union of whole row groups. A row's columns
 //    are the suffix of that set starting at the row itself.
 //  * Each stored entry (r, c) receives one contribution per nested pair of
 //    basis specs of r and c. Two pairs of interior-disjoint Elems can have at
 //    most two nested pairs, so an entry has one or two contributions; they are
 //    summed in a fixed order and written once. Kernels check this bound and
 //    report a violation instead of writing wrong values.
+//  * One wavefront owns one row group: it builds the candidate set in its own
+//    slice of LDS with wave-level scans, a bitonic sort and ballots (no block
+//    barriers), then streams the group's rows with consecutive lanes on
+//    consecutive CSR slots. Row groups whose candidate set exceeds the wave
+//    slice go to a block-per-group path with a larger LDS budget.
 //  * Pattern (count pass + scan) and values (fill pass) are computed on the
-//    device in the same call; the fill pass writes col/A/B with consecutive
-//    threads on consecutive CSR slots.
+//    device in the same call. The only host synchronisation is at the end (or
+//    when the output buffers must grow).
 
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -52,11 +55,18 @@ namespace {
 // ---------------------------------------------------------------------------
 // Limits of this build (checked; exceeded limits return FEM2D_ERR_UNSUPPORTED)
 // ---------------------------------------------------------------------------
-constexpr int kBlock = 256;        // threads per row-group block
-constexpr int kMaxSrc = 128;       // nested Elems per row group
-constexpr int kMaxRaw = 1024;      // (Elem, row group) memberships gathered per row group
-constexpr int kMaxDist = 1024;     // distinct row groups in one candidate set
-constexpr int kMaxRows = 1024;     // rows per row group
+constexpr int kWave = 64;
+constexpr int kBlock = 256;        // threads per block (4 wavefronts)
+constexpr int kWavesPerBlock = kBlock / kWave;
+// wave path (one wavefront per row group)
+constexpr int kWS = 32;            // nested Elems per row group
+constexpr int kWR = 128;           // memberships gathered per row group
+constexpr int kWD = 128;           // distinct row groups in one candidate set
+// block path (one block per row group)
+constexpr int kMaxSrc = 128;
+constexpr int kMaxRaw = 1024;
+constexpr int kMaxDist = 1024;
+constexpr int kMaxRows = 1024;
 constexpr int kMaxDepth = 24;      // h-refinement depth below a base Elem
 constexpr int kMaxRelDepth = 12;   // dyadic halvings per axis covered by the Gram library
 constexpr int kMaxOrder = 20;      // MAX_POLYNOMIAL_ORDER (mesh.rs:42)
@@ -68,10 +78,10 @@ enum Rel { kLocal = 0, kAnc = 1, kDesc = 2 };
 
 // device error flags
 enum ErrBits : int {
-  kErrView = 1,        // view violates Domain invariants
-  kErrSize = 2,        // a per-group limit of this build was exceeded
-  kErrContrib = 4,     // an entry with more than two contributions
-  kErrDepth = 8,       // refinement deeper than kMaxDepth / kMaxRelDepth
+  kErrView = 1,     // view violates Domain invariants
+  kErrSize = 2,     // a per-group limit of this build was exceeded
+  kErrContrib = 4,  // an entry with more than two contributions / a DoF with more than two specs
+  kErrDepth = 8,    // refinement deeper than kMaxDepth / kMaxRelDepth
 };
 
 struct ElemGeo {
@@ -106,35 +116,53 @@ struct Work {
   int* max_child;
   int* n_child;
   int4* dyad;  // (du, iu, dv, iv): position of the Elem inside its base Elem
+  long long* slot0;  // per DoF: (elem << 32 | index) of its basis specs
+  long long* slot1;
+  int* bs_elem;       // per basis spec: owning Elem
+  unsigned* bs_pack;  // per basis spec: dir | i << 8 | j << 16
   int* dof_e1;
   int* dof_e2;
   int* dof_k1;
   int* dof_k2;
-  int* dof_cnt;
   int* dof_flag;
   int* dof_grp;  // inclusive scan of dof_flag
   int* g_start;  // [G+1]
   int* g_e1;
   int* g_e2;
-  int* mem_cnt;  // [E+1] membership counts per Elem
-  int* mem_ptr;  // [E+1] exclusive offsets of mem_cnt
+  int* mem_flag;  // [n_bs+1] 1 where a membership (run of one row group) starts
+  int* mem_idx;   // [n_bs+1] exclusive scan of mem_flag
+  int* mem_ptr;   // [E+1]
   int* mem_group;
   int* mem_bsoff;
-  int64_t* row_len;  // [N+1] stored entries per row
-  int64_t* row_ptr;  // [N+1] exclusive offsets of row_len
+  int64_t* row_len;  // [N+1]
+  int64_t* row_ptr;  // [N+1]
   double* lib;
   const double* glq;  // u points, u weights, v points, v weights (kMaxGlq each)
   int* err;
+  int* n_large;       // row groups handled by the block path
+  int* large_list;
+  unsigned char* is_large;
+  long long n_dofs;
 };
 
 struct LibShape {
-  int p1;     // orders 0..p1-1 per axis
-  int nr;     // ranges per axis
-  int nu, nv; // Gauss points per axis
+  int p1;      // orders 0..p1-1 per axis
+  int nr;      // ranges per axis
+  int nu, nv;  // Gauss points per axis
   int basis;
 };
 
+struct FillOut {
+  int* col;
+  double* aval;
+  double* bval;
+  long long capacity;  // entries the output buffers hold
+  int a_is_cc, b_is_cc;
+};
+
 __device__ __forceinline__ void set_err(int* err, int bits) { atomicOr(err, bits); }
+
+__device__ __forceinline__ int n_groups_of(const Work& w) { return w.n_dofs > 0 ? w.dof_grp[w.n_dofs - 1] : 0; }
 
 // ---------------------------------------------------------------------------
 // Shape-function spaces (point evaluation, same recurrences and operation
@@ -235,11 +263,80 @@ __device__ __forceinline__ void range_of(int rid, double& s, double& c) {
   c = lo + s;
 }
 
+__device__ __forceinline__ int rel_range_id(int4 P, int4 Q, int axis) {
+  const int dp = axis == 0 ? P.x : P.z;
+  const int ip = axis == 0 ? P.y : P.w;
+  const int dq = axis == 0 ? Q.x : Q.z;
+  const int iq = axis == 0 ? Q.y : Q.w;
+  const int dd = dq - dp;
+  const int ii = iq - (ip << dd);
+  return (1 << dd) - 1 + ii;
+}
+
+// Block prefactors of one nested pair of Elems: P holds the p-side spec (the
+// ancestor, or the Elem itself), Q the q-side spec (CurlCurl: the curl of p is
+// scaled by Q's para_scale (1,1), the curl of q by P's sub-range scale, the
+// ratio comes from the spec with the larger det_jac; L2Inner: both glq_scales,
+// the larger det_jac; see integrals.rs:26-92, 292-354, basis.rs:225-252).
+__device__ __forceinline__ void block_coefs(const Work& w, int P, int Q, bool local, double* cA, double* cB,
+                                            int& ru, int& rv) {
+  const ElemGeo gp = w.geo[P], gq = w.geo[Q];
+  double su = 1.0, sv = 1.0;
+  ru = 0;
+  rv = 0;
+  if (!local) {
+    const int4 dp = w.dyad[P], dq = w.dyad[Q];
+    ru = rel_range_id(dp, dq, 0);
+    rv = rel_range_id(dp, dq, 1);
+    su = ldexp(1.0, -(dq.x - dp.x));
+    sv = ldexp(1.0, -(dq.z - dp.z));
+  }
+  const bool p_big = gp.det >= gq.det;
+  const double r_uv = p_big ? gp.uv : gq.uv;
+  const double r_vu = p_big ? gp.vu : gq.vu;
+  const double dmax = p_big ? gp.det : gq.det;
+  const double im = gp.inv_mu;
+  cA[0] = im * ((gp.ju0 * 1.0) * (gq.ju0 * su)) * r_uv;
+  cA[1] = -(im * (gp.ju0 * (gq.jv1 * sv)));
+  cA[2] = -(im * (gp.jv1 * (gq.ju0 * su)));
+  cA[3] = im * ((gp.jv1 * 1.0) * (gq.jv1 * sv)) * r_vu;
+  const double gs = su * sv;
+  cB[0] = gp.eps * gs * dmax * gp.ju0 * gq.ju0;
+  cB[1] = gp.eps * gs * dmax * gp.jv1 * gq.jv1;
+}
+
+// CurlCurl and L2Inner contributions of one nested pair (p, q)
+__device__ __forceinline__ void pair_values(const double* __restrict__ lib, const LibShape& L, const double* cA,
+                                            const double* cB, int ru, int rv, unsigned pbs, unsigned qbs, double& cc,
+                                            double& l2) {
+  const int pd = pbs & 0xff, pi = (pbs >> 8) & 0xff, pj = (pbs >> 16) & 0xff;
+  const int qd = qbs & 0xff, qi = (qbs >> 8) & 0xff, qj = (qbs >> 16) & 0xff;
+  const long long tab = (long long)L.p1 * L.p1;
+  const double* gx = lib + (long long)ru * kKinds * tab;
+  const double* gy = lib + ((long long)L.nr + rv) * kKinds * tab;
+  const int ix = pi * L.p1 + qi, iy = pj * L.p1 + qj;
+  cc = 0.0;
+  l2 = 0.0;
+  if (pd == FEM2D_DIR_U && qd == FEM2D_DIR_U) {
+    const double xnn = gx[kNN * tab + ix];
+    cc = cA[0] * xnn * gy[kTdTd * tab + iy];
+    l2 = cB[0] * xnn * gy[kTT * tab + iy];
+  } else if (pd == FEM2D_DIR_U && qd == FEM2D_DIR_V) {
+    cc = cA[1] * gx[kNTd * tab + ix] * gy[kTdN * tab + iy];
+  } else if (pd == FEM2D_DIR_V && qd == FEM2D_DIR_U) {
+    cc = cA[2] * gx[kTdN * tab + ix] * gy[kNTd * tab + iy];
+  } else if (pd == FEM2D_DIR_V && qd == FEM2D_DIR_V) {
+    const double ynn = gy[kNN * tab + iy];
+    cc = cA[3] * gx[kTdTd * tab + ix] * ynn;
+    l2 = cB[1] * gx[kTT * tab + ix] * ynn;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Prep kernels
 // ---------------------------------------------------------------------------
 
-__global__ void k_init(Work w, int n_elems, long long n_dofs) {
+__global__ void k_init(Work w, int n_elems, long long n_dofs, long long n_bs) {
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t < n_elems) {
     w.first_child[t] = 0x7fffffff;
@@ -247,15 +344,13 @@ __global__ void k_init(Work w, int n_elems, long long n_dofs) {
     w.n_child[t] = 0;
   }
   if (t < n_dofs) {
-    w.dof_e1[t] = 0x7fffffff;
-    w.dof_e2[t] = -1;
-    w.dof_k1[t] = -1;
-    w.dof_k2[t] = -1;
-    w.dof_cnt[t] = 0;
+    w.slot0[t] = -1;
+    w.slot1[t] = -1;
   }
   if (t == 0) {
     *w.err = 0;
-    w.mem_cnt[n_elems] = 0;
+    *w.n_large = 0;
+    w.mem_flag[n_bs] = 0;
     w.row_len[n_dofs] = 0;
   }
 }
@@ -294,14 +389,14 @@ __global__ void k_elem_prep(DevView v, Work w) {
   for (int t = depth - 1; t >= 0; --t) {
     const double mu = (u0 + u1) / 2.0, mv = (v0 + v1) / 2.0;
     const int l = locs[t];
-    const bool split_u = l <= FEM2D_LOC_E;  // SW, SE, NW, NE, W, E
-    const bool split_v = l <= FEM2D_LOC_NE || l >= FEM2D_LOC_S;
-    const bool hi_u = l == FEM2D_LOC_SE || l == FEM2D_LOC_NE || l == FEM2D_LOC_E;
-    const bool hi_v = l == FEM2D_LOC_NW || l == FEM2D_LOC_NE || l == FEM2D_LOC_N;
     if (l > FEM2D_LOC_N) {
       set_err(w.err, kErrView);
       return;
     }
+    const bool split_u = l <= FEM2D_LOC_E;  // SW, SE, NW, NE, W, E
+    const bool split_v = l <= FEM2D_LOC_NE || l >= FEM2D_LOC_S;
+    const bool hi_u = l == FEM2D_LOC_SE || l == FEM2D_LOC_NE || l == FEM2D_LOC_E;
+    const bool hi_v = l == FEM2D_LOC_NW || l == FEM2D_LOC_NE || l == FEM2D_LOC_N;
     if (split_u) {
       if (hi_u) u0 = mu; else u1 = mu;
       iu = 2 * iu + (hi_u ? 1 : 0);
@@ -344,113 +439,110 @@ __global__ void k_child_check(DevView v, Work w) {
   if (!(n == 2 || n == 4) || w.max_child[e] - w.first_child[e] + 1 != n) set_err(w.err, kErrView);
 }
 
-// DoF -> the (at most two) Elems holding its basis specs
-__global__ void k_dof_sig(DevView v, Work w) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= v.n_elems) return;
-  for (long long t = v.bs_off[e]; t < v.bs_off[e + 1]; ++t) {
-    const int d = v.bs_dof[t];
-    if (d < 0 || d >= v.n_dofs) {
-      set_err(w.err, kErrView);
-      continue;
-    }
-    atomicMin(&w.dof_e1[d], e);
-    atomicMax(&w.dof_e2[d], e);
-    atomicAdd(&w.dof_cnt[d], 1);
+// Per basis spec: owning Elem, packed (dir, i, j), and the DoF's spec slots.
+__global__ void k_bs_slots(DevView v, Work w) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= v.n_bs) return;
+  int lo = 0, hi = v.n_elems;  // bs_off[lo] <= t < bs_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (v.bs_off[mid] <= t) lo = mid; else hi = mid;
   }
+  const int e = lo;
+  w.bs_elem[t] = e;
+  w.bs_pack[t] = unsigned(v.bs_dir[t]) | (unsigned(v.bs_i[t]) << 8) | (unsigned(v.bs_j[t]) << 16);
+  const int d = v.bs_dof[t];
+  if (d < 0 || d >= v.n_dofs) {
+    set_err(w.err, kErrView);
+    return;
+  }
+  const long long val = ((long long)e << 32) | (long long)(t - v.bs_off[e]);
+  if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot0[d]), ~0ull, (unsigned long long)val) != ~0ull)
+    if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot1[d]), ~0ull, (unsigned long long)val) != ~0ull)
+      set_err(w.err, kErrContrib);
 }
 
-__global__ void k_dof_idx(DevView v, Work w) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= v.n_elems) return;
-  const long long b0 = v.bs_off[e];
-  for (long long t = b0; t < v.bs_off[e + 1]; ++t) {
-    const int d = v.bs_dof[t];
-    if (d < 0 || d >= v.n_dofs) continue;
-    const int cnt = w.dof_cnt[d];
-    const int e1 = w.dof_e1[d], e2 = w.dof_e2[d];
-    if (cnt == 2 && e1 == e2) {
-      set_err(w.err, kErrView);  // two specs of one DoF on the same Elem
-      continue;
-    }
-    if (e == e1) w.dof_k1[d] = int(t - b0);
-    else if (e == e2) w.dof_k2[d] = int(t - b0);
+__device__ __forceinline__ void dof_sig(const Work& w, long long d, int& e1, int& k1, int& e2, int& k2) {
+  long long s0 = w.slot0[d], s1 = w.slot1[d];
+  if (s1 >= 0 && (s1 >> 32) < (s0 >> 32)) {
+    const long long t = s0;
+    s0 = s1;
+    s1 = t;
   }
+  e1 = s0 >= 0 ? int(s0 >> 32) : -1;
+  k1 = s0 >= 0 ? int(s0 & 0xffffffff) : -1;
+  e2 = s1 >= 0 ? int(s1 >> 32) : -1;
+  k2 = s1 >= 0 ? int(s1 & 0xffffffff) : -1;
 }
 
-__global__ void k_dof_flags(Work w, long long n_dofs) {
-  long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (d >= n_dofs) return;
-  const int cnt = w.dof_cnt[d];
-  if (cnt < 1 || cnt > 2) set_err(w.err, cnt > 2 ? kErrContrib : kErrView);
-  const int e1 = w.dof_e1[d];
-  const int e2 = cnt == 2 ? w.dof_e2[d] : -1;
+__global__ void k_dof_flags(Work w) {
+  const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (d >= w.n_dofs) return;
+  int e1, k1, e2, k2;
+  dof_sig(w, d, e1, k1, e2, k2);
+  if (e1 < 0 || (e2 >= 0 && e2 == e1)) set_err(w.err, kErrView);  // no spec, or two on one Elem
+  w.dof_e1[d] = e1;
+  w.dof_k1[d] = k1;
+  w.dof_e2[d] = e2;
+  w.dof_k2[d] = k2;
   int flag = 1;
   if (d > 0) {
-    const int pc = w.dof_cnt[d - 1];
-    const int p1 = w.dof_e1[d - 1];
-    const int p2 = pc == 2 ? w.dof_e2[d - 1] : -1;
+    int p1, pk1, p2, pk2;
+    dof_sig(w, d - 1, p1, pk1, p2, pk2);
     flag = (p1 != e1 || p2 != e2) ? 1 : 0;
   }
   w.dof_flag[d] = flag;
 }
 
-__global__ void k_group_starts(Work w, long long n_dofs) {
-  long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (d >= n_dofs) return;
+__global__ void k_group_starts(Work w) {
+  const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (d >= w.n_dofs) return;
   if (w.dof_flag[d]) {
     const int g = w.dof_grp[d] - 1;
     w.g_start[g] = int(d);
     w.g_e1[g] = w.dof_e1[d];
-    w.g_e2[g] = w.dof_cnt[d] == 2 ? w.dof_e2[d] : -1;
+    w.g_e2[g] = w.dof_e2[d];
   }
-  if (d == n_dofs - 1) w.g_start[w.dof_grp[d]] = int(n_dofs);
+  if (d == w.n_dofs - 1) w.g_start[w.dof_grp[d]] = int(w.n_dofs);
 }
 
-// Each Elem's basis-spec list as runs of whole row groups (memberships).
-__global__ void k_mem_count(DevView v, Work w) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= v.n_elems) return;
-  const long long b0 = v.bs_off[e], b1 = v.bs_off[e + 1];
-  int runs = 0;
-  long long t = b0;
-  while (t < b1) {
-    const int d0 = v.bs_dof[t];
-    if (d0 < 0 || d0 >= v.n_dofs) {
-      set_err(w.err, kErrView);
-      break;
-    }
-    const int g = w.dof_grp[d0] - 1;
-    const int gs = w.g_start[g], gn = w.g_start[g + 1] - gs;
-    if (d0 != gs || t + gn > b1) {
-      set_err(w.err, kErrView);
-      break;
-    }
-    for (int k = 1; k < gn; ++k)
-      if (v.bs_dof[t + k] != d0 + k) set_err(w.err, kErrView);
-    ++runs;
-    t += gn;
+// Each Elem's basis-spec list as runs of whole row groups (memberships): flag
+// the first spec of every run and check that each run is one complete group.
+__global__ void k_mem_flags(DevView v, Work w) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= v.n_bs) return;
+  const int e = w.bs_elem[t];
+  const int d = v.bs_dof[t];
+  if (d < 0 || d >= v.n_dofs) {
+    w.mem_flag[t] = 0;
+    return;
   }
-  w.mem_cnt[e] = runs;
+  const int g = w.dof_grp[d] - 1;
+  const bool first = t == v.bs_off[e];
+  const bool last = t + 1 == v.bs_off[e + 1];
+  bool start = first;
+  if (!first) {
+    const int dp = v.bs_dof[t - 1];
+    start = !(dp >= 0 && dp < v.n_dofs && dp + 1 == d && w.dof_grp[dp] - 1 == g);
+  }
+  bool end = last;
+  if (!last) {
+    const int dn = v.bs_dof[t + 1];
+    end = !(dn >= 0 && dn < v.n_dofs && dn == d + 1 && w.dof_grp[dn] - 1 == g);
+  }
+  if ((start && d != w.g_start[g]) || (end && d != w.g_start[g + 1] - 1)) set_err(w.err, kErrView);
+  w.mem_flag[t] = start ? 1 : 0;
 }
 
 __global__ void k_mem_fill(DevView v, Work w) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= v.n_elems) return;
-  const long long b0 = v.bs_off[e], b1 = v.bs_off[e + 1];
-  int slot = w.mem_ptr[e];
-  const int end = w.mem_ptr[e + 1];
-  long long t = b0;
-  while (t < b1 && slot < end) {
-    const int d0 = v.bs_dof[t];
-    if (d0 < 0 || d0 >= v.n_dofs) break;
-    const int g = w.dof_grp[d0] - 1;
-    const int gn = w.g_start[g + 1] - w.g_start[g];
-    w.mem_group[slot] = g;
-    w.mem_bsoff[slot] = int(t - b0);
-    ++slot;
-    t += gn;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t < v.n_bs && w.mem_flag[t]) {
+    const int m = w.mem_idx[t];
+    const int e = w.bs_elem[t];
+    w.mem_group[m] = w.dof_grp[v.bs_dof[t]] - 1;
+    w.mem_bsoff[m] = int(t - v.bs_off[e]);
   }
+  if (t <= v.n_elems) w.mem_ptr[t] = w.mem_idx[v.bs_off[t]];
 }
 
 // 1D Gram library: lib[axis][range][kind][a][b] = sum_m w_m f_a(x'_m) g_b(x_m)
@@ -491,21 +583,313 @@ __global__ void k_gram_lib(Work w, LibShape L) {
 }
 
 // ---------------------------------------------------------------------------
-// Row-group candidate sets (shared by the count and fill passes)
+// Nested-Elem sources of a row group (shared by both paths)
 // ---------------------------------------------------------------------------
 
-struct SrcRec {
-  double cA[4];  // CurlCurl prefactors UU, UV, VU, VV
-  double cB[2];  // L2Inner prefactors UU, VV
-  int x;         // nested Elem
-  int k;         // which Elem of the row group (0, 1)
-  int rel;       // kLocal, kAnc, kDesc
-  int ru, rv;    // Gram-library range ids
-  int pad;
+// Enumerates the Elems nested with each Elem of row group g: the Elem itself,
+// its ancestors and its descendants (breadth first). Sequential; returns the
+// count, or -1 if more than cap.
+__device__ int enumerate_sources(const DevView& v, const Work& w, int g, int cap, int* xs, int* info) {
+  int n = 0;
+  const int e_k[2] = {w.g_e1[g], w.g_e2[g]};
+  for (int k = 0; k < 2; ++k) {
+    const int E = e_k[k];
+    if (E < 0) break;
+    if (n >= cap) return -1;
+    xs[n] = E;
+    info[n] = k | (kLocal << 1);
+    ++n;
+    for (int a = v.parent[E]; a >= 0; a = v.parent[a]) {
+      if (n >= cap) return -1;
+      xs[n] = a;
+      info[n] = k | (kAnc << 1);
+      ++n;
+    }
+    // descendants: breadth first with the source list as the queue
+    const int q0 = n;
+    int head = -1;
+    while (true) {
+      const int pe = head < 0 ? E : xs[head];
+      const int nc = w.n_child[pe];
+      const int fc = w.first_child[pe];
+      for (int c = 0; c < nc; ++c) {
+        if (n >= cap) return -1;
+        xs[n] = fc + c;
+        info[n] = k | (kDesc << 1);
+        ++n;
+      }
+      head = head < 0 ? q0 : head + 1;
+      if (head >= n) break;
+    }
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Wave path: one wavefront per row group
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int y = __shfl_up(x, d, kWave);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+struct WaveShared {
+  double cA[kWS][4];
+  double cB[kWS][2];
+  int src_x[kWS];
+  int src_info[kWS];  // k | rel << 1 | ru << 3 | rv << 17
+  unsigned long long key[kWR];  // row group << 20 | raw index
+  int raw_meta[kWR];            // source | bs offset << 8
+  int dstart[kWD];
+  int cum[kWD + 1];
+  int dsrc0[kWD], dsrc1[kWD];
+  int doff0[kWD], doff1[kWD];
+  int n_src, n_dist, total, u_self;
 };
 
+// Builds row group g's candidate set in this wave's LDS slice.
+// Returns 0 when built, 1 when the group is too large for the wave path,
+// 2 on an invalid domain (error flag set).
+template <bool kCoef>
+__device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S, int lane) {
+  if (lane == 0) {
+    int info[kWS];
+    const int n = enumerate_sources(v, w, g, kWS, S.src_x, info);
+    for (int s = 0; s < (n < 0 ? 0 : n); ++s) S.src_info[s] = info[s];
+    S.n_src = n;
+  }
+  wave_sync();
+  const int n_src = S.n_src;
+  if (n_src < 0) return 1;
+  // membership counts -> gather offsets (n_src <= 32)
+  int cnt = 0, X = -1;
+  if (lane < n_src) {
+    X = S.src_x[lane];
+    cnt = w.mem_ptr[X + 1] - w.mem_ptr[X];
+  }
+  const int incl = wave_incl_scan(cnt, lane);
+  const int n_raw = __shfl(incl, kWave - 1, kWave);
+  if (n_raw > kWR) return 1;
+  if (lane < n_src) {
+    const int base = incl - cnt;
+    const int m0 = w.mem_ptr[X];
+    for (int m = 0; m < cnt; ++m) {
+      const int r = base + m;
+      S.key[r] = ((unsigned long long)(unsigned)w.mem_group[m0 + m] << 20) | (unsigned long long)r;
+      S.raw_meta[r] = lane | (w.mem_bsoff[m0 + m] << 8);
+    }
+    if (kCoef) {
+      const int info = S.src_info[lane];
+      const int k = info & 1, rel = (info >> 1) & 3;
+      const int E = k == 0 ? w.g_e1[g] : w.g_e2[g];
+      const int P = rel == kAnc ? X : E;
+      const int Q = rel == kAnc ? E : X;
+      int ru, rv;
+      block_coefs(w, P, Q, rel == kLocal, S.cA[lane], S.cB[lane], ru, rv);
+      S.src_info[lane] = k | (rel << 1) | (ru << 3) | (rv << 17);
+    }
+  }
+  int np2 = 1;
+  while (np2 < n_raw) np2 <<= 1;
+  for (int r = n_raw + lane; r < np2; r += kWave) S.key[r] = ~0ull;
+  wave_sync();
+  // bitonic sort by (row group, gather order)
+  for (int kk = 2; kk <= np2; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < np2; i += kWave) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const unsigned long long a = S.key[i], b = S.key[ixj];
+          if ((a > b) == ((i & kk) == 0)) {
+            S.key[i] = b;
+            S.key[ixj] = a;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+  // dedupe: first pass writes every new row group, second the second sources
+  int running = 0;
+  int bad = 0;
+  for (int base = 0; base < n_raw; base += kWave) {
+    const int t = base + lane;
+    const bool valid = t < n_raw;
+    const unsigned long long kt = valid ? S.key[t] : 0ull;
+    const int h = valid ? int(kt >> 20) : -1;
+    const int hp = (valid && t > 0) ? int(S.key[t - 1] >> 20) : -2;
+    const bool is_new = valid && h != hp;
+    const unsigned long long mask = __ballot(is_new);
+    const unsigned long long upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+    const int didx = running + __popcll(mask & upto) - 1;
+    if (valid && didx >= kWD) bad |= 1;
+    if (valid && is_new && didx < kWD) {
+      const int meta = S.raw_meta[int(kt & 0xFFFFF)];
+      const int gs = w.g_start[h];
+      S.dstart[didx] = gs;
+      S.cum[didx] = w.g_start[h + 1] - gs;  // size, prefix-summed below
+      S.dsrc0[didx] = meta & 0xff;
+      S.doff0[didx] = meta >> 8;
+      S.dsrc1[didx] = -1;
+    }
+    wave_sync();
+    if (valid && !is_new && didx < kWD) {
+      const int hpp = t >= 2 ? int(S.key[t - 2] >> 20) : -3;
+      if (hpp == h) bad |= 2;  // a third source for one column
+      const int meta = S.raw_meta[int(kt & 0xFFFFF)];
+      S.dsrc1[didx] = meta & 0xff;
+      S.doff1[didx] = meta >> 8;
+    }
+    running += __popcll(mask);
+    wave_sync();
+  }
+  if (__ballot(bad != 0)) {
+    if (__ballot((bad & 2) != 0)) {
+      if (lane == 0) set_err(w.err, kErrContrib);
+      return 2;
+    }
+    return 1;
+  }
+  const int n_dist = running;
+  // prefix sums of the distinct group sizes; locate the group itself
+  int carry = 0;
+  const int gs_self = w.g_start[g];
+  for (int base = 0; base < n_dist; base += kWave) {
+    const int t = base + lane;
+    const int sz = t < n_dist ? S.cum[t] : 0;
+    const int inc = wave_incl_scan(sz, lane);
+    if (t < n_dist) {
+      S.cum[t] = carry + inc - sz;
+      if (S.dstart[t] == gs_self) S.u_self = carry + inc - sz;
+    }
+    carry += __shfl(inc, kWave - 1, kWave);
+  }
+  if (lane == 0) {
+    S.cum[n_dist] = carry;
+    S.n_dist = n_dist;
+    S.total = carry;
+  }
+  wave_sync();
+  return 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_count_wave(DevView v, Work w) {
+  __shared__ WaveShared shared[kWavesPerBlock];
+  const int lane = threadIdx.x % kWave;
+  const int wid = threadIdx.x / kWave;
+  WaveShared& S = shared[wid];
+  const int n_groups = n_groups_of(w);
+  const int stride = gridDim.x * kWavesPerBlock;
+  for (int g = blockIdx.x * kWavesPerBlock + wid; g < n_groups; g += stride) {
+    if (lane == 0) S.u_self = -1;
+    wave_sync();
+    const int st = wave_build<false>(v, w, g, S, lane);
+    if (st == 1) {
+      if (lane == 0) {
+        w.is_large[g] = 1;
+        w.large_list[atomicAdd(w.n_large, 1)] = g;
+      }
+    } else if (st == 0) {
+      if (lane == 0) w.is_large[g] = 0;
+      const int u_self = S.u_self, total = S.total;
+      if (u_self < 0) {
+        if (lane == 0) set_err(w.err, kErrView);
+      } else {
+        const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
+        for (int r = r0 + lane; r < r1; r += kWave) w.row_len[r] = (int64_t)(total - (u_self + (r - r0)));
+      }
+    }
+    wave_sync();
+  }
+}
+
+__device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int c, double cc, double l2) {
+  if (pos < o.capacity) {
+    o.col[pos] = c;
+    o.aval[pos] = o.a_is_cc ? cc : l2;
+    o.bval[pos] = o.b_is_cc ? cc : l2;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_wave(DevView v, Work w, LibShape L, FillOut o) {
+  __shared__ WaveShared shared[kWavesPerBlock];
+  const int lane = threadIdx.x % kWave;
+  const int wid = threadIdx.x / kWave;
+  WaveShared& S = shared[wid];
+  const int n_groups = n_groups_of(w);
+  const int stride = gridDim.x * kWavesPerBlock;
+  const double* __restrict__ lib = w.lib;
+  for (int g = blockIdx.x * kWavesPerBlock + wid; g < n_groups; g += stride) {
+    if (w.is_large[g]) continue;
+    if (lane == 0) S.u_self = -1;
+    wave_sync();
+    if (wave_build<true>(v, w, g, S, lane) != 0 || S.u_self < 0) {
+      wave_sync();
+      continue;
+    }
+    const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
+    const int e1 = w.g_e1[g], e2 = w.g_e2[g];
+    const int n_dist = S.n_dist, total = S.total, u_self = S.u_self;
+    const long long b1 = v.bs_off[e1];
+    const long long b2 = e2 >= 0 ? v.bs_off[e2] : 0;
+    for (int r = r0; r < r1; ++r) {
+      const int u_r = u_self + (r - r0);
+      const int len = total - u_r;
+      const long long out0 = w.row_ptr[r];
+      const unsigned rb[2] = {w.bs_pack[b1 + w.dof_k1[r]], e2 >= 0 ? w.bs_pack[b2 + w.dof_k2[r]] : 0u};
+      for (int j = lane; j < len; j += kWave) {
+        const int u = u_r + j;
+        int a = 0, b = n_dist;  // cum[a] <= u < cum[b]
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (S.cum[mid] <= u) a = mid; else b = mid;
+        }
+        const int within = u - S.cum[a];
+        const int c = S.dstart[a] + within;
+        double cc_sum = 0.0, l2_sum = 0.0;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int sidx = side == 0 ? S.dsrc0[a] : S.dsrc1[a];
+          if (sidx < 0) break;
+          const int info = S.src_info[sidx];
+          const int k = info & 1, rel = (info >> 1) & 3;
+          const int ru = (info >> 3) & 0x3fff, rv = (info >> 17) & 0x3fff;
+          const int off = side == 0 ? S.doff0[a] : S.doff1[a];
+          const unsigned cbs = w.bs_pack[v.bs_off[S.src_x[sidx]] + off + within];
+          double cc, l2;
+          if (rel == kAnc) pair_values(lib, L, S.cA[sidx], S.cB[sidx], ru, rv, cbs, rb[k], cc, l2);
+          else pair_values(lib, L, S.cA[sidx], S.cB[sidx], ru, rv, rb[k], cbs, cc, l2);
+          cc_sum += cc;
+          l2_sum += l2;
+        }
+        write_entry(o, out0 + j, c, cc_sum, l2_sum);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block path: one block per large row group
+// ---------------------------------------------------------------------------
+
 struct GroupShared {
-  SrcRec src[kMaxSrc];
+  double cA[kMaxSrc][4];
+  double cB[kMaxSrc][2];
+  int src_x[kMaxSrc];
+  int src_info[kMaxSrc];
+  int raw_cnt[kMaxSrc + 1];
   unsigned long long key[kMaxRaw];
   short raw_src[kMaxRaw];
   int raw_off[kMaxRaw];
@@ -516,20 +900,9 @@ struct GroupShared {
   int row_rel[kMaxRows + 1];
   unsigned int row_bs[2][kMaxRows];
   int n_src, n_raw, n_dist, total, u_self, fail;
-  int raw_cnt[kMaxSrc + 1];
 };
 
 static_assert(sizeof(GroupShared) <= 64 * 1024, "row-group LDS footprint must fit 64 KiB");
-
-__device__ __forceinline__ int rel_range_id(int4 P, int4 Q, int axis) {
-  const int dp = axis == 0 ? P.x : P.z;
-  const int ip = axis == 0 ? P.y : P.w;
-  const int dq = axis == 0 ? Q.x : Q.z;
-  const int iq = axis == 0 ? Q.y : Q.w;
-  const int dd = dq - dp;
-  const int ii = iq - (ip << dd);
-  return (1 << dd) - 1 + ii;
-}
 
 // Builds the candidate set of row group g in LDS. Returns false (block-uniform)
 // if a limit was exceeded; the error flag is set then.
@@ -538,43 +911,16 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
   const int tid = threadIdx.x;
   if (tid == 0) {
     S.fail = 0;
-    int n = 0;
-    const int e_k[2] = {w.g_e1[g], w.g_e2[g]};
-    for (int k = 0; k < 2 && !S.fail; ++k) {
-      const int E = e_k[k];
-      if (E < 0) break;
-      // E itself
-      if (n >= kMaxSrc) { S.fail = 1; break; }
-      S.src[n].x = E; S.src[n].k = k; S.src[n].rel = kLocal; ++n;
-      // ancestors
-      for (int a = v.parent[E]; a >= 0; a = v.parent[a]) {
-        if (n >= kMaxSrc) { S.fail = 1; break; }
-        S.src[n].x = a; S.src[n].k = k; S.src[n].rel = kAnc; ++n;
-      }
-      // descendants, breadth first, using the source list as the queue
-      int q = n;
-      int head = -1;  // start with E's children
-      while (!S.fail) {
-        const int parent_e = head < 0 ? E : S.src[head].x;
-        const int nc = w.n_child[parent_e];
-        const int fc = w.first_child[parent_e];
-        for (int c = 0; c < nc; ++c) {
-          if (n >= kMaxSrc) { S.fail = 1; break; }
-          S.src[n].x = fc + c; S.src[n].k = k; S.src[n].rel = kDesc; ++n;
-        }
-        head = head < 0 ? q : head + 1;
-        if (head >= n) break;
-      }
-    }
-    S.n_src = n;
-    // membership counts -> raw offsets
+    const int n = enumerate_sources(v, w, g, kMaxSrc, S.src_x, S.src_info);
+    if (n < 0) S.fail = 1;
+    S.n_src = n < 0 ? 0 : n;
     int tot = 0;
-    for (int s = 0; s < n && !S.fail; ++s) {
+    for (int s = 0; s < S.n_src; ++s) {
       S.raw_cnt[s] = tot;
-      const int X = S.src[s].x;
+      const int X = S.src_x[s];
       tot += w.mem_ptr[X + 1] - w.mem_ptr[X];
     }
-    S.raw_cnt[n] = tot;
+    S.raw_cnt[S.n_src] = tot;
     if (tot > kMaxRaw) S.fail = 1;
     S.n_raw = tot;
   }
@@ -588,9 +934,8 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
     }
   }
   const int n_src = S.n_src, n_raw = S.n_raw;
-  // gather memberships
   for (int s = tid; s < n_src; s += blockDim.x) {
-    const int X = S.src[s].x;
+    const int X = S.src_x[s];
     const int m0 = w.mem_ptr[X], m1 = w.mem_ptr[X + 1];
     int r = S.raw_cnt[s];
     for (int m = m0; m < m1; ++m, ++r) {
@@ -599,38 +944,16 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
       S.raw_off[r] = w.mem_bsoff[m];
     }
     if (kWithCoef) {
-      // block prefactors (see file header): P = p-side Elem, Q = q-side Elem
-      const int E = S.src[s].k == 0 ? w.g_e1[g] : w.g_e2[g];
-      const int rel = S.src[s].rel;
+      const int info = S.src_info[s];
+      const int k = info & 1, rel = (info >> 1) & 3;
+      const int E = k == 0 ? w.g_e1[g] : w.g_e2[g];
       const int P = rel == kAnc ? X : E;
       const int Q = rel == kAnc ? E : X;
-      const ElemGeo gp = w.geo[P], gq = w.geo[Q];
-      int ru = 0, rv = 0;
-      double su = 1.0, sv = 1.0;
-      if (rel != kLocal) {
-        const int4 dp = w.dyad[P], dq = w.dyad[Q];
-        ru = rel_range_id(dp, dq, 0);
-        rv = rel_range_id(dp, dq, 1);
-        su = ldexp(1.0, -(dq.x - dp.x));
-        sv = ldexp(1.0, -(dq.z - dp.z));
-      }
-      const bool p_big = gp.det >= gq.det;
-      const double r_uv = p_big ? gp.uv : gq.uv;
-      const double r_vu = p_big ? gp.vu : gq.vu;
-      const double dmax = gp.det >= gq.det ? gp.det : gq.det;
-      const double im = gp.inv_mu;
-      S.src[s].cA[0] = im * ((gp.ju0 * 1.0) * (gq.ju0 * su)) * r_uv;
-      S.src[s].cA[1] = -(im * (gp.ju0 * (gq.jv1 * sv)));
-      S.src[s].cA[2] = -(im * (gp.jv1 * (gq.ju0 * su)));
-      S.src[s].cA[3] = im * ((gp.jv1 * 1.0) * (gq.jv1 * sv)) * r_vu;
-      const double gs = su * sv;
-      S.src[s].cB[0] = gp.eps * gs * dmax * gp.ju0 * gq.ju0;
-      S.src[s].cB[1] = gp.eps * gs * dmax * gp.jv1 * gq.jv1;
-      S.src[s].ru = ru;
-      S.src[s].rv = rv;
+      int ru, rv;
+      block_coefs(w, P, Q, rel == kLocal, S.cA[s], S.cB[s], ru, rv);
+      S.src_info[s] = k | (rel << 1) | (ru << 3) | (rv << 17);
     }
   }
-  // pad and sort (bitonic) by (row group, gather order)
   int np2 = 1;
   while (np2 < n_raw) np2 <<= 1;
   for (int r = n_raw + tid; r < np2; r += blockDim.x) S.key[r] = ~0ull;
@@ -641,8 +964,7 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
         const int ixj = i ^ j;
         if (ixj > i) {
           const unsigned long long a = S.key[i], b = S.key[ixj];
-          const bool up = (i & kk) == 0;
-          if ((a > b) == up) {
+          if ((a > b) == ((i & kk) == 0)) {
             S.key[i] = b;
             S.key[ixj] = a;
           }
@@ -651,14 +973,16 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
       __syncthreads();
     }
   }
-  // dedupe into distinct row groups (one or two sources each) and prefix sums
   if (tid == 0) {
     int nd = 0, run = 0, self = -1, prev = -1;
     for (int t = 0; t < n_raw; ++t) {
       const int h = int(S.key[t] >> 20);
       const int r = int(S.key[t] & 0xFFFFF);
       if (h != prev) {
-        if (nd >= kMaxDist) { S.fail = 1; break; }
+        if (nd >= kMaxDist) {
+          S.fail = 1;
+          break;
+        }
         const int gs = w.g_start[h];
         S.dstart[nd] = gs;
         S.cum[nd] = run;
@@ -671,7 +995,10 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
         ++nd;
         prev = h;
       } else {
-        if (S.ds1[nd - 1] >= 0) { S.fail = 2; break; }
+        if (S.ds1[nd - 1] >= 0) {
+          S.fail = 2;
+          break;
+        }
         S.ds1[nd - 1] = S.raw_src[r];
         S.doff1[nd - 1] = S.raw_off[r];
       }
@@ -692,10 +1019,12 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
   return true;
 }
 
-__global__ void __launch_bounds__(kBlock) k_group_count(DevView v, Work w, int n_groups) {
+__global__ void __launch_bounds__(kBlock) k_count_large(DevView v, Work w) {
   extern __shared__ unsigned char smem[];
   GroupShared& S = *reinterpret_cast<GroupShared*>(smem);
-  for (int g = blockIdx.x; g < n_groups; g += gridDim.x) {
+  const int n_large = *w.n_large;
+  for (int i = blockIdx.x; i < n_large; i += gridDim.x) {
+    const int g = w.large_list[i];
     const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
     if (r1 - r0 > kMaxRows) {
       if (threadIdx.x == 0) set_err(w.err, kErrSize);
@@ -703,75 +1032,43 @@ __global__ void __launch_bounds__(kBlock) k_group_count(DevView v, Work w, int n
     }
     if (!build_group<false>(v, w, g, S)) continue;
     for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x)
-      w.row_len[r] = (long long)(S.total - (S.u_self + (r - r0)));
+      w.row_len[r] = (int64_t)(S.total - (S.u_self + (r - r0)));
     __syncthreads();
   }
 }
 
-__device__ __forceinline__ unsigned pack_bs(const DevView& v, long long t) {
-  return unsigned(v.bs_dir[t]) | (unsigned(v.bs_i[t]) << 8) | (unsigned(v.bs_j[t]) << 16);
-}
-
-// CurlCurl and L2Inner contributions of one nested pair (p, q)
-__device__ __forceinline__ void pair_values(const double* __restrict__ lib, const LibShape& L, const SrcRec& s,
-                                            unsigned pbs, unsigned qbs, double& cc, double& l2) {
-  const int pd = pbs & 0xff, pi = (pbs >> 8) & 0xff, pj = (pbs >> 16) & 0xff;
-  const int qd = qbs & 0xff, qi = (qbs >> 8) & 0xff, qj = (qbs >> 16) & 0xff;
-  const long long tab = (long long)L.p1 * L.p1;
-  const double* gx = lib + (long long)s.ru * kKinds * tab;
-  const double* gy = lib + ((long long)L.nr + s.rv) * kKinds * tab;
-  const int ix = pi * L.p1 + qi, iy = pj * L.p1 + qj;
-  cc = 0.0;
-  l2 = 0.0;
-  if (pd == FEM2D_DIR_U && qd == FEM2D_DIR_U) {
-    const double xnn = gx[kNN * tab + ix];
-    cc = s.cA[0] * xnn * gy[kTdTd * tab + iy];
-    l2 = s.cB[0] * xnn * gy[kTT * tab + iy];
-  } else if (pd == FEM2D_DIR_U && qd == FEM2D_DIR_V) {
-    cc = s.cA[1] * gx[kNTd * tab + ix] * gy[kTdN * tab + iy];
-  } else if (pd == FEM2D_DIR_V && qd == FEM2D_DIR_U) {
-    cc = s.cA[2] * gx[kTdN * tab + ix] * gy[kNTd * tab + iy];
-  } else if (pd == FEM2D_DIR_V && qd == FEM2D_DIR_V) {
-    const double ynn = gy[kNN * tab + iy];
-    cc = s.cA[3] * gx[kTdTd * tab + ix] * ynn;
-    l2 = s.cB[1] * gx[kTT * tab + ix] * ynn;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_group_fill(DevView v, Work w, int n_groups, LibShape L, int a_is_cc,
-                                                      int b_is_cc, int* __restrict__ col, double* __restrict__ aval,
-                                                      double* __restrict__ bval) {
+__global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibShape L, FillOut o) {
   extern __shared__ unsigned char smem[];
   GroupShared& S = *reinterpret_cast<GroupShared*>(smem);
   const int64_t* __restrict__ row_ptr = w.row_ptr;
-  for (int g = blockIdx.x; g < n_groups; g += gridDim.x) {
+  const int n_large = *w.n_large;
+  for (int i = blockIdx.x; i < n_large; i += gridDim.x) {
+    const int g = w.large_list[i];
     const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
     const int nrows = r1 - r0;
     if (nrows > kMaxRows) continue;
     if (!build_group<true>(v, w, g, S)) continue;
     const long long seg0 = row_ptr[r0];
     const int e_k[2] = {w.g_e1[g], w.g_e2[g]};
-    for (int i = threadIdx.x; i <= nrows; i += blockDim.x) {
-      S.row_rel[i] = int(row_ptr[r0 + i] - seg0);
-      if (i < nrows) {
-        const int r = r0 + i;
-        S.row_bs[0][i] = pack_bs(v, v.bs_off[e_k[0]] + w.dof_k1[r]);
-        S.row_bs[1][i] = e_k[1] >= 0 ? pack_bs(v, v.bs_off[e_k[1]] + w.dof_k2[r]) : 0u;
+    for (int q = threadIdx.x; q <= nrows; q += blockDim.x) {
+      S.row_rel[q] = int(row_ptr[r0 + q] - seg0);
+      if (q < nrows) {
+        const int r = r0 + q;
+        S.row_bs[0][q] = w.bs_pack[v.bs_off[e_k[0]] + w.dof_k1[r]];
+        S.row_bs[1][q] = e_k[1] >= 0 ? w.bs_pack[v.bs_off[e_k[1]] + w.dof_k2[r]] : 0u;
       }
     }
     __syncthreads();
     const int seg_n = S.row_rel[nrows];
     const int n_dist = S.n_dist;
     for (int p = threadIdx.x; p < seg_n; p += blockDim.x) {
-      // row of this slot
       int lo = 0, hi = nrows;  // row_rel[lo] <= p < row_rel[hi]
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (S.row_rel[mid] <= p) lo = mid; else hi = mid;
       }
       const int u = S.u_self + lo + (p - S.row_rel[lo]);
-      // distinct row group holding union index u
-      int a = 0, b = n_dist;  // cum[a] <= u < cum[b]
+      int a = 0, b = n_dist;
       while (b - a > 1) {
         const int mid = (a + b) >> 1;
         if (S.cum[mid] <= u) a = mid; else b = mid;
@@ -782,20 +1079,19 @@ __global__ void __launch_bounds__(kBlock) k_group_fill(DevView v, Work w, int n_
       for (int side = 0; side < 2; ++side) {
         const int sidx = side == 0 ? S.ds0[a] : S.ds1[a];
         if (sidx < 0) break;
-        const SrcRec& s = S.src[sidx];
+        const int info = S.src_info[sidx];
+        const int k = info & 1, rel = (info >> 1) & 3;
+        const int ru = (info >> 3) & 0x3fff, rv = (info >> 17) & 0x3fff;
         const int off = side == 0 ? S.doff0[a] : S.doff1[a];
-        const unsigned cbs = pack_bs(v, v.bs_off[s.x] + off + within);
-        const unsigned rbs = S.row_bs[s.k][lo];
+        const unsigned cbs = w.bs_pack[v.bs_off[S.src_x[sidx]] + off + within];
+        const unsigned rbs = S.row_bs[k][lo];
         double cc, l2;
-        if (s.rel == kAnc) pair_values(w.lib, L, s, cbs, rbs, cc, l2);
-        else pair_values(w.lib, L, s, rbs, cbs, cc, l2);
+        if (rel == kAnc) pair_values(w.lib, L, S.cA[sidx], S.cB[sidx], ru, rv, cbs, rbs, cc, l2);
+        else pair_values(w.lib, L, S.cA[sidx], S.cB[sidx], ru, rv, rbs, cbs, cc, l2);
         cc_sum += cc;
         l2_sum += l2;
       }
-      const long long o = seg0 + p;
-      col[o] = c;
-      aval[o] = a_is_cc ? cc_sum : l2_sum;
-      bval[o] = b_is_cc ? cc_sum : l2_sum;
+      write_entry(o, seg0 + p, c, cc_sum, l2_sum);
     }
     __syncthreads();
   }
@@ -839,9 +1135,11 @@ struct fem2d_ctx {
   std::vector<DevBuf> bufs;  // grow-only work buffers, by slot
   DevBuf scan_tmp;
   DevBuf out_col, out_a, out_b;
-  int* h_pinned = nullptr;  // err, n_groups, nnz (as two ints) staging
+  long long out_capacity = 0;
+  long long* h_pinned = nullptr;  // readback staging: nnz, err + n_large, n_groups
+  double* h_glq = nullptr;        // pinned Gauss-Legendre staging
   fem2d_timing timing{};
-  int max_grid = 0;
+  int n_cu = 0;
 };
 
 namespace {
@@ -930,7 +1228,7 @@ int check_common(int cc, long long n_dofs, int glq_u, int glq_v, int basis, int 
   return FEM2D_OK;
 }
 
-int err_to_status(int bits) {
+int err_to_status(long long bits) {
   if (bits & kErrView) {
     set_error("domain view violates the Domain invariants (dof ids, parent links or basis-spec layout)", 0);
     return FEM2D_ERR_INVALID_VIEW;
@@ -943,7 +1241,7 @@ int err_to_status(int bits) {
     set_error("refinement deeper than this build supports (24 levels, 12 halvings per axis)", 0);
     return FEM2D_ERR_UNSUPPORTED;
   }
-  set_error("a row group exceeds this build's limits (256 nested Elems, 1024 memberships or rows)", 0);
+  set_error("a row group exceeds this build's limits (128 nested Elems, 1024 memberships or rows)", 0);
   return FEM2D_ERR_UNSUPPORTED;
 }
 
@@ -969,9 +1267,14 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_pinned), 64, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_glq), 4 * kMaxGlq * sizeof(double), hipHostMallocDefault));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
-  ctx->max_grid = prop.multiProcessorCount * 8;
+  ctx->n_cu = prop.multiProcessorCount;
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fill_large),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
   *out = ctx;
   return FEM2D_OK;
 }
@@ -987,6 +1290,7 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+  if (ctx->h_glq) (void)hipHostFree(ctx->h_glq);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1033,6 +1337,19 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
     set_error("domain view has null basis-spec arrays", 0);
     return FEM2D_ERR_INVALID_VIEW;
   }
+  if (nbs > 0x7fffffffLL) {
+    set_error("more basis specs than this build supports (2^31)", 0);
+    return FEM2D_ERR_UNSUPPORTED;
+  }
+  if (ne > 0 && v->bs_offsets[0] != 0) {
+    set_error("domain view basis-spec offsets must start at 0", 0);
+    return FEM2D_ERR_INVALID_VIEW;
+  }
+  for (int e = 0; e < ne; ++e)
+    if (v->bs_offsets[e + 1] < v->bs_offsets[e]) {
+      set_error("domain view basis-spec offsets decrease", 0);
+      return FEM2D_ERR_INVALID_VIEW;
+    }
   auto* dd = new fem2d_device_domain();
   dd->device = ctx->device;
   dd->cc = v->cc;
@@ -1056,10 +1373,11 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
   }
   int rc = FEM2D_OK;
   std::vector<long long> off(v->bs_offsets, v->bs_offsets + ne + 1);
+  if (ne == 0) off.assign(1, 0);
   if ((rc = upload(v->parent, size_t(ne), &dd->parent)) || (rc = upload(v->loc, size_t(ne), &dd->loc)) ||
       (rc = upload(v->element_box, size_t(4 * ne), &dd->box)) ||
       (rc = upload(v->materials, size_t(2 * ne), &dd->mat)) ||
-      (rc = upload(off.data(), size_t(ne + 1), &dd->bs_off)) ||
+      (rc = upload(off.data(), off.size(), &dd->bs_off)) ||
       (rc = upload(v->bs_dir, size_t(nbs), &dd->bs_dir)) || (rc = upload(v->bs_i, size_t(nbs), &dd->bs_i)) ||
       (rc = upload(v->bs_j, size_t(nbs), &dd->bs_j)) || (rc = upload(v->bs_dof, size_t(nbs), &dd->bs_dof))) {
     fem2d_device_domain_free(dd);
@@ -1085,9 +1403,14 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     return FEM2D_ERR_INVALID_ARG;
   }
   if (int rc = check_common(dd->cc, dd->n_dofs, glq_u, glq_v, basis, ai, bi, precision)) return rc;
+  if (dd->n_bs == 0 || dd->n_elems == 0) {
+    set_error("domain view has DoFs but no basis specs", 0);
+    return FEM2D_ERR_INVALID_VIEW;
+  }
   HIP_TRY(hipSetDevice(ctx->device));
   const int ne = dd->n_elems;
   const long long nd = dd->n_dofs;
+  const long long nbs = dd->n_bs;
   const int nu = glq_u != 0 ? glq_u : default_ngq(dd->max_i);
   const int nv = glq_v != 0 ? glq_v : default_ngq(dd->max_j);
   if (nu > kMaxGlq || nv > kMaxGlq || nu < 1 || nv < 1) {
@@ -1100,25 +1423,30 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   }
   if (dd->max_du > kMaxRelDepth || dd->max_dv > kMaxRelDepth) return err_to_status(kErrDepth);
 
-  // work buffers
+  // work buffers (grow-only, reused across calls)
   Work w{};
+  w.n_dofs = nd;
   int rc = FEM2D_OK;
   int k = 0;
   double* glq_dev = nullptr;
-  if ((rc = slot(ctx, k++, size_t(std::max(ne, 1)), &w.geo)) ||
-      (rc = slot(ctx, k++, size_t(std::max(ne, 1)), &w.first_child)) ||
-      (rc = slot(ctx, k++, size_t(std::max(ne, 1)), &w.max_child)) ||
-      (rc = slot(ctx, k++, size_t(std::max(ne, 1)), &w.n_child)) ||
-      (rc = slot(ctx, k++, size_t(std::max(ne, 1)), &w.dyad)) ||
-      (rc = slot(ctx, k++, size_t(nd), &w.dof_e1)) || (rc = slot(ctx, k++, size_t(nd), &w.dof_e2)) ||
-      (rc = slot(ctx, k++, size_t(nd), &w.dof_k1)) || (rc = slot(ctx, k++, size_t(nd), &w.dof_k2)) ||
-      (rc = slot(ctx, k++, size_t(nd), &w.dof_cnt)) || (rc = slot(ctx, k++, size_t(nd), &w.dof_flag)) ||
-      (rc = slot(ctx, k++, size_t(nd), &w.dof_grp)) || (rc = slot(ctx, k++, size_t(nd + 1), &w.g_start)) ||
-      (rc = slot(ctx, k++, size_t(nd), &w.g_e1)) || (rc = slot(ctx, k++, size_t(nd), &w.g_e2)) ||
-      (rc = slot(ctx, k++, size_t(ne + 1), &w.mem_cnt)) || (rc = slot(ctx, k++, size_t(ne + 1), &w.mem_ptr)) ||
-      (rc = slot(ctx, k++, size_t(nd + 1), &w.row_len)) || (rc = slot(ctx, k++, size_t(nd + 1), &w.row_ptr)) || (rc = slot(ctx, k++, size_t(4 * kMaxGlq), &glq_dev)) ||
-      (rc = slot(ctx, k++, size_t(1), &w.err)))
+  const size_t E1 = size_t(std::max(ne, 1)), N1 = size_t(nd + 1), B1 = size_t(nbs + 1);
+  if ((rc = slot(ctx, k++, E1, &w.geo)) || (rc = slot(ctx, k++, E1, &w.first_child)) ||
+      (rc = slot(ctx, k++, E1, &w.max_child)) || (rc = slot(ctx, k++, E1, &w.n_child)) ||
+      (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, N1, &w.slot0)) ||
+      (rc = slot(ctx, k++, N1, &w.slot1)) || (rc = slot(ctx, k++, B1, &w.bs_elem)) ||
+      (rc = slot(ctx, k++, B1, &w.bs_pack)) || (rc = slot(ctx, k++, N1, &w.dof_e1)) ||
+      (rc = slot(ctx, k++, N1, &w.dof_e2)) || (rc = slot(ctx, k++, N1, &w.dof_k1)) ||
+      (rc = slot(ctx, k++, N1, &w.dof_k2)) || (rc = slot(ctx, k++, N1, &w.dof_flag)) ||
+      (rc = slot(ctx, k++, N1, &w.dof_grp)) || (rc = slot(ctx, k++, N1, &w.g_start)) ||
+      (rc = slot(ctx, k++, N1, &w.g_e1)) || (rc = slot(ctx, k++, N1, &w.g_e2)) ||
+      (rc = slot(ctx, k++, B1, &w.mem_flag)) || (rc = slot(ctx, k++, B1, &w.mem_idx)) ||
+      (rc = slot(ctx, k++, size_t(ne + 1), &w.mem_ptr)) || (rc = slot(ctx, k++, B1, &w.mem_group)) ||
+      (rc = slot(ctx, k++, B1, &w.mem_bsoff)) || (rc = slot(ctx, k++, N1, &w.row_len)) ||
+      (rc = slot(ctx, k++, N1, &w.row_ptr)) || (rc = slot(ctx, k++, size_t(4 * kMaxGlq), &glq_dev)) ||
+      (rc = slot(ctx, k++, size_t(2), &w.err)) || (rc = slot(ctx, k++, N1, &w.large_list)) ||
+      (rc = slot(ctx, k++, N1, &w.is_large)))
     return rc;
+  w.n_large = w.err + 1;
   w.glq = glq_dev;
 
   LibShape L;
@@ -1129,18 +1457,28 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   L.basis = basis;
   const long long lib_n = 2LL * L.nr * kKinds * L.p1 * L.p1;
   if ((rc = slot(ctx, k++, size_t(lib_n), &w.lib))) return rc;
-  const int k_mem = k++;
-  const int k_out0 = k;  // spare
 
-  // Gauss-Legendre nodes and weights (host Newton iteration, FP64)
-  std::vector<double> glq(4 * kMaxGlq, 0.0);
-  fem2d::gauss_legendre_nodes(nu, glq.data(), glq.data() + kMaxGlq);
-  fem2d::gauss_legendre_nodes(nv, glq.data() + 2 * kMaxGlq, glq.data() + 3 * kMaxGlq);
+  // scan temporaries
+  size_t tmp_bytes = 0, t2 = 0;
+  HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, t2, w.dof_flag, w.dof_grp, int(nd), ctx->stream));
+  tmp_bytes = std::max(tmp_bytes, t2);
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.mem_flag, w.mem_idx, int(nbs + 1), ctx->stream));
+  tmp_bytes = std::max(tmp_bytes, t2);
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.row_len, w.row_ptr, int(nd + 1), ctx->stream));
+  tmp_bytes = std::max(tmp_bytes, t2);
+  if ((rc = ensure(ctx->scan_tmp, tmp_bytes))) return rc;
+
+  hipStream_t st = ctx->stream;
+  // the previous call's readback of the pinned staging buffers has completed
+  // (every call ends with a stream synchronisation)
+  std::fill(ctx->h_glq, ctx->h_glq + 4 * kMaxGlq, 0.0);
+  fem2d::gauss_legendre_nodes(nu, ctx->h_glq, ctx->h_glq + kMaxGlq);
+  fem2d::gauss_legendre_nodes(nv, ctx->h_glq + 2 * kMaxGlq, ctx->h_glq + 3 * kMaxGlq);
 
   DevView v;
   v.n_elems = ne;
   v.n_dofs = nd;
-  v.n_bs = dd->n_bs;
+  v.n_bs = nbs;
   v.parent = dd->parent;
   v.loc = dd->loc;
   v.box = dd->box;
@@ -1151,97 +1489,93 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   v.bs_j = dd->bs_j;
   v.bs_dof = dd->bs_dof;
 
-  hipStream_t st = ctx->stream;
-  HIP_TRY(hipEventRecord(ctx->ev[0], st));
-  HIP_TRY(hipMemcpyAsync(glq_dev, glq.data(), glq.size() * sizeof(double), hipMemcpyHostToDevice, st));
   const int B = 256;
-  k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd);
+  const unsigned wave_grid = unsigned(std::max(1, ctx->n_cu * 8));
+  const unsigned large_grid = unsigned(std::max(1, ctx->n_cu * 2));
+  HIP_TRY(hipEventRecord(ctx->ev[0], st));
+  HIP_TRY(hipMemcpyAsync(glq_dev, ctx->h_glq, 4 * kMaxGlq * sizeof(double), hipMemcpyHostToDevice, st));
+  k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd, nbs);
   k_elem_prep<<<grid_for(ne, B), B, 0, st>>>(v, w);
   k_child_check<<<grid_for(ne, B), B, 0, st>>>(v, w);
-  k_dof_sig<<<grid_for(ne, B), B, 0, st>>>(v, w);
-  k_dof_idx<<<grid_for(ne, B), B, 0, st>>>(v, w);
-  k_dof_flags<<<grid_for(nd, B), B, 0, st>>>(w, nd);
+  k_bs_slots<<<grid_for(nbs, B), B, 0, st>>>(v, w);
+  k_dof_flags<<<grid_for(nd, B), B, 0, st>>>(w);
   HIP_TRY(hipGetLastError());
-
-  // dof_flag -> inclusive scan -> row-group ids
-  size_t tmp_bytes = 0, t2 = 0;
-  HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, w.dof_flag, w.dof_grp, int(nd), st));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.row_len, w.row_ptr, int(nd + 1), st));
-  tmp_bytes = std::max(tmp_bytes, t2);
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.mem_cnt, w.mem_ptr, ne + 1, st));
-  tmp_bytes = std::max(tmp_bytes, t2);
-  if ((rc = ensure(ctx->scan_tmp, tmp_bytes))) return rc;
   HIP_TRY(hipcub::DeviceScan::InclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.dof_flag, w.dof_grp, int(nd), st));
-  k_group_starts<<<grid_for(nd, B), B, 0, st>>>(w, nd);
-  k_mem_count<<<grid_for(ne, B), B, 0, st>>>(v, w);
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.mem_cnt, w.mem_ptr, ne + 1, st));
-  // total memberships and number of row groups
-  HIP_TRY(hipMemcpyAsync(ctx->h_pinned, w.mem_ptr + ne, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 1, w.dof_grp + (nd - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 2, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (ctx->h_pinned[2]) return err_to_status(ctx->h_pinned[2]);
-  const int n_mem = ctx->h_pinned[0];
-  const int n_groups = ctx->h_pinned[1];
-  if ((rc = slot(ctx, k_mem, size_t(2 * std::max(n_mem, 1)), &w.mem_group))) return rc;
-  w.mem_bsoff = w.mem_group + std::max(n_mem, 1);
-  (void)k_out0;
-  k_mem_fill<<<grid_for(ne, B), B, 0, st>>>(v, w);
+  k_group_starts<<<grid_for(nd, B), B, 0, st>>>(w);
+  k_mem_flags<<<grid_for(nbs, B), B, 0, st>>>(v, w);
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.mem_flag, w.mem_idx, int(nbs + 1), st));
+  k_mem_fill<<<grid_for(std::max<long long>(nbs, ne + 1), B), B, 0, st>>>(v, w);
   k_gram_lib<<<grid_for(lib_n, B), B, 0, st>>>(w, L);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[1], st));
 
   // symbolic: row lengths per row group, then row_ptr
-  const size_t shmem = sizeof(GroupShared);
-  const unsigned ggrid = unsigned(std::min<long long>(n_groups, (long long)ctx->max_grid * 4));
-  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_group_count),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
-  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_group_fill),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
   HIP_TRY(hipEventRecord(ctx->ev[4], st));
-  k_group_count<<<ggrid, kBlock, shmem, st>>>(v, w, n_groups);
-  HIP_TRY(hipGetLastError());
+  k_count_wave<<<wave_grid, kBlock, 0, st>>>(v, w);
   HIP_TRY(hipEventRecord(ctx->ev[5], st));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.row_len, w.row_ptr, int(nd + 1), st));
-  HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 4, w.row_ptr + nd, sizeof(long long), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 2, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipEventRecord(ctx->ev[2], st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (ctx->h_pinned[2]) return err_to_status(ctx->h_pinned[2]);
-  long long nnz = 0;
-  std::memcpy(&nnz, ctx->h_pinned + 4, sizeof(long long));
-
-  // numeric: columns and values
-  if ((rc = ensure(ctx->out_col, size_t(nnz) * sizeof(int))) || (rc = ensure(ctx->out_a, size_t(nnz) * 8)) ||
-      (rc = ensure(ctx->out_b, size_t(nnz) * 8)))
-    return rc;
-  HIP_TRY(hipEventRecord(ctx->ev[6], st));
-  k_group_fill<<<ggrid, kBlock, shmem, st>>>(v, w, n_groups, L, ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0,
-                                              bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0,
-                                              static_cast<int*>(ctx->out_col.p), static_cast<double*>(ctx->out_a.p),
-                                              static_cast<double*>(ctx->out_b.p));
+  k_count_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[7], st));
-  HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 2, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipEventRecord(ctx->ev[3], st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (ctx->h_pinned[2]) return err_to_status(ctx->h_pinned[2]);
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.row_len, w.row_ptr, int(nd + 1), st));
+  HIP_TRY(hipEventRecord(ctx->ev[2], st));
+
+  // numeric: columns and values into the output buffers; if they were too
+  // small, the kernels write nothing beyond capacity and the pass is repeated
+  // after growing them
+  long long nnz = 0;
+  int n_large = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    FillOut o;
+    o.col = static_cast<int*>(ctx->out_col.p);
+    o.aval = static_cast<double*>(ctx->out_a.p);
+    o.bval = static_cast<double*>(ctx->out_b.p);
+    o.capacity = ctx->out_capacity;
+    o.a_is_cc = ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
+    o.b_is_cc = bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
+    HIP_TRY(hipEventRecord(ctx->ev[6], st));
+    if (o.capacity > 0) k_fill_wave<<<wave_grid, kBlock, 0, st>>>(v, w, L, o);
+    HIP_TRY(hipEventRecord(ctx->ev[7], st));
+    if (o.capacity > 0) k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 0, w.row_ptr + nd, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 1, w.err, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 2, w.dof_grp + (nd - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int err_pair[2];
+    std::memcpy(err_pair, ctx->h_pinned + 1, sizeof(err_pair));
+    if (err_pair[0]) return err_to_status(err_pair[0]);
+    nnz = ctx->h_pinned[0];
+    n_large = err_pair[1];
+    if (nnz <= ctx->out_capacity) break;
+    if (attempt == 1) {
+      set_error("output capacity mismatch after growing the buffers", 0);
+      return FEM2D_ERR_HIP;
+    }
+    const size_t cap = size_t(nnz + nnz / 16 + 1024);
+    if ((rc = ensure(ctx->out_col, cap * sizeof(int))) || (rc = ensure(ctx->out_a, cap * 8)) ||
+        (rc = ensure(ctx->out_b, cap * 8)))
+      return rc;
+    ctx->out_capacity = (long long)cap;
+  }
 
   float t01 = 0, t12 = 0, t23 = 0, t03 = 0, tc = 0, tf = 0;
-  (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
-  (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
-  ctx->timing.count_kernel_ms = tc;
-  ctx->timing.fill_kernel_ms = tf;
-  ctx->timing.nnz = nnz;
   (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
   (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
   (void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
   (void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
+  (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
+  (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
   ctx->timing.prep_ms = t01;
   ctx->timing.symbolic_ms = t12;
   ctx->timing.fill_ms = t23;
   ctx->timing.total_ms = t03;
-  ctx->timing.n_groups = n_groups;
+  ctx->timing.count_kernel_ms = tc;
+  ctx->timing.fill_kernel_ms = tf;
+  ctx->timing.nnz = nnz;
+  int ng = 0;
+  std::memcpy(&ng, ctx->h_pinned + 2, sizeof(int));
+  ctx->timing.n_groups = ng;
+  ctx->timing.n_large = n_large;
 
   out->n = nd;
   out->nnz = nnz;

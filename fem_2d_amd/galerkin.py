@@ -94,7 +94,7 @@ class Context:
         check(self._lib.fem2d_ctx_last_timing(self._h, C.byref(t)))
         return dict(total_ms=t.total_ms, prep_ms=t.prep_ms, symbolic_ms=t.symbolic_ms, fill_ms=t.fill_ms,
                     n_groups=t.n_groups, count_kernel_ms=t.count_kernel_ms, fill_kernel_ms=t.fill_kernel_ms,
-                    nnz=t.nnz)
+                    nnz=t.nnz, n_large=t.n_large)
 
 
 _default_ctx: Optional[Context] = None

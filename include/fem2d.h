@@ -171,9 +171,10 @@ typedef struct fem2d_timing {
   double symbolic_ms; /* per-group column sets and row lengths, row_ptr scan */
   double fill_ms;     /* column, A and B values of every stored entry */
   int64_t n_groups;   /* row groups (runs of DoFs whose basis specs share their Elems) */
-  double count_kernel_ms; /* the row-length kernel alone */
-  double fill_kernel_ms;  /* the entry-writing kernel alone */
+  double count_kernel_ms; /* the wave-per-group row-length kernel alone */
+  double fill_kernel_ms;  /* the wave-per-group entry-writing kernel alone */
   int64_t nnz;
+  int64_t n_large;    /* row groups assembled one block per group (large column sets) */
 } fem2d_timing;
 int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out);
 
