@@ -62,6 +62,8 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kWS = 32;            // nested Elems per row group
 constexpr int kWR = 128;           // memberships gathered per row group
 constexpr int kWD = 128;           // distinct row groups in one candidate set
+constexpr int kWC = 256;           // candidate columns from the group's first row on
+constexpr int kWRows = 128;        // rows per row group
 // block path (one block per row group)
 constexpr int kMaxSrc = 128;
 constexpr int kMaxRaw = 1024;
@@ -108,6 +110,7 @@ struct DevView {
   const unsigned char* bs_i;
   const unsigned char* bs_j;
   const int* bs_dof;
+  int p1;  // orders per axis in the Gram library (max expansion order + 1)
 };
 
 struct Work {
@@ -305,30 +308,28 @@ __device__ __forceinline__ void block_coefs(const Work& w, int P, int Q, bool lo
   cB[1] = gp.eps * gs * dmax * gp.jv1 * gq.jv1;
 }
 
-// CurlCurl and L2Inner contributions of one nested pair (p, q)
+// CurlCurl and L2Inner contributions of one nested pair (p, q). Directions are
+// U = 0 / V = 1 (W never reaches the device: the view check rejects it). With
+// pair = 2 dir_p + dir_q the CurlCurl x-kind is NN, NT', T'N, T'T' = pair and
+// the y-kind is 3 - pair; L2Inner is non-zero for UU (x NN, y TT) and VV
+// (x TT, y NN) only, and shares one factor with CurlCurl.
 __device__ __forceinline__ void pair_values(const double* __restrict__ lib, const LibShape& L, const double* cA,
                                             const double* cB, int ru, int rv, unsigned pbs, unsigned qbs, double& cc,
                                             double& l2) {
-  const int pd = pbs & 0xff, pi = (pbs >> 8) & 0xff, pj = (pbs >> 16) & 0xff;
-  const int qd = qbs & 0xff, qi = (qbs >> 8) & 0xff, qj = (qbs >> 16) & 0xff;
-  const long long tab = (long long)L.p1 * L.p1;
-  const double* gx = lib + (long long)ru * kKinds * tab;
-  const double* gy = lib + ((long long)L.nr + rv) * kKinds * tab;
+  const int pd = pbs & 1, pi = (pbs >> 8) & 0xff, pj = (pbs >> 16) & 0xff;
+  const int qd = qbs & 1, qi = (qbs >> 8) & 0xff, qj = (qbs >> 16) & 0xff;
+  const int tab = L.p1 * L.p1;
+  const double* gx = lib + (long long)ru * (kKinds * tab);
+  const double* gy = lib + ((long long)L.nr + rv) * (kKinds * tab);
   const int ix = pi * L.p1 + qi, iy = pj * L.p1 + qj;
-  cc = 0.0;
+  const int pair = 2 * pd + qd;
+  const double fx = gx[pair * tab + ix];
+  const double fy = gy[(3 - pair) * tab + iy];
+  cc = cA[pair] * fx * fy;
   l2 = 0.0;
-  if (pd == FEM2D_DIR_U && qd == FEM2D_DIR_U) {
-    const double xnn = gx[kNN * tab + ix];
-    cc = cA[0] * xnn * gy[kTdTd * tab + iy];
-    l2 = cB[0] * xnn * gy[kTT * tab + iy];
-  } else if (pd == FEM2D_DIR_U && qd == FEM2D_DIR_V) {
-    cc = cA[1] * gx[kNTd * tab + ix] * gy[kTdN * tab + iy];
-  } else if (pd == FEM2D_DIR_V && qd == FEM2D_DIR_U) {
-    cc = cA[2] * gx[kTdN * tab + ix] * gy[kNTd * tab + iy];
-  } else if (pd == FEM2D_DIR_V && qd == FEM2D_DIR_V) {
-    const double ynn = gy[kNN * tab + iy];
-    cc = cA[3] * gx[kTdTd * tab + ix] * ynn;
-    l2 = cB[1] * gx[kTT * tab + ix] * ynn;
+  if (pd == qd) {
+    if (pd == 0) l2 = cB[0] * fx * gy[kTT * tab + iy];
+    else l2 = cB[1] * gx[kTT * tab + ix] * fy;
   }
 }
 
@@ -452,7 +453,8 @@ __global__ void k_bs_slots(DevView v, Work w) {
   w.bs_elem[t] = e;
   w.bs_pack[t] = unsigned(v.bs_dir[t]) | (unsigned(v.bs_i[t]) << 8) | (unsigned(v.bs_j[t]) << 16);
   const int d = v.bs_dof[t];
-  if (d < 0 || d >= v.n_dofs) {
+  // H(curl) specs are U or V directed with orders inside the Gram library
+  if (d < 0 || d >= v.n_dofs || v.bs_dir[t] > FEM2D_DIR_V || v.bs_i[t] >= v.p1 || v.bs_j[t] >= v.p1) {
     set_err(w.err, kErrView);
     return;
   }
@@ -644,30 +646,43 @@ __device__ __forceinline__ int wave_incl_scan(int x, int lane) {
   return x;
 }
 
+// Per-wave LDS of the candidate-set build (both wave kernels)
 struct WaveShared {
-  double cA[kWS][4];
-  double cB[kWS][2];
   int src_x[kWS];
   int src_info[kWS];  // k | rel << 1 | ru << 3 | rv << 17
-  unsigned long long key[kWR];  // row group << 20 | raw index
-  int raw_meta[kWR];            // source | bs offset << 8
+  union {
+    struct {
+      unsigned long long key[kWR];  // row group << 20 | raw index
+      int raw_meta[kWR];            // source | bs offset << 8
+    } sort;
+    uint2 row_pk[kWRows];  // fill: packed specs of the group's rows on its first / second Elem
+  } u;
   int dstart[kWD];
   int cum[kWD + 1];
-  int dsrc0[kWD], dsrc1[kWD];
-  int doff0[kWD], doff1[kWD];
+  int dmeta0[kWD];  // source | bs offset << 8 of the first contribution to a candidate group
+  int dmeta1[kWD];  // the same for the second contribution, -1 if none
   int n_src, n_dist, total, u_self;
 };
 
-// Builds row group g's candidate set in this wave's LDS slice.
-// Returns 0 when built, 1 when the group is too large for the wave path,
-// 2 on an invalid domain (error flag set).
-template <bool kCoef>
-__device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S, int lane) {
+// Per-wave LDS of the fill only: source prefactors and the staged columns
+// t = 0..T-1 (candidate index u_self + t) of the group's first row
+struct WaveFill {
+  double cA[kWS][4];
+  double cB[kWS][2];
+  int col_c[kWC];
+  unsigned col_p0[kWC];  // packed spec | source << 24
+  unsigned col_p1[kWC];  // packed spec | source << 24, ~0 if none
+};
+
+// Builds row group g's candidate set in this wave's LDS slice. Returns 0 when
+// built and inside the wave path's limits, 1 when the group needs the block
+// path, 2 on an invalid domain (error flag set). With cA/cB non-null the
+// per-source prefactors are computed as well.
+__device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S, int lane, double (*cA)[4],
+                          double (*cB)[2]) {
   if (lane == 0) {
-    int info[kWS];
-    const int n = enumerate_sources(v, w, g, kWS, S.src_x, info);
-    for (int s = 0; s < (n < 0 ? 0 : n); ++s) S.src_info[s] = info[s];
-    S.n_src = n;
+    S.n_src = enumerate_sources(v, w, g, kWS, S.src_x, S.src_info);
+    S.u_self = -1;
   }
   wave_sync();
   const int n_src = S.n_src;
@@ -686,23 +701,23 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
     const int m0 = w.mem_ptr[X];
     for (int m = 0; m < cnt; ++m) {
       const int r = base + m;
-      S.key[r] = ((unsigned long long)(unsigned)w.mem_group[m0 + m] << 20) | (unsigned long long)r;
-      S.raw_meta[r] = lane | (w.mem_bsoff[m0 + m] << 8);
+      S.u.sort.key[r] = ((unsigned long long)(unsigned)w.mem_group[m0 + m] << 20) | (unsigned long long)r;
+      S.u.sort.raw_meta[r] = lane | (w.mem_bsoff[m0 + m] << 8);
     }
-    if (kCoef) {
+    if (cA) {
       const int info = S.src_info[lane];
       const int k = info & 1, rel = (info >> 1) & 3;
       const int E = k == 0 ? w.g_e1[g] : w.g_e2[g];
       const int P = rel == kAnc ? X : E;
       const int Q = rel == kAnc ? E : X;
       int ru, rv;
-      block_coefs(w, P, Q, rel == kLocal, S.cA[lane], S.cB[lane], ru, rv);
+      block_coefs(w, P, Q, rel == kLocal, cA[lane], cB[lane], ru, rv);
       S.src_info[lane] = k | (rel << 1) | (ru << 3) | (rv << 17);
     }
   }
   int np2 = 1;
   while (np2 < n_raw) np2 <<= 1;
-  for (int r = n_raw + lane; r < np2; r += kWave) S.key[r] = ~0ull;
+  for (int r = n_raw + lane; r < np2; r += kWave) S.u.sort.key[r] = ~0ull;
   wave_sync();
   // bitonic sort by (row group, gather order)
   for (int kk = 2; kk <= np2; kk <<= 1) {
@@ -710,10 +725,10 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
       for (int i = lane; i < np2; i += kWave) {
         const int ixj = i ^ j;
         if (ixj > i) {
-          const unsigned long long a = S.key[i], b = S.key[ixj];
+          const unsigned long long a = S.u.sort.key[i], b = S.u.sort.key[ixj];
           if ((a > b) == ((i & kk) == 0)) {
-            S.key[i] = b;
-            S.key[ixj] = a;
+            S.u.sort.key[i] = b;
+            S.u.sort.key[ixj] = a;
           }
         }
       }
@@ -726,30 +741,26 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
   for (int base = 0; base < n_raw; base += kWave) {
     const int t = base + lane;
     const bool valid = t < n_raw;
-    const unsigned long long kt = valid ? S.key[t] : 0ull;
+    const unsigned long long kt = valid ? S.u.sort.key[t] : 0ull;
     const int h = valid ? int(kt >> 20) : -1;
-    const int hp = (valid && t > 0) ? int(S.key[t - 1] >> 20) : -2;
+    const int hp = (valid && t > 0) ? int(S.u.sort.key[t - 1] >> 20) : -2;
     const bool is_new = valid && h != hp;
     const unsigned long long mask = __ballot(is_new);
     const unsigned long long upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
     const int didx = running + __popcll(mask & upto) - 1;
     if (valid && didx >= kWD) bad |= 1;
     if (valid && is_new && didx < kWD) {
-      const int meta = S.raw_meta[int(kt & 0xFFFFF)];
       const int gs = w.g_start[h];
       S.dstart[didx] = gs;
       S.cum[didx] = w.g_start[h + 1] - gs;  // size, prefix-summed below
-      S.dsrc0[didx] = meta & 0xff;
-      S.doff0[didx] = meta >> 8;
-      S.dsrc1[didx] = -1;
+      S.dmeta0[didx] = S.u.sort.raw_meta[int(kt & 0xFFFFF)];
+      S.dmeta1[didx] = -1;
     }
     wave_sync();
     if (valid && !is_new && didx < kWD) {
-      const int hpp = t >= 2 ? int(S.key[t - 2] >> 20) : -3;
+      const int hpp = t >= 2 ? int(S.u.sort.key[t - 2] >> 20) : -3;
       if (hpp == h) bad |= 2;  // a third source for one column
-      const int meta = S.raw_meta[int(kt & 0xFFFFF)];
-      S.dsrc1[didx] = meta & 0xff;
-      S.doff1[didx] = meta >> 8;
+      S.dmeta1[didx] = S.u.sort.raw_meta[int(kt & 0xFFFFF)];
     }
     running += __popcll(mask);
     wave_sync();
@@ -781,6 +792,11 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
     S.total = carry;
   }
   wave_sync();
+  if (S.u_self < 0) {
+    if (lane == 0) set_err(w.err, kErrView);
+    return 2;
+  }
+  if (S.total - S.u_self > kWC || w.g_start[g + 1] - gs_self > kWRows) return 1;
   return 0;
 }
 
@@ -790,11 +806,11 @@ __global__ void __launch_bounds__(kBlock) k_count_wave(DevView v, Work w) {
   const int wid = threadIdx.x / kWave;
   WaveShared& S = shared[wid];
   const int n_groups = n_groups_of(w);
+  // static assignment: a shared work counter would serialise on device-scope
+  // atomics, which span the XCDs' separate L2 caches
   const int stride = gridDim.x * kWavesPerBlock;
   for (int g = blockIdx.x * kWavesPerBlock + wid; g < n_groups; g += stride) {
-    if (lane == 0) S.u_self = -1;
-    wave_sync();
-    const int st = wave_build<false>(v, w, g, S, lane);
+    const int st = wave_build(v, w, g, S, lane, nullptr, nullptr);
     if (st == 1) {
       if (lane == 0) {
         w.is_large[g] = 1;
@@ -803,12 +819,8 @@ __global__ void __launch_bounds__(kBlock) k_count_wave(DevView v, Work w) {
     } else if (st == 0) {
       if (lane == 0) w.is_large[g] = 0;
       const int u_self = S.u_self, total = S.total;
-      if (u_self < 0) {
-        if (lane == 0) set_err(w.err, kErrView);
-      } else {
-        const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
-        for (int r = r0 + lane; r < r1; r += kWave) w.row_len[r] = (int64_t)(total - (u_self + (r - r0)));
-      }
+      const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
+      for (int r = r0 + lane; r < r1; r += kWave) w.row_len[r] = (int64_t)(total - (u_self + (r - r0)));
     }
     wave_sync();
   }
@@ -822,59 +834,97 @@ __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int
   }
 }
 
+// contribution of one (row spec, column spec) pair through source s
+__device__ __forceinline__ void wave_contrib(const double* __restrict__ lib, const LibShape& L, const WaveShared& S,
+                                             const WaveFill& F, unsigned cpk, uint2 rp, double& cc, double& l2) {
+  const int s = int(cpk >> 24) & (kWS - 1);
+  const int info = S.src_info[s];
+  const int k = info & 1, rel = (info >> 1) & 3;
+  const int ru = (info >> 3) & 0x3fff, rv = (info >> 17) & 0x3fff;
+  const unsigned rpk = k ? rp.y : rp.x;
+  const unsigned cbs = cpk & 0xffffffu;
+  // the p side of the reference's pair is the ancestor's spec
+  const unsigned pbs = rel == kAnc ? cbs : rpk;
+  const unsigned qbs = rel == kAnc ? rpk : cbs;
+  pair_values(lib, L, F.cA[s], F.cB[s], ru, rv, pbs, qbs, cc, l2);
+}
+
+// One wavefront per row group: build the candidate set, stage the group's row
+// specs and its candidate columns in LDS, then write the group's contiguous
+// CSR segment with consecutive lanes on consecutive entries. Entry p of the
+// segment lies in row q = max{q : off(q) <= p}, off(q) = q T - q (q - 1) / 2,
+// where T is the first row's length; its candidate column is q + p - off(q).
 __global__ void __launch_bounds__(kBlock) k_fill_wave(DevView v, Work w, LibShape L, FillOut o) {
   __shared__ WaveShared shared[kWavesPerBlock];
+  __shared__ WaveFill fill[kWavesPerBlock];
   const int lane = threadIdx.x % kWave;
   const int wid = threadIdx.x / kWave;
   WaveShared& S = shared[wid];
+  WaveFill& F = fill[wid];
   const int n_groups = n_groups_of(w);
-  const int stride = gridDim.x * kWavesPerBlock;
   const double* __restrict__ lib = w.lib;
+  const int stride = gridDim.x * kWavesPerBlock;
   for (int g = blockIdx.x * kWavesPerBlock + wid; g < n_groups; g += stride) {
     if (w.is_large[g]) continue;
-    if (lane == 0) S.u_self = -1;
-    wave_sync();
-    if (wave_build<true>(v, w, g, S, lane) != 0 || S.u_self < 0) {
+    if (wave_build(v, w, g, S, lane, F.cA, F.cB) != 0) {
       wave_sync();
       continue;
     }
     const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
+    const int nrows = r1 - r0;
     const int e1 = w.g_e1[g], e2 = w.g_e2[g];
-    const int n_dist = S.n_dist, total = S.total, u_self = S.u_self;
+    const int n_dist = S.n_dist, u_self = S.u_self;
+    const int T = S.total - u_self;
     const long long b1 = v.bs_off[e1];
     const long long b2 = e2 >= 0 ? v.bs_off[e2] : 0;
-    for (int r = r0; r < r1; ++r) {
-      const int u_r = u_self + (r - r0);
-      const int len = total - u_r;
-      const long long out0 = w.row_ptr[r];
-      const unsigned rb[2] = {w.bs_pack[b1 + w.dof_k1[r]], e2 >= 0 ? w.bs_pack[b2 + w.dof_k2[r]] : 0u};
-      for (int j = lane; j < len; j += kWave) {
-        const int u = u_r + j;
-        int a = 0, b = n_dist;  // cum[a] <= u < cum[b]
-        while (b - a > 1) {
-          const int mid = (a + b) >> 1;
-          if (S.cum[mid] <= u) a = mid; else b = mid;
-        }
-        const int within = u - S.cum[a];
-        const int c = S.dstart[a] + within;
-        double cc_sum = 0.0, l2_sum = 0.0;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-          const int sidx = side == 0 ? S.dsrc0[a] : S.dsrc1[a];
-          if (sidx < 0) break;
-          const int info = S.src_info[sidx];
-          const int k = info & 1, rel = (info >> 1) & 3;
-          const int ru = (info >> 3) & 0x3fff, rv = (info >> 17) & 0x3fff;
-          const int off = side == 0 ? S.doff0[a] : S.doff1[a];
-          const unsigned cbs = w.bs_pack[v.bs_off[S.src_x[sidx]] + off + within];
-          double cc, l2;
-          if (rel == kAnc) pair_values(lib, L, S.cA[sidx], S.cB[sidx], ru, rv, cbs, rb[k], cc, l2);
-          else pair_values(lib, L, S.cA[sidx], S.cB[sidx], ru, rv, rb[k], cbs, cc, l2);
-          cc_sum += cc;
-          l2_sum += l2;
-        }
-        write_entry(o, out0 + j, c, cc_sum, l2_sum);
+    for (int q = lane; q < nrows; q += kWave) {
+      const int r = r0 + q;
+      S.u.row_pk[q] = make_uint2(w.bs_pack[b1 + w.dof_k1[r]], e2 >= 0 ? w.bs_pack[b2 + w.dof_k2[r]] : 0u);
+    }
+    for (int t = lane; t < T; t += kWave) {
+      const int u = u_self + t;
+      int a = 0, b = n_dist;  // cum[a] <= u < cum[b]
+      while (b - a > 1) {
+        const int mid = (a + b) >> 1;
+        if (S.cum[mid] <= u) a = mid; else b = mid;
       }
+      const int within = u - S.cum[a];
+      F.col_c[t] = S.dstart[a] + within;
+      const int m0 = S.dmeta0[a];
+      F.col_p0[t] = w.bs_pack[v.bs_off[S.src_x[m0 & 0xff]] + (m0 >> 8) + within] | (unsigned(m0 & 0xff) << 24);
+      const int m1 = S.dmeta1[a];
+      F.col_p1[t] = m1 < 0 ? ~0u
+                           : (w.bs_pack[v.bs_off[S.src_x[m1 & 0xff]] + (m1 >> 8) + within] |
+                              (unsigned(m1 & 0xff) << 24));
+    }
+    wave_sync();
+    const long long seg0 = w.row_ptr[r0];
+    const int seg_n = nrows * T - nrows * (nrows - 1) / 2;
+    const float bq = 2.0f * float(T) + 1.0f;
+    for (int p = lane; p < seg_n; p += kWave) {
+      int q = int((bq - sqrtf(fmaxf(bq * bq - 8.0f * float(p), 0.0f))) * 0.5f);
+      q = min(max(q, 0), nrows - 1);
+      int oq = q * T - q * (q - 1) / 2;
+      while (oq > p) {  // at most one step: the float root is within one row
+        --q;
+        oq -= T - q;
+      }
+      while (q + 1 < nrows && oq + (T - q) <= p) {
+        oq += T - q;
+        ++q;
+      }
+      const int t = q + (p - oq);
+      const unsigned p0 = F.col_p0[t], p1 = F.col_p1[t];
+      const uint2 rp = S.u.row_pk[q];
+      double cc, l2;
+      wave_contrib(lib, L, S, F, p0, rp, cc, l2);
+      if (p1 != ~0u) {
+        double cc1, l21;
+        wave_contrib(lib, L, S, F, p1, rp, cc1, l21);
+        cc += cc1;
+        l2 += l21;
+      }
+      write_entry(o, seg0 + p, F.col_c[t], cc, l2);
     }
     wave_sync();
   }
@@ -1140,6 +1190,8 @@ struct fem2d_ctx {
   double* h_glq = nullptr;        // pinned Gauss-Legendre staging
   fem2d_timing timing{};
   int n_cu = 0;
+  int count_grid = 0;  // resident blocks of the wave kernels on this device
+  int fill_grid = 0;
 };
 
 namespace {
@@ -1271,6 +1323,13 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   ctx->n_cu = prop.multiProcessorCount;
+  // the wave kernels assign row groups statically: launch exactly the blocks
+  // that are resident at once
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave, kBlock, 0));
+  ctx->count_grid = std::max(1, per_cu) * ctx->n_cu;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_wave, kBlock, 0));
+  ctx->fill_grid = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fill_large),
@@ -1443,7 +1502,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       (rc = slot(ctx, k++, size_t(ne + 1), &w.mem_ptr)) || (rc = slot(ctx, k++, B1, &w.mem_group)) ||
       (rc = slot(ctx, k++, B1, &w.mem_bsoff)) || (rc = slot(ctx, k++, N1, &w.row_len)) ||
       (rc = slot(ctx, k++, N1, &w.row_ptr)) || (rc = slot(ctx, k++, size_t(4 * kMaxGlq), &glq_dev)) ||
-      (rc = slot(ctx, k++, size_t(2), &w.err)) || (rc = slot(ctx, k++, N1, &w.large_list)) ||
+      (rc = slot(ctx, k++, size_t(8), &w.err)) || (rc = slot(ctx, k++, N1, &w.large_list)) ||
       (rc = slot(ctx, k++, N1, &w.is_large)))
     return rc;
   w.n_large = w.err + 1;
@@ -1488,9 +1547,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   v.bs_i = dd->bs_i;
   v.bs_j = dd->bs_j;
   v.bs_dof = dd->bs_dof;
+  v.p1 = L.p1;
 
   const int B = 256;
-  const unsigned wave_grid = unsigned(std::max(1, ctx->n_cu * 8));
   const unsigned large_grid = unsigned(std::max(1, ctx->n_cu * 2));
   HIP_TRY(hipEventRecord(ctx->ev[0], st));
   HIP_TRY(hipMemcpyAsync(glq_dev, ctx->h_glq, 4 * kMaxGlq * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1511,7 +1570,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
 
   // symbolic: row lengths per row group, then row_ptr
   HIP_TRY(hipEventRecord(ctx->ev[4], st));
-  k_count_wave<<<wave_grid, kBlock, 0, st>>>(v, w);
+  k_count_wave<<<unsigned(ctx->count_grid), kBlock, 0, st>>>(v, w);
   HIP_TRY(hipEventRecord(ctx->ev[5], st));
   k_count_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w);
   HIP_TRY(hipGetLastError());
@@ -1532,7 +1591,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     o.a_is_cc = ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
     o.b_is_cc = bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
     HIP_TRY(hipEventRecord(ctx->ev[6], st));
-    if (o.capacity > 0) k_fill_wave<<<wave_grid, kBlock, 0, st>>>(v, w, L, o);
+    if (o.capacity > 0) k_fill_wave<<<unsigned(ctx->fill_grid), kBlock, 0, st>>>(v, w, L, o);
     HIP_TRY(hipEventRecord(ctx->ev[7], st));
     if (o.capacity > 0) k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
     HIP_TRY(hipGetLastError());
