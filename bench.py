@@ -81,6 +81,47 @@ def cpu_baseline(w, total_pairs: int, nnz: int, target_s: float):
     }
 
 
+def profiled_traffic(workload: str):
+    """HBM bytes per launch of the fill kernel on this workload, from the committed rocprofv3
+    --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a 1 GiB copy); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"k_fill_wave_{workload}.md")
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                return int(ln.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
+def host_boundary(w, nnz: int, calls: int = 3):
+    """The drop-in entry point with host arrays on both sides: view upload, assembly and the
+    CSR download into host memory (PCIe-inclusive)."""
+    import ctypes as C
+
+    import fem_2d_amd as F
+    from fem_2d_amd import _native as N
+
+    lib = N.load()
+    ctx = F.Context(0)
+    view = w.domain.view()
+
+    def call():
+        out = N.GepCsr()
+        F.errors.check(lib.fem2d_galerkin_sample_gep_hcurl(ctx._h, C.byref(view), w.basis.code, F.CurlCurl.code,
+                                                           F.L2Inner.code, w.glq[0], w.glq[1], 0, C.byref(out)))
+        lib.fem2d_gep_csr_free(C.byref(out))
+
+    call()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        call()
+    dt = (time.perf_counter() - t0) / calls
+    ctx.close()
+    return {"value": nnz / dt, "unit": "nonzeros/s", "ms_per_call": dt * 1e3,
+            "entry_point": "fem2d_galerkin_sample_gep_hcurl (host view in, host CSR out)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="waveguide_64x32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-boundary", action="store_true",
+                    help="skip timing the host-array entry point (upload + assembly + download)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for scaling HBM byte counters)")
@@ -196,6 +239,9 @@ def main():
             "algorithmic_bytes_per_launch": BYTES_PER_ENTRY * nnz,
         },
     }
+    line["roofline"]["traffic"] = profiled_traffic(w.name)
+    if not args.no_host_boundary:
+        line["host_boundary"] = host_boundary(w, nnz)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, pairs, nnz, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -115,9 +115,12 @@ struct DevView {
 
 struct Work {
   ElemGeo* geo;
-  int* first_child;
-  int* max_child;
-  int* n_child;
+  int* depth;        // ancestors of each Elem
+  int* n_desc;       // descendants of each Elem
+  int* desc_cursor;  // fill cursor of each Elem's descendant part
+  int* nested_len;   // [E+1] 1 + depth + n_desc
+  int* nested_ptr;   // [E+1] exclusive scan of nested_len
+  int* nested;       // per Elem: its descendants (any order), itself, its ancestors (parent first); X << 2 | rel
   int4* dyad;  // (du, iu, dv, iv): position of the Elem inside its base Elem
   long long* slot0;  // per DoF: (elem << 32 | index) of its basis specs
   long long* slot1;
@@ -140,12 +143,16 @@ struct Work {
   int64_t* row_len;  // [N+1]
   int64_t* row_ptr;  // [N+1]
   double* lib;
-  const double* glq;  // u points, u weights, v points, v weights (kMaxGlq each)
   int* err;
   int* n_large;       // row groups handled by the block path
   int* large_list;
   unsigned char* is_large;
   long long n_dofs;
+};
+
+// Gauss-Legendre nodes and weights per axis, passed by value to the library kernel
+struct GlqArgs {
+  double u_pts[kMaxGlq], u_wts[kMaxGlq], v_pts[kMaxGlq], v_wts[kMaxGlq];
 };
 
 struct LibShape {
@@ -340,9 +347,9 @@ __device__ __forceinline__ void pair_values(const double* __restrict__ lib, cons
 __global__ void k_init(Work w, int n_elems, long long n_dofs, long long n_bs) {
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t < n_elems) {
-    w.first_child[t] = 0x7fffffff;
-    w.max_child[t] = -1;
-    w.n_child[t] = 0;
+    w.depth[t] = 0;
+    w.n_desc[t] = 0;
+    w.desc_cursor[t] = 0;
   }
   if (t < n_dofs) {
     w.slot0[t] = -1;
@@ -379,12 +386,9 @@ __global__ void k_elem_prep(DevView v, Work w) {
     locs[depth++] = v.loc[x];
     x = p;
   }
-  if (v.parent[e] >= 0) {
-    const int p = v.parent[e];
-    atomicMin(&w.first_child[p], e);
-    atomicMax(&w.max_child[p], e);
-    atomicAdd(&w.n_child[p], 1);
-  }
+  // the chain is valid: count this Elem as a descendant of each ancestor
+  w.depth[e] = depth;
+  for (int a = v.parent[e], i = 0; i < depth; ++i, a = v.parent[a]) atomicAdd(&w.n_desc[a], 1);
   double u0 = -1.0, u1 = 1.0, v0 = -1.0, v1 = 1.0;
   int du = 0, iu = 0, dv = 0, iv = 0;
   for (int t = depth - 1; t >= 0; --t) {
@@ -431,13 +435,26 @@ __global__ void k_elem_prep(DevView v, Work w) {
   w.geo[e] = g;
 }
 
-// children of an Elem must have consecutive ids (h_refinement.rs:119-134)
-__global__ void k_child_check(DevView v, Work w) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// nested-list length of each Elem: itself, its ancestors, its descendants
+__global__ void k_nested_len(Work w, int n_elems) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n_elems) w.nested_len[e] = 1 + w.depth[e] + w.n_desc[e];
+  if (e == n_elems) w.nested_len[e] = 0;
+}
+
+// Each Elem writes itself and its ancestors into its own list (after the
+// descendant part) and itself into each ancestor's descendant part.
+__global__ void k_nested_fill(DevView v, Work w) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= v.n_elems) return;
-  const int n = w.n_child[e];
-  if (n == 0) return;
-  if (!(n == 2 || n == 4) || w.max_child[e] - w.first_child[e] + 1 != n) set_err(w.err, kErrView);
+  const int depth = w.depth[e];
+  const int own = w.nested_ptr[e] + w.n_desc[e];
+  w.nested[own] = (e << 2) | kLocal;
+  for (int a = v.parent[e], i = 0; i < depth; ++i, a = v.parent[a]) {
+    w.nested[own + 1 + i] = (a << 2) | kAnc;
+    const int slot = atomicAdd(&w.desc_cursor[a], 1);
+    w.nested[w.nested_ptr[a] + slot] = (e << 2) | kDesc;
+  }
 }
 
 // Per basis spec: owning Elem, packed (dir, i, j), and the DoF's spec slots.
@@ -549,7 +566,7 @@ __global__ void k_mem_fill(DevView v, Work w) {
 
 // 1D Gram library: lib[axis][range][kind][a][b] = sum_m w_m f_a(x'_m) g_b(x_m)
 // with x'_m = x_m * s + c mapped into the dyadic sub-range (glq.rs:238-249).
-__global__ void k_gram_lib(Work w, LibShape L) {
+__global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
   const long long per_axis = (long long)L.nr * kKinds * L.p1 * L.p1;
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= 2 * per_axis) return;
@@ -564,8 +581,8 @@ __global__ void k_gram_lib(Work w, LibShape L) {
   double s, c;
   range_of(rid, s, c);
   const int np = axis == 0 ? L.nu : L.nv;
-  const double* pts = w.glq + (axis == 0 ? 0 : 2 * kMaxGlq);
-  const double* wts = pts + kMaxGlq;
+  const double* pts = axis == 0 ? glq.u_pts : glq.v_pts;
+  const double* wts = axis == 0 ? glq.u_wts : glq.v_wts;
   // which shape of p (first) and q (second): 0 N, 1 N', 2 T, 3 T'
   int fp = 0, fq = 0;
   switch (kind) {
@@ -588,41 +605,23 @@ __global__ void k_gram_lib(Work w, LibShape L) {
 // Nested-Elem sources of a row group (shared by both paths)
 // ---------------------------------------------------------------------------
 
-// Enumerates the Elems nested with each Elem of row group g: the Elem itself,
-// its ancestors and its descendants (breadth first). Sequential; returns the
-// count, or -1 if more than cap.
-__device__ int enumerate_sources(const DevView& v, const Work& w, int g, int cap, int* xs, int* info) {
-  int n = 0;
-  const int e_k[2] = {w.g_e1[g], w.g_e2[g]};
-  for (int k = 0; k < 2; ++k) {
-    const int E = e_k[k];
-    if (E < 0) break;
-    if (n >= cap) return -1;
-    xs[n] = E;
-    info[n] = k | (kLocal << 1);
-    ++n;
-    for (int a = v.parent[E]; a >= 0; a = v.parent[a]) {
-      if (n >= cap) return -1;
-      xs[n] = a;
-      info[n] = k | (kAnc << 1);
-      ++n;
-    }
-    // descendants: breadth first with the source list as the queue
-    const int q0 = n;
-    int head = -1;
-    while (true) {
-      const int pe = head < 0 ? E : xs[head];
-      const int nc = w.n_child[pe];
-      const int fc = w.first_child[pe];
-      for (int c = 0; c < nc; ++c) {
-        if (n >= cap) return -1;
-        xs[n] = fc + c;
-        info[n] = k | (kDesc << 1);
-        ++n;
-      }
-      head = head < 0 ? q0 : head + 1;
-      if (head >= n) break;
-    }
+// The sources of row group g: the Elems nested with its first Elem (k = 0)
+// and with its second Elem (k = 1), read from the nested lists. Returns the
+// count; thread t < count of the calling group (stride nthreads) stores source t.
+__device__ __forceinline__ int gather_sources(const Work& w, int g, int cap, int* xs, int* info, int t,
+                                              int nthreads) {
+  const int e1 = w.g_e1[g], e2 = w.g_e2[g];
+  if (e1 < 0) return 0;  // a DoF without basis specs (flagged as an invalid view)
+  const int a1 = w.nested_ptr[e1], n1 = w.nested_ptr[e1 + 1] - a1;
+  const int a2 = e2 >= 0 ? w.nested_ptr[e2] : 0;
+  const int n2 = e2 >= 0 ? w.nested_ptr[e2 + 1] - a2 : 0;
+  const int n = n1 + n2;
+  if (n > cap) return n;
+  for (int s = t; s < n; s += nthreads) {
+    const int k = s < n1 ? 0 : 1;
+    const int x = w.nested[k == 0 ? a1 + s : a2 + (s - n1)];
+    xs[s] = x >> 2;
+    info[s] = k | ((x & 3) << 1);
   }
   return n;
 }
@@ -680,13 +679,10 @@ struct WaveFill {
 // per-source prefactors are computed as well.
 __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S, int lane, double (*cA)[4],
                           double (*cB)[2]) {
-  if (lane == 0) {
-    S.n_src = enumerate_sources(v, w, g, kWS, S.src_x, S.src_info);
-    S.u_self = -1;
-  }
+  const int n_src = gather_sources(w, g, kWS, S.src_x, S.src_info, lane, kWave);
+  if (n_src > kWS) return 1;
+  if (lane == 0) S.u_self = -1;
   wave_sync();
-  const int n_src = S.n_src;
-  if (n_src < 0) return 1;
   // membership counts -> gather offsets (n_src <= 32)
   int cnt = 0, X = -1;
   if (lane < n_src) {
@@ -801,6 +797,7 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
 }
 
 __global__ void __launch_bounds__(kBlock) k_count_wave(DevView v, Work w) {
+  if (*w.err) return;  // an invalid view was detected: nothing below may trust it
   __shared__ WaveShared shared[kWavesPerBlock];
   const int lane = threadIdx.x % kWave;
   const int wid = threadIdx.x / kWave;
@@ -827,7 +824,7 @@ __global__ void __launch_bounds__(kBlock) k_count_wave(DevView v, Work w) {
 }
 
 __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int c, double cc, double l2) {
-  if (pos < o.capacity) {
+  if (pos >= 0 && pos < o.capacity) {
     o.col[pos] = c;
     o.aval[pos] = o.a_is_cc ? cc : l2;
     o.bval[pos] = o.b_is_cc ? cc : l2;
@@ -849,12 +846,42 @@ __device__ __forceinline__ void wave_contrib(const double* __restrict__ lib, con
   pair_values(lib, L, F.cA[s], F.cB[s], ru, rv, pbs, qbs, cc, l2);
 }
 
+// Entry p of a row group's segment: row q = max{q : off(q) <= p} with
+// off(q) = q T - q (q - 1) / 2, candidate column t = q + p - off(q).
+__device__ __forceinline__ void segment_entry(int p, int T, int nrows, float bq, const WaveShared& S,
+                                              const WaveFill& F, const double* __restrict__ lib, const LibShape& L,
+                                              int& c, double& cc, double& l2) {
+  int q = int((bq - sqrtf(fmaxf(bq * bq - 8.0f * float(p), 0.0f))) * 0.5f);
+  q = min(max(q, 0), nrows - 1);
+  int oq = q * T - q * (q - 1) / 2;
+  while (oq > p) {  // at most one step: the float root is within one row
+    --q;
+    oq -= T - q;
+  }
+  while (q + 1 < nrows && oq + (T - q) <= p) {
+    oq += T - q;
+    ++q;
+  }
+  const int t = q + (p - oq);
+  const unsigned p0 = F.col_p0[t], p1 = F.col_p1[t];
+  const uint2 rp = S.u.row_pk[q];
+  c = F.col_c[t];
+  wave_contrib(lib, L, S, F, p0, rp, cc, l2);
+  if (p1 != ~0u) {
+    double cc1, l21;
+    wave_contrib(lib, L, S, F, p1, rp, cc1, l21);
+    cc += cc1;
+    l2 += l21;
+  }
+}
+
 // One wavefront per row group: build the candidate set, stage the group's row
 // specs and its candidate columns in LDS, then write the group's contiguous
 // CSR segment with consecutive lanes on consecutive entries. Entry p of the
 // segment lies in row q = max{q : off(q) <= p}, off(q) = q T - q (q - 1) / 2,
 // where T is the first row's length; its candidate column is q + p - off(q).
 __global__ void __launch_bounds__(kBlock) k_fill_wave(DevView v, Work w, LibShape L, FillOut o) {
+  if (*w.err) return;  // an invalid view was detected: nothing below may trust it
   __shared__ WaveShared shared[kWavesPerBlock];
   __shared__ WaveFill fill[kWavesPerBlock];
   const int lane = threadIdx.x % kWave;
@@ -901,30 +928,15 @@ __global__ void __launch_bounds__(kBlock) k_fill_wave(DevView v, Work w, LibShap
     const long long seg0 = w.row_ptr[r0];
     const int seg_n = nrows * T - nrows * (nrows - 1) / 2;
     const float bq = 2.0f * float(T) + 1.0f;
-    for (int p = lane; p < seg_n; p += kWave) {
-      int q = int((bq - sqrtf(fmaxf(bq * bq - 8.0f * float(p), 0.0f))) * 0.5f);
-      q = min(max(q, 0), nrows - 1);
-      int oq = q * T - q * (q - 1) / 2;
-      while (oq > p) {  // at most one step: the float root is within one row
-        --q;
-        oq -= T - q;
-      }
-      while (q + 1 < nrows && oq + (T - q) <= p) {
-        oq += T - q;
-        ++q;
-      }
-      const int t = q + (p - oq);
-      const unsigned p0 = F.col_p0[t], p1 = F.col_p1[t];
-      const uint2 rp = S.u.row_pk[q];
-      double cc, l2;
-      wave_contrib(lib, L, S, F, p0, rp, cc, l2);
-      if (p1 != ~0u) {
-        double cc1, l21;
-        wave_contrib(lib, L, S, F, p1, rp, cc1, l21);
-        cc += cc1;
-        l2 += l21;
-      }
-      write_entry(o, seg0 + p, F.col_c[t], cc, l2);
+    // two entries per lane and iteration: their LDS and library loads overlap
+    for (int p = lane; p < seg_n; p += 2 * kWave) {
+      int c0, c1 = 0;
+      double cc0, l20, cc1 = 0.0, l21 = 0.0;
+      segment_entry(p, T, nrows, bq, S, F, lib, L, c0, cc0, l20);
+      const bool second = p + kWave < seg_n;
+      if (second) segment_entry(p + kWave, T, nrows, bq, S, F, lib, L, c1, cc1, l21);
+      write_entry(o, seg0 + p, c0, cc0, l20);
+      if (second) write_entry(o, seg0 + p + kWave, c1, cc1, l21);
     }
     wave_sync();
   }
@@ -959,11 +971,15 @@ static_assert(sizeof(GroupShared) <= 64 * 1024, "row-group LDS footprint must fi
 template <bool kWithCoef>
 __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared& S) {
   const int tid = threadIdx.x;
+  const int n = gather_sources(w, g, kMaxSrc, S.src_x, S.src_info, tid, blockDim.x);
+  if (n > kMaxSrc) {  // block-uniform
+    if (tid == 0) set_err(w.err, kErrSize);
+    return false;
+  }
+  __syncthreads();
   if (tid == 0) {
     S.fail = 0;
-    const int n = enumerate_sources(v, w, g, kMaxSrc, S.src_x, S.src_info);
-    if (n < 0) S.fail = 1;
-    S.n_src = n < 0 ? 0 : n;
+    S.n_src = n;
     int tot = 0;
     for (int s = 0; s < S.n_src; ++s) {
       S.raw_cnt[s] = tot;
@@ -1070,6 +1086,7 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
 }
 
 __global__ void __launch_bounds__(kBlock) k_count_large(DevView v, Work w) {
+  if (*w.err) return;  // an invalid view was detected: nothing below may trust it
   extern __shared__ unsigned char smem[];
   GroupShared& S = *reinterpret_cast<GroupShared*>(smem);
   const int n_large = *w.n_large;
@@ -1088,6 +1105,7 @@ __global__ void __launch_bounds__(kBlock) k_count_large(DevView v, Work w) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibShape L, FillOut o) {
+  if (*w.err) return;  // an invalid view was detected: nothing below may trust it
   extern __shared__ unsigned char smem[];
   GroupShared& S = *reinterpret_cast<GroupShared*>(smem);
   const int64_t* __restrict__ row_ptr = w.row_ptr;
@@ -1147,6 +1165,15 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
   }
 }
 
+// Status of the call into the context's mapped pinned block: nnz, error bits,
+// large-group count, row groups. The host reads it after the stream sync.
+__global__ void k_finish(Work w, long long* status) {
+  status[0] = w.row_ptr[w.n_dofs];
+  status[1] = *w.err;
+  status[2] = *w.n_large;
+  status[3] = n_groups_of(w);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -1166,6 +1193,7 @@ struct fem2d_device_domain {
   long long n_bs = 0;
   int max_i = 0, max_j = 0;
   int max_du = 0, max_dv = 0;
+  long long nested_cap = 0;  // sum over Elems of 1 + 2 * depth: entries of the nested lists
   // device arrays
   int* parent = nullptr;
   unsigned char* loc = nullptr;
@@ -1186,8 +1214,9 @@ struct fem2d_ctx {
   DevBuf scan_tmp;
   DevBuf out_col, out_a, out_b;
   long long out_capacity = 0;
-  long long* h_pinned = nullptr;  // readback staging: nnz, err + n_large, n_groups
-  double* h_glq = nullptr;        // pinned Gauss-Legendre staging
+  long long* h_status = nullptr;  // mapped pinned status block: nnz, err, n_large, n_groups
+  long long* d_status = nullptr;  // its device address
+  GlqArgs glq{};
   fem2d_timing timing{};
   int n_cu = 0;
   int count_grid = 0;  // resident blocks of the wave kernels on this device
@@ -1318,8 +1347,8 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   ctx->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
-  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_pinned), 64, hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_glq), 4 * kMaxGlq * sizeof(double), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_status), 64, hipHostMallocMapped));
+  HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->d_status), ctx->h_status, 0));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   ctx->n_cu = prop.multiProcessorCount;
@@ -1348,8 +1377,7 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
     if (b->p) (void)hipFree(b->p);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
-  if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
-  if (ctx->h_glq) (void)hipHostFree(ctx->h_glq);
+  if (ctx->h_status) (void)hipHostFree(ctx->h_status);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1429,6 +1457,7 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
     }
     dd->max_du = std::max(dd->max_du, du);
     dd->max_dv = std::max(dd->max_dv, dv);
+    dd->nested_cap += 1 + 2LL * steps;
   }
   int rc = FEM2D_OK;
   std::vector<long long> off(v->bs_offsets, v->bs_offsets + ne + 1);
@@ -1487,10 +1516,11 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   w.n_dofs = nd;
   int rc = FEM2D_OK;
   int k = 0;
-  double* glq_dev = nullptr;
   const size_t E1 = size_t(std::max(ne, 1)), N1 = size_t(nd + 1), B1 = size_t(nbs + 1);
-  if ((rc = slot(ctx, k++, E1, &w.geo)) || (rc = slot(ctx, k++, E1, &w.first_child)) ||
-      (rc = slot(ctx, k++, E1, &w.max_child)) || (rc = slot(ctx, k++, E1, &w.n_child)) ||
+  if ((rc = slot(ctx, k++, E1, &w.geo)) || (rc = slot(ctx, k++, E1, &w.depth)) ||
+      (rc = slot(ctx, k++, E1, &w.n_desc)) || (rc = slot(ctx, k++, E1, &w.desc_cursor)) ||
+      (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_len)) || (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_ptr)) ||
+      (rc = slot(ctx, k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) ||
       (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, N1, &w.slot0)) ||
       (rc = slot(ctx, k++, N1, &w.slot1)) || (rc = slot(ctx, k++, B1, &w.bs_elem)) ||
       (rc = slot(ctx, k++, B1, &w.bs_pack)) || (rc = slot(ctx, k++, N1, &w.dof_e1)) ||
@@ -1501,12 +1531,11 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       (rc = slot(ctx, k++, B1, &w.mem_flag)) || (rc = slot(ctx, k++, B1, &w.mem_idx)) ||
       (rc = slot(ctx, k++, size_t(ne + 1), &w.mem_ptr)) || (rc = slot(ctx, k++, B1, &w.mem_group)) ||
       (rc = slot(ctx, k++, B1, &w.mem_bsoff)) || (rc = slot(ctx, k++, N1, &w.row_len)) ||
-      (rc = slot(ctx, k++, N1, &w.row_ptr)) || (rc = slot(ctx, k++, size_t(4 * kMaxGlq), &glq_dev)) ||
+      (rc = slot(ctx, k++, N1, &w.row_ptr)) || 
       (rc = slot(ctx, k++, size_t(8), &w.err)) || (rc = slot(ctx, k++, N1, &w.large_list)) ||
       (rc = slot(ctx, k++, N1, &w.is_large)))
     return rc;
   w.n_large = w.err + 1;
-  w.glq = glq_dev;
 
   LibShape L;
   L.p1 = std::max(dd->max_i, dd->max_j) + 1;
@@ -1525,14 +1554,16 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   tmp_bytes = std::max(tmp_bytes, t2);
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.row_len, w.row_ptr, int(nd + 1), ctx->stream));
   tmp_bytes = std::max(tmp_bytes, t2);
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.nested_len, w.nested_ptr, ne + 1, ctx->stream));
+  tmp_bytes = std::max(tmp_bytes, t2);
   if ((rc = ensure(ctx->scan_tmp, tmp_bytes))) return rc;
 
   hipStream_t st = ctx->stream;
   // the previous call's readback of the pinned staging buffers has completed
   // (every call ends with a stream synchronisation)
-  std::fill(ctx->h_glq, ctx->h_glq + 4 * kMaxGlq, 0.0);
-  fem2d::gauss_legendre_nodes(nu, ctx->h_glq, ctx->h_glq + kMaxGlq);
-  fem2d::gauss_legendre_nodes(nv, ctx->h_glq + 2 * kMaxGlq, ctx->h_glq + 3 * kMaxGlq);
+  ctx->glq = GlqArgs{};
+  fem2d::gauss_legendre_nodes(nu, ctx->glq.u_pts, ctx->glq.u_wts);
+  fem2d::gauss_legendre_nodes(nv, ctx->glq.v_pts, ctx->glq.v_wts);
 
   DevView v;
   v.n_elems = ne;
@@ -1552,10 +1583,11 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   const int B = 256;
   const unsigned large_grid = unsigned(std::max(1, ctx->n_cu * 2));
   HIP_TRY(hipEventRecord(ctx->ev[0], st));
-  HIP_TRY(hipMemcpyAsync(glq_dev, ctx->h_glq, 4 * kMaxGlq * sizeof(double), hipMemcpyHostToDevice, st));
   k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd, nbs);
   k_elem_prep<<<grid_for(ne, B), B, 0, st>>>(v, w);
-  k_child_check<<<grid_for(ne, B), B, 0, st>>>(v, w);
+  k_nested_len<<<grid_for(ne + 1, B), B, 0, st>>>(w, ne);
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.nested_len, w.nested_ptr, ne + 1, st));
+  k_nested_fill<<<grid_for(ne, B), B, 0, st>>>(v, w);
   k_bs_slots<<<grid_for(nbs, B), B, 0, st>>>(v, w);
   k_dof_flags<<<grid_for(nd, B), B, 0, st>>>(w);
   HIP_TRY(hipGetLastError());
@@ -1564,7 +1596,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   k_mem_flags<<<grid_for(nbs, B), B, 0, st>>>(v, w);
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.mem_flag, w.mem_idx, int(nbs + 1), st));
   k_mem_fill<<<grid_for(std::max<long long>(nbs, ne + 1), B), B, 0, st>>>(v, w);
-  k_gram_lib<<<grid_for(lib_n, B), B, 0, st>>>(w, L);
+  k_gram_lib<<<grid_for(lib_n, B), B, 0, st>>>(w, L, ctx->glq);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[1], st));
 
@@ -1595,16 +1627,14 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     HIP_TRY(hipEventRecord(ctx->ev[7], st));
     if (o.capacity > 0) k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 0, w.row_ptr + nd, sizeof(long long), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 1, w.err, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ctx->h_pinned + 2, w.dof_grp + (nd - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    k_finish<<<1, 1, 0, st>>>(w, ctx->d_status);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[3], st));
     HIP_TRY(hipStreamSynchronize(st));
-    int err_pair[2];
-    std::memcpy(err_pair, ctx->h_pinned + 1, sizeof(err_pair));
-    if (err_pair[0]) return err_to_status(err_pair[0]);
-    nnz = ctx->h_pinned[0];
-    n_large = err_pair[1];
+    const volatile long long* stat = ctx->h_status;
+    if (stat[1]) return err_to_status(stat[1]);
+    nnz = stat[0];
+    n_large = int(stat[2]);
     if (nnz <= ctx->out_capacity) break;
     if (attempt == 1) {
       set_error("output capacity mismatch after growing the buffers", 0);
@@ -1631,9 +1661,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   ctx->timing.count_kernel_ms = tc;
   ctx->timing.fill_kernel_ms = tf;
   ctx->timing.nnz = nnz;
-  int ng = 0;
-  std::memcpy(&ng, ctx->h_pinned + 2, sizeof(int));
-  ctx->timing.n_groups = ng;
+  ctx->timing.n_groups = ctx->h_status[3];
   ctx->timing.n_large = n_large;
 
   out->n = nd;

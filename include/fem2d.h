@@ -86,9 +86,7 @@ enum fem2d_href_kind { FEM2D_HREF_T = 0, FEM2D_HREF_U = 1, FEM2D_HREF_V = 2 };
  * ------------------------------------------------------------------------- */
 
 /* Flattened view of a reference Domain (domain.rs:42-50). Arrays are owned by
- * the caller and only read during the call. Elem ids are array indices; the
- * children of an Elem must have consecutive ids, as every h-refinement of the
- * reference assigns them (h_refinement.rs:119-134). */
+ * the caller and only read during the call. Elem ids are array indices. */
 typedef struct fem2d_domain_view {
   int32_t cc;        /* Domain::cc */
   int32_t n_elems;   /* mesh.elems.len() */

@@ -44,7 +44,7 @@ def load():
         lib.oracle_default_ngq.argtypes = [C.c_int]
         lib.oracle_default_ngq.restype = C.c_int
         lib.oracle_assemble.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                        C.c_int64, C.POINTER(OracleGep)]
+                                        C.c_int64, C.POINTER(C.c_uint8), C.POINTER(OracleGep)]
         lib.oracle_assemble.restype = C.c_int
         lib.oracle_gep_free.argtypes = [C.POINTER(OracleGep)]
         _lib = lib
@@ -102,13 +102,19 @@ class OracleResult:
 
 
 def assemble(view, basis: int = 0, a_integral: int = 0, b_integral: int = 1, glq=(8, 8), n_threads: int = 1,
-             elem_limit: int = -1):
+             elem_limit: int = -1, row_mask=None):
     """Reference-algorithm assembly of a fem2d_domain_view (ctypes struct). Returns
-    (status, OracleResult or None); status follows galerkin.rs's error order."""
+    (status, OracleResult or None); status follows galerkin.rs's error order.
+    row_mask (bool array over DoFs, optional): compute only the rows it selects (complete
+    and exact), which keeps full-size domains checkable in seconds."""
     g = OracleGep()
     gu, gv = (0, 0) if glq is None else glq
+    mask = None
+    if row_mask is not None:
+        mask = np.ascontiguousarray(np.asarray(row_mask, dtype=np.uint8))
+        assert mask.size == view.n_dofs
     rc = load().oracle_assemble(C.addressof(view), basis, a_integral, b_integral, gu, gv, n_threads, elem_limit,
-                                C.byref(g))
+                                None if mask is None else mask.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(g))
     if rc != 0:
         return rc, None
     try:

@@ -502,7 +502,9 @@ struct ElemResult {
 };
 
 // One task of the parallel loop over elems (galerkin.rs:73-184)
-static void assemble_elem(const Setup& s, int e, int ai, int bi, ElemResult& out) {
+// row_mask (nullable): only pairs whose entry row min(dof_p, dof_q) is set are
+// integrated, so masked rows come out complete and exact at any domain size.
+static void assemble_elem(const Setup& s, int e, int ai, int bi, const uint8_t* row_mask, ElemResult& out) {
   const fem2d_domain_view* v = s.v;
   const double eps = v->materials[2 * e], mu = v->materials[2 * e + 1];
   BasisFn local = sample_basis_fn(s, e, -1);
@@ -510,6 +512,7 @@ static void assemble_elem(const Setup& s, int e, int ai, int bi, ElemResult& out
   std::vector<std::pair<Key, double>> ea, eb;
   for (int64_t p = b0; p < b1; ++p)
     for (int64_t q = p; q < b1; ++q) {
+      if (row_mask && !row_mask[std::min(v->bs_dof[p], v->bs_dof[q])]) continue;
       const int pd = v->bs_dir[p], qd = v->bs_dir[q];
       double a = integrate(s, ai, pd, qd, v->bs_i[p], v->bs_j[p], v->bs_i[q], v->bs_j[q], local, local, eps, mu);
       double b = integrate(s, bi, pd, qd, v->bs_i[p], v->bs_j[p], v->bs_i[q], v->bs_j[q], local, local, eps, mu);
@@ -535,6 +538,7 @@ static void assemble_elem(const Setup& s, int e, int ai, int bi, ElemResult& out
       const BasisFn& ps = sampled[k].second;
       const BasisFn& ql = desc_local[k];
       for (int64_t q = v->bs_offsets[d]; q < v->bs_offsets[d + 1]; ++q) {
+        if (row_mask && !row_mask[std::min(v->bs_dof[p], v->bs_dof[q])]) continue;
         const int pd = v->bs_dir[p], qd = v->bs_dir[q];
         double a = integrate(s, ai, pd, qd, v->bs_i[p], v->bs_j[p], v->bs_i[q], v->bs_j[q], ps, ql, eps, mu);
         double b = integrate(s, bi, pd, qd, v->bs_i[p], v->bs_j[p], v->bs_i[q], v->bs_j[q], ps, ql, eps, mu);
@@ -596,8 +600,9 @@ int oracle_default_ngq(int max_order) { return oracle::default_ngq(max_order); }
 // (1 wrong cc, 2 empty DoF set, 3 invalid GLQ settings) before any work.
 // n_threads > 1 runs elem tasks in parallel; results are merged in elem order.
 // elem_limit >= 0 restricts the loop to elems [0, elem_limit) (a bounded sample).
+// row_mask (nullable, n_dofs bytes) restricts the result to the rows it sets.
 int oracle_assemble(const fem2d_domain_view* v, int basis, int ai, int bi, int glq_u, int glq_v, int n_threads,
-                    int64_t elem_limit, oracle_gep* out) {
+                    int64_t elem_limit, const uint8_t* row_mask, oracle_gep* out) {
   if (v->cc != FEM2D_CC_HCURL) return FEM2D_ERR_WRONG_CC;
   if (v->n_dofs == 0) return FEM2D_ERR_EMPTY_DOFS;
   const bool given = glq_u > 0 || glq_v > 0;
@@ -620,13 +625,13 @@ int oracle_assemble(const fem2d_domain_view* v, int basis, int ai, int bi, int g
   std::vector<oracle::ElemResult> res(ne);
   const int nt = std::max(1, n_threads);
   if (nt == 1) {
-    for (int e = 0; e < ne; ++e) oracle::assemble_elem(s, e, ai, bi, res[e]);
+    for (int e = 0; e < ne; ++e) oracle::assemble_elem(s, e, ai, bi, row_mask, res[e]);
   } else {
     std::atomic<int> next{0};
     std::vector<std::thread> pool;
     for (int t = 0; t < nt; ++t)
       pool.emplace_back([&]() {
-        for (int e = next++; e < ne; e = next++) oracle::assemble_elem(s, e, ai, bi, res[e]);
+        for (int e = next++; e < ne; e = next++) oracle::assemble_elem(s, e, ai, bi, row_mask, res[e]);
       });
     for (auto& th : pool) th.join();
   }

@@ -37,3 +37,29 @@ def assert_gep_matches(gep, ref, rtol: float = RTOL):
         assert worst[name] <= rtol, (
             f"{name}[{rows[k]},{ref.col[k]}]: got {got[k]!r}, oracle {exp[k]!r}, rel {rel[k]:.3e}")
     return worst
+
+
+def assert_rows_match(gep, ref, mask, rtol: float = RTOL):
+    """The rows selected by mask: pattern bit-exact, values within rtol. ref holds only those
+    rows (the oracle's row-mask mode); the scale floor uses the GPU diagonal for columns
+    outside the mask (those diagonals are checked wherever their row is selected)."""
+    rows = gep.rows()
+    sel = mask[rows]
+    r, c = rows[sel], gep.col[sel]
+    rr = ref.rows()
+    np.testing.assert_array_equal(r, rr)
+    np.testing.assert_array_equal(c, ref.col)
+    worst = {}
+    for name, allv, exp in (("A", gep.a_val, ref.a_val), ("B", gep.b_val, ref.b_val)):
+        diag = np.zeros(gep.n)
+        on = rows == gep.col
+        diag[rows[on]] = np.abs(allv[on])
+        scale = np.maximum(np.abs(exp), np.sqrt(diag[r] * diag[c]))
+        got = allv[sel]
+        err = np.abs(got - exp)
+        rel = np.where(scale > 0, err / np.where(scale > 0, scale, 1.0), np.where(err > 0, np.inf, 0.0))
+        k = int(np.argmax(rel)) if rel.size else 0
+        worst[name] = float(rel[k]) if rel.size else 0.0
+        assert worst[name] <= rtol, (
+            f"{name}[{r[k]},{c[k]}]: got {got[k]!r}, oracle {exp[k]!r}, rel {rel[k]:.3e}")
+    return worst

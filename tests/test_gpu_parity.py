@@ -78,26 +78,60 @@ def test_errors_in_reference_order(ctx):
         F.galerkin_sample_gep_hcurl(dom, (8, 3), ctx=ctx)
 
 
-def test_invalid_view_rejected(ctx):
-    """A view whose DoF ids are out of range is rejected by the device checks, not assembled."""
-    dom, glq = cases.nalg_problem()
-    a = dom.arrays()
-    bad_dof = a["bs_dof"].copy()
-    bad_dof[3] = dom.num_dofs + 5
-    keep = dict(a)
-    v = F.Domain.__new__(F.Domain)
-    view = type(dom.view())()
-    arrs = {}
+def _view_from_arrays(a):
+    """A fem2d_domain_view over (possibly corrupted) copies of Domain.arrays()."""
+    view = F._native.DomainView()
+    keep = {}
     for name, ctype in [("parent", C.c_int32), ("loc", C.c_uint8), ("element_box", C.c_double),
                         ("materials", C.c_double), ("orders", C.c_uint8), ("bs_offsets", C.c_int64),
-                        ("bs_dir", C.c_uint8), ("bs_i", C.c_uint8), ("bs_j", C.c_uint8)]:
-        arrs[name] = np.ascontiguousarray(keep[name].ravel())
-        setattr(view, name, arrs[name].ctypes.data_as(C.POINTER(ctype)))
-    arrs["bs_dof"] = np.ascontiguousarray(bad_dof)
-    view.bs_dof = arrs["bs_dof"].ctypes.data_as(C.POINTER(C.c_int32))
+                        ("bs_dir", C.c_uint8), ("bs_i", C.c_uint8), ("bs_j", C.c_uint8), ("bs_dof", C.c_int32)]:
+        keep[name] = np.ascontiguousarray(a[name].ravel())
+        setattr(view, name, keep[name].ctypes.data_as(C.POINTER(ctype)))
     view.cc = 0
     view.n_elems = a["n_elems"]
     view.n_dofs = a["n_dofs"]
+    return view, keep
+
+
+def _corrupt(kind, a):
+    a = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in a.items()}
+    if kind == "dof_out_of_range":
+        a["bs_dof"][3] = a["n_dofs"] + 5
+    elif kind == "w_direction":
+        a["bs_dir"][5] = 2
+    elif kind == "order_beyond_tables":
+        a["bs_i"][5] = 30
+    elif kind == "three_specs_one_dof":
+        a["bs_dof"][1] = a["bs_dof"][0]
+        a["bs_dof"][2] = a["bs_dof"][0]
+    elif kind == "bad_href_loc":
+        child = int(np.flatnonzero(a["parent"] >= 0)[0])
+        a["loc"][child] = 9
+    elif kind == "parent_cycle":
+        a["parent"][0] = 1
+        a["parent"][1] = 0
+    elif kind == "split_group":
+        # swap the DoFs of two specs of one Elem that belong to different row groups
+        off = a["bs_offsets"]
+        e = int(np.argmax(np.diff(off)))
+        lo, hi = int(off[e]), int(off[e + 1]) - 1
+        a["bs_dof"][lo], a["bs_dof"][hi] = a["bs_dof"][hi], a["bs_dof"][lo]
+    else:
+        raise ValueError(kind)
+    return a
+
+
+@pytest.mark.parametrize("kind", ["dof_out_of_range", "w_direction", "order_beyond_tables", "three_specs_one_dof",
+                                  "bad_href_loc", "parent_cycle", "split_group"])
+def test_invalid_view_rejected(ctx, kind):
+    """Views that violate the Domain invariants are rejected with a status code (device checks
+    or host checks), never assembled and never allowed to address memory outside the inputs."""
+    print(f"[invalid-view] kind={kind}", flush=True)
+    dom, glq = cases.mixed_refinements()
+    view, keep = _view_from_arrays(_corrupt(kind, dom.arrays()))
     out = F._native.GepCsr()
     rc = F._native.load().fem2d_galerkin_sample_gep_hcurl(ctx._h, C.byref(view), 0, 0, 1, 8, 8, 0, C.byref(out))
-    assert rc == F._native.ERR_INVALID_VIEW
+    assert rc in (F._native.ERR_INVALID_VIEW, F._native.ERR_UNSUPPORTED), (rc, F._native.last_error())
+    # the context stays usable
+    gep = F.galerkin_sample_gep_hcurl(dom, glq, F.HierPoly, F.CurlCurl, F.L2Inner, ctx=ctx)
+    assert gep.nnz > 0
