@@ -214,20 +214,37 @@ __device__ void hier_poly(int n, double x, double& N, double& Nd, double& T, dou
 // q_{n,k} = -(2k+1) / (n(n-1)/2) for k < n, k = n (mod 2); c_2..c_8 are the
 // Euclidean normalisation constants of hierarchical_basis_fns.rs:206-209.
 __device__ void kol(int n, double x, double& N, double& Nd, double& T, double& Td) {
-  double L[kMaxOrder + 1], Ld[kMaxOrder + 1];
-  for (int i = 0; i <= n; ++i) {
-    const double fi = double(i);
-    if (i == 0) {
-      L[i] = 1.0; Ld[i] = 0.0;
-    } else if (i == 1) {
-      L[i] = x; Ld[i] = 1.0;
-    } else {
-      L[i] = ((2.0 * fi - 1.0) * x * L[i - 1] - (fi - 1.0) * L[i - 2]) / fi;
-      Ld[i] = fi * L[i - 1] + x * Ld[i - 1];
+  // L_k and L'_k are produced by the recurrence and folded into the T sum in
+  // increasing k (no per-thread arrays)
+  const bool bubble = n >= 2;
+  const int den = bubble ? n * (n - 1) / 2 : 1;
+  double lk = 1.0, dk = 0.0;  // L_0, L'_0
+  double lm1 = 0.0;           // L_{k-1}
+  double sv = 0.0, sd = 0.0;
+  for (int k = 0; k <= n; ++k) {
+    if (k == 1) {
+      lm1 = lk;
+      lk = x;
+      dk = 1.0;
+    } else if (k >= 2) {
+      const double fk = double(k);
+      const double ln = ((2.0 * fk - 1.0) * x * lk - (fk - 1.0) * lm1) / fk;
+      const double dn = fk * lk + x * dk;
+      lm1 = lk;
+      lk = ln;
+      dk = dn;
+    }
+    if (bubble) {
+      int num = 0;
+      if (k == n) num = den;
+      else if ((n - k) % 2 == 0) num = -(2 * k + 1);
+      const double wgt = double(num) / double(den);
+      sv += wgt * lk;
+      sd += wgt * dk;
     }
   }
-  N = L[n];
-  Nd = Ld[n];
+  N = lk;
+  Nd = dk;
   if (n == 0) {
     T = 1.0 - x; Td = -1.0;
     return;
@@ -236,18 +253,8 @@ __device__ void kol(int n, double x, double& N, double& Nd, double& T, double& T
     T = 1.0 + x; Td = 1.0;
     return;
   }
-  const double kNorm[7] = {0.968246, 2.561738, 0.838525, 4.248161, 0.816397, 5.882766, 0.808509};
-  const int den = n * (n - 1) / 2;
-  const double c = n - 2 < 7 ? kNorm[n - 2] : 1.0;
-  double sv = 0.0, sd = 0.0;
-  for (int k = 0; k <= n; ++k) {
-    int num = 0;
-    if (k == n) num = den;
-    else if ((n - k) % 2 == 0) num = -(2 * k + 1);
-    const double w = double(num) / double(den);
-    sv += w * L[k];
-    sd += w * Ld[k];
-  }
+  const double c = n == 2 ? 0.968246 : n == 3 ? 2.561738 : n == 4 ? 0.838525 : n == 5 ? 4.248161
+                 : n == 6 ? 0.816397 : n == 7 ? 5.882766 : n == 8 ? 0.808509 : 1.0;
   T = sv * c;
   Td = sd * c;
 }
@@ -567,6 +574,14 @@ __global__ void k_mem_fill(DevView v, Work w) {
 // 1D Gram library: lib[axis][range][kind][a][b] = sum_m w_m f_a(x'_m) g_b(x_m)
 // with x'_m = x_m * s + c mapped into the dyadic sub-range (glq.rs:238-249).
 __global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
+  __shared__ double s_pts[2][kMaxGlq], s_wts[2][kMaxGlq];
+  for (int i = threadIdx.x; i < kMaxGlq; i += blockDim.x) {
+    s_pts[0][i] = glq.u_pts[i];
+    s_wts[0][i] = glq.u_wts[i];
+    s_pts[1][i] = glq.v_pts[i];
+    s_wts[1][i] = glq.v_wts[i];
+  }
+  __syncthreads();
   const long long per_axis = (long long)L.nr * kKinds * L.p1 * L.p1;
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= 2 * per_axis) return;
@@ -581,8 +596,8 @@ __global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
   double s, c;
   range_of(rid, s, c);
   const int np = axis == 0 ? L.nu : L.nv;
-  const double* pts = axis == 0 ? glq.u_pts : glq.v_pts;
-  const double* wts = axis == 0 ? glq.u_wts : glq.v_wts;
+  const double* pts = s_pts[axis];
+  const double* wts = s_wts[axis];
   // which shape of p (first) and q (second): 0 N, 1 N', 2 T, 3 T'
   int fp = 0, fq = 0;
   switch (kind) {
@@ -1222,6 +1237,8 @@ struct fem2d_ctx {
   int count_grid = 0;  // resident blocks of the wave kernels on this device
   int fill_grid = 0;
 };
+
+int fem2d::capi::ctx_device(fem2d_ctx* ctx) { return ctx->device; }
 
 namespace {
 

@@ -3,6 +3,8 @@
 
 #include <string>
 
+struct fem2d_ctx;
+
 namespace fem2d {
 
 // Gauss-Legendre nodes (ascending) and weights on (-1, 1).
@@ -12,6 +14,8 @@ namespace capi {
 void set_error(const std::string& msg, int detail);
 // Maps the in-flight exception to a status code and records its message.
 int translate_exception();
+// HIP device of a context
+int ctx_device(fem2d_ctx* ctx);
 }  // namespace capi
 
 }  // namespace fem2d

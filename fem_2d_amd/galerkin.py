@@ -150,6 +150,26 @@ class GEP:
         for r, c, v in zip(rows, self.col, vals):
             yield (int(r), int(c)), float(v)
 
+    def _as_csr(self) -> N.GepCsr:
+        """A borrowed fem2d_gep_csr over this GEP's arrays (valid while self lives)."""
+        self._keep = [np.ascontiguousarray(x) for x in (self.row_ptr, self.col, self.a_val, self.b_val)]
+        rp, col, a, b = self._keep
+        g = N.GepCsr()
+        g.n, g.nnz = self.n, self.nnz
+        g.row_ptr = rp.ctypes.data_as(C.POINTER(C.c_int64))
+        g.col = col.ctypes.data_as(C.POINTER(C.c_int32))
+        g.a_val = a.ctypes.data_as(C.POINTER(C.c_double))
+        g.b_val = b.ctypes.data_as(C.POINTER(C.c_double))
+        return g
+
+    def print_to_petsc_binary_files(self, dir: str, prefix: str, ctx: Optional["Context"] = None) -> None:
+        """GEP::print_to_petsc_binary_files (linalg.rs:44-52): writes {dir}/tmp/{prefix}_a.dat and
+        {dir}/tmp/{prefix}_b.dat as PETSc AIJ binary matrices (both triangles, big-endian)."""
+        ctx = ctx or default_context()
+        g = self._as_csr()
+        check(N.load().fem2d_gep_csr_print_to_petsc_binary_files(ctx._h, C.byref(g), str(dir).encode(),
+                                                                 str(prefix).encode()))
+
     def to_dense(self):
         """Full symmetric dense (A, B) (sparse_matrix.rs:168-182)."""
         A = np.zeros((self.n, self.n))
@@ -223,6 +243,11 @@ class DeviceDomain:
         check(self._lib.fem2d_assemble_device(self.ctx._h, self._h, BSpace.code, AI.code, BI.code, gu, gv, 0,
                                               C.byref(out)))
         return out
+
+    def print_to_petsc_binary_files(self, dev: N.GepDevice, dir: str, prefix: str) -> None:
+        """PETSc AIJ binary files of a device-resident GEP (see GEP.print_to_petsc_binary_files)."""
+        check(self._lib.fem2d_gep_device_print_to_petsc_binary_files(self.ctx._h, C.byref(dev), str(dir).encode(),
+                                                                     str(prefix).encode()))
 
     def download(self, dev: N.GepDevice) -> GEP:
         out = N.GepCsr()

@@ -161,6 +161,20 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
                           int glq_u, int glq_v, int precision, fem2d_gep_device* out);
 int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d_gep_csr* out);
 
+/* PETSc AIJ binary files of a GEP, replacing GEP::print_to_petsc_binary_files
+ * (linalg.rs:44-52) with AIJMatrixBinary (sparse_matrix.rs:184-265): writes
+ * "{dir}/tmp/{prefix}_a.dat" and "{dir}/tmp/{prefix}_b.dat", each the full
+ * symmetric matrix (both triangles, rows in order, columns ascending) as
+ * big-endian u32 1211216, rows, columns, nnz, row lengths, column indices and
+ * f64 values. The transpose and the byte order are produced on the device.
+ * The device variant takes a GEP as fem2d_assemble_device returns it; the host
+ * variant checks the CSR and uploads it first. FEM2D_ERR_IO if a file cannot
+ * be written (the reference's std::io::Error). */
+int fem2d_gep_device_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep_device* gep, const char* dir,
+                                                 const char* prefix);
+int fem2d_gep_csr_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep_csr* gep, const char* dir,
+                                              const char* prefix);
+
 /* Per-stage device time of the last assembly on this context, measured with
  * HIP events recorded on the context's stream (milliseconds). */
 typedef struct fem2d_timing {
