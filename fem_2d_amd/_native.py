@@ -119,6 +119,7 @@ SIGNATURES = {
     "fem2d_assemble_device": (
         C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepDevice)]),
     "fem2d_gep_device_download": (C.c_int, [vp, P(GepDevice), P(GepCsr)]),
+    "fem2d_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
     "fem2d_gep_device_print_to_petsc_binary_files": (C.c_int, [vp, P(GepDevice), C.c_char_p, C.c_char_p]),
     "fem2d_gep_csr_print_to_petsc_binary_files": (C.c_int, [vp, P(GepCsr), C.c_char_p, C.c_char_p]),
     "fem2d_ctx_last_timing": (C.c_int, [vp, P(Timing)]),

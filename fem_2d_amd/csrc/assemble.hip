@@ -41,9 +41,11 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -64,6 +66,8 @@ constexpr int kWR = 128;           // memberships gathered per row group
 constexpr int kWD = 128;           // distinct row groups in one candidate set
 constexpr int kWC = 256;           // candidate columns from the group's first row on
 constexpr int kWRows = 128;        // rows per row group
+constexpr int kCoef = 8;           // doubles per source record: CurlCurl prefactors (4), L2Inner (2), padding
+constexpr int kChunk = 1024;       // CSR entries per work item of the streaming fill
 // block path (one block per row group)
 constexpr int kMaxSrc = 128;
 constexpr int kMaxRaw = 1024;
@@ -74,6 +78,7 @@ constexpr int kMaxRelDepth = 12;   // dyadic halvings per axis covered by the Gr
 constexpr int kMaxOrder = 20;      // MAX_POLYNOMIAL_ORDER (mesh.rs:42)
 constexpr int kMaxGlq = 64;        // Gauss points per axis
 constexpr int kKinds = 5;          // Gram kinds
+static_assert(kMaxOrder < 32 && kMaxOrder * (kMaxOrder + 1) < 512, "packed spec fields");
 
 enum GramKind { kNN = 0, kNTd = 1, kTdN = 2, kTdTd = 3, kTT = 4 };
 enum Rel { kLocal = 0, kAnc = 1, kDesc = 2 };
@@ -118,14 +123,13 @@ struct Work {
   int* depth;        // ancestors of each Elem
   int* n_desc;       // descendants of each Elem
   int* desc_cursor;  // fill cursor of each Elem's descendant part
-  int* nested_len;   // [E+1] 1 + depth + n_desc
-  int* nested_ptr;   // [E+1] exclusive scan of nested_len
+  int* nested_ptr;   // [E+1] exclusive scan of the lengths 1 + depth + n_desc
   int* nested;       // per Elem: its descendants (any order), itself, its ancestors (parent first); X << 2 | rel
   int4* dyad;  // (du, iu, dv, iv): position of the Elem inside its base Elem
   long long* slot0;  // per DoF: (elem << 32 | index) of its basis specs
   long long* slot1;
   int* bs_elem;       // per basis spec: owning Elem
-  unsigned* bs_pack;  // per basis spec: dir | i << 8 | j << 16
+  unsigned* bs_pack;  // per basis spec, packed by pack_spec
   int* dof_e1;
   int* dof_e2;
   int* dof_k1;
@@ -143,10 +147,24 @@ struct Work {
   int64_t* row_len;  // [N+1]
   int64_t* row_ptr;  // [N+1]
   double* lib;
+  int lib_range_stride;  // library values per range: p1 * p1 * kKinds
+  int lib_nr;            // ranges per axis
   int* err;
   int* n_large;       // row groups handled by the block path
   int* large_list;
   unsigned char* is_large;
+  // per row group, fixed-stride records written by the describe kernel and
+  // streamed by the fill kernel (groups of the block path have T = -1)
+  int4* meta_hdr;      // [G] T (first row's length), rows, sources, unused
+  double* meta_coef;   // [G * kWS * kCoef] per source: CurlCurl (4) and L2Inner (2) prefactors, then
+                       // int4 {x range base, y range base, k | rel << 1, 0} of the library
+  uint4* meta_col;     // [G * kWC] per candidate column t: column, packed spec of the first and second
+                       // contribution (~0 if none), their sources (bits 0-7 and 8-15)
+  uint2* meta_row;     // [G * kWRows] per row: packed specs on the group's first / second Elem
+  int meta_cap;        // row groups the records have room for
+  int64_t* seg_start;  // [G+1] first CSR entry of each row group
+  int* chunk_first;    // [chunk_cap] row group holding the first entry of each chunk
+  long long chunk_cap;
   long long n_dofs;
 };
 
@@ -157,6 +175,8 @@ struct GlqArgs {
 
 struct LibShape {
   int p1;      // orders 0..p1-1 per axis
+  int tab;     // p1 * p1: values per kind and range
+  int range_stride;  // kKinds * tab: values per range
   int nr;      // ranges per axis
   int nu, nv;  // Gauss points per axis
   int basis;
@@ -327,23 +347,45 @@ __device__ __forceinline__ void block_coefs(const Work& w, int P, int Q, bool lo
 // pair = 2 dir_p + dir_q the CurlCurl x-kind is NN, NT', T'N, T'T' = pair and
 // the y-kind is 3 - pair; L2Inner is non-zero for UU (x NN, y TT) and VV
 // (x TT, y NN) only, and shares one factor with CurlCurl.
-__device__ __forceinline__ void pair_values(const double* __restrict__ lib, const LibShape& L, const double* cA,
-                                            const double* cB, int ru, int rv, unsigned pbs, unsigned qbs, double& cc,
-                                            double& l2) {
-  const int pd = pbs & 1, pi = (pbs >> 8) & 0xff, pj = (pbs >> 16) & 0xff;
-  const int qd = qbs & 1, qi = (qbs >> 8) & 0xff, qj = (qbs >> 16) & 0xff;
-  const int tab = L.p1 * L.p1;
-  const double* gx = lib + (long long)ru * (kKinds * tab);
-  const double* gy = lib + ((long long)L.nr + rv) * (kKinds * tab);
-  const int ix = pi * L.p1 + qi, iy = pj * L.p1 + qj;
-  const int pair = 2 * pd + qd;
-  const double fx = gx[pair * tab + ix];
-  const double fy = gy[(3 - pair) * tab + iy];
+// Packed basis spec: dir (1 bit), i (5), j (5), i * p1 (9), j * p1 (9). The
+// products are the row offsets of i and j in a p1 x p1 library table, so a
+// spec pair addresses its table entry without multiplications.
+__host__ __device__ __forceinline__ unsigned pack_spec(unsigned d, unsigned i, unsigned j, int p1) {
+  return (d & 1u) | (i << 1) | (j << 6) | ((i * unsigned(p1)) << 11) | ((j * unsigned(p1)) << 20);
+}
+__device__ __forceinline__ unsigned spec_dir(unsigned x) { return x & 1u; }
+__device__ __forceinline__ unsigned spec_i(unsigned x) { return (x >> 1) & 31u; }
+__device__ __forceinline__ unsigned spec_j(unsigned x) { return (x >> 6) & 31u; }
+__device__ __forceinline__ unsigned spec_ip1(unsigned x) { return (x >> 11) & 511u; }
+__device__ __forceinline__ unsigned spec_jp1(unsigned x) { return (x >> 20) & 511u; }
+
+// element `idx` of a uniform base with a 32-bit byte offset (lets the compiler
+// use the scalar-base + 32-bit vector-offset addressing of global loads)
+template <class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ base, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * unsigned(sizeof(T)));
+}
+
+// CurlCurl and L2Inner contributions of one nested pair (p, q). bx and by are
+// the library's range bases on the x and y axes; values are stored as
+// [axis][range][kind][a][b] with a = p's order and b = q's order. With
+// pair = 2 dir_p + dir_q the CurlCurl x-kind is NN, NT', T'N, T'T' = pair and
+// the y-kind is 3 - pair; L2Inner is non-zero for UU (x NN, y TT) and VV
+// (x TT, y NN) only and shares one factor with CurlCurl.
+__device__ __forceinline__ void pair_values(const double* __restrict__ lib, unsigned tab, const double* cA,
+                                            const double* cB, unsigned bx, unsigned by, unsigned pbs, unsigned qbs,
+                                            double& cc, double& l2) {
+  const unsigned pd = spec_dir(pbs), qd = spec_dir(qbs);
+  const unsigned ix = bx + spec_ip1(pbs) + spec_i(qbs);
+  const unsigned iy = by + spec_jp1(pbs) + spec_j(qbs);
+  const unsigned pair = 2u * pd + qd;
+  const double fx = ld32(lib, ix + __umul24(pair, tab));
+  const double fy = ld32(lib, iy + __umul24(3u - pair, tab));
   cc = cA[pair] * fx * fy;
   l2 = 0.0;
   if (pd == qd) {
-    if (pd == 0) l2 = cB[0] * fx * gy[kTT * tab + iy];
-    else l2 = cB[1] * gx[kTT * tab + ix] * fy;
+    if (pd == 0) l2 = cB[0] * fx * ld32(lib, iy + __umul24(unsigned(kTT), tab));
+    else l2 = cB[1] * ld32(lib, ix + __umul24(unsigned(kTT), tab)) * fy;
   }
 }
 
@@ -443,11 +485,13 @@ __global__ void k_elem_prep(DevView v, Work w) {
 }
 
 // nested-list length of each Elem: itself, its ancestors, its descendants
-__global__ void k_nested_len(Work w, int n_elems) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n_elems) w.nested_len[e] = 1 + w.depth[e] + w.n_desc[e];
-  if (e == n_elems) w.nested_len[e] = 0;
-}
+// (read by the scan that places the lists; 0 past the last Elem)
+struct NestedLen {
+  const int* depth;
+  const int* n_desc;
+  int n_elems;
+  __host__ __device__ int operator()(int e) const { return e < n_elems ? 1 + depth[e] + n_desc[e] : 0; }
+};
 
 // Each Elem writes itself and its ancestors into its own list (after the
 // descendant part) and itself into each ancestor's descendant part.
@@ -464,28 +508,29 @@ __global__ void k_nested_fill(DevView v, Work w) {
   }
 }
 
-// Per basis spec: owning Elem, packed (dir, i, j), and the DoF's spec slots.
+// Per basis spec: owning Elem, packed spec, and the DoF's spec slots (one
+// wavefront per Elem, lanes over its spec list).
 __global__ void k_bs_slots(DevView v, Work w) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= v.n_bs) return;
-  int lo = 0, hi = v.n_elems;  // bs_off[lo] <= t < bs_off[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (v.bs_off[mid] <= t) lo = mid; else hi = mid;
+  const int lane = threadIdx.x % kWave;
+  const long long wave = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave;
+  const long long waves = (gridDim.x * (long long)blockDim.x) / kWave;
+  for (long long e = wave; e < v.n_elems; e += waves) {
+    const long long t0 = v.bs_off[e], t1 = v.bs_off[e + 1];
+    for (long long t = t0 + lane; t < t1; t += kWave) {
+      w.bs_elem[t] = int(e);
+      w.bs_pack[t] = pack_spec(v.bs_dir[t], v.bs_i[t], v.bs_j[t], v.p1);
+      const int d = v.bs_dof[t];
+      // H(curl) specs are U or V directed with orders inside the Gram library
+      if (d < 0 || d >= v.n_dofs || v.bs_dir[t] > FEM2D_DIR_V || v.bs_i[t] >= v.p1 || v.bs_j[t] >= v.p1) {
+        set_err(w.err, kErrView);
+        continue;
+      }
+      const long long val = (e << 32) | (t - t0);
+      if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot0[d]), ~0ull, (unsigned long long)val) != ~0ull)
+        if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot1[d]), ~0ull, (unsigned long long)val) != ~0ull)
+          set_err(w.err, kErrContrib);
+    }
   }
-  const int e = lo;
-  w.bs_elem[t] = e;
-  w.bs_pack[t] = unsigned(v.bs_dir[t]) | (unsigned(v.bs_i[t]) << 8) | (unsigned(v.bs_j[t]) << 16);
-  const int d = v.bs_dof[t];
-  // H(curl) specs are U or V directed with orders inside the Gram library
-  if (d < 0 || d >= v.n_dofs || v.bs_dir[t] > FEM2D_DIR_V || v.bs_i[t] >= v.p1 || v.bs_j[t] >= v.p1) {
-    set_err(w.err, kErrView);
-    return;
-  }
-  const long long val = ((long long)e << 32) | (long long)(t - v.bs_off[e]);
-  if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot0[d]), ~0ull, (unsigned long long)val) != ~0ull)
-    if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot1[d]), ~0ull, (unsigned long long)val) != ~0ull)
-      set_err(w.err, kErrContrib);
 }
 
 __device__ __forceinline__ void dof_sig(const Work& w, long long d, int& e1, int& k1, int& e2, int& k2) {
@@ -520,22 +565,22 @@ __global__ void k_dof_flags(Work w) {
   w.dof_flag[d] = flag;
 }
 
-__global__ void k_group_starts(Work w) {
-  const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (d >= w.n_dofs) return;
-  if (w.dof_flag[d]) {
-    const int g = w.dof_grp[d] - 1;
-    w.g_start[g] = int(d);
-    w.g_e1[g] = w.dof_e1[d];
-    w.g_e2[g] = w.dof_e2[d];
-  }
-  if (d == w.n_dofs - 1) w.g_start[w.dof_grp[d]] = int(w.n_dofs);
-}
-
-// Each Elem's basis-spec list as runs of whole row groups (memberships): flag
-// the first spec of every run and check that each run is one complete group.
-__global__ void k_mem_flags(DevView v, Work w) {
+// Row-group starts and Elems (threads over DoFs), and each Elem's basis-spec
+// list as runs of whole row groups, "memberships" (threads over specs): the
+// first spec of every run is flagged, and each run must be one complete group
+// (a run starts at a group's first DoF and ends at its last).
+__global__ void k_groups_memberships(DevView v, Work w) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t < w.n_dofs) {
+    const long long d = t;
+    if (w.dof_flag[d]) {
+      const int g = w.dof_grp[d] - 1;
+      w.g_start[g] = int(d);
+      w.g_e1[g] = w.dof_e1[d];
+      w.g_e2[g] = w.dof_e2[d];
+    }
+    if (d == w.n_dofs - 1) w.g_start[w.dof_grp[d]] = int(w.n_dofs);
+  }
   if (t >= v.n_bs) return;
   const int e = w.bs_elem[t];
   const int d = v.bs_dof[t];
@@ -556,7 +601,9 @@ __global__ void k_mem_flags(DevView v, Work w) {
     const int dn = v.bs_dof[t + 1];
     end = !(dn >= 0 && dn < v.n_dofs && dn == d + 1 && w.dof_grp[dn] - 1 == g);
   }
-  if ((start && d != w.g_start[g]) || (end && d != w.g_start[g + 1] - 1)) set_err(w.err, kErrView);
+  const bool group_first = w.dof_flag[d] != 0;
+  const bool group_last = d + 1 == v.n_dofs || w.dof_flag[d + 1] != 0;
+  if ((start && !group_first) || (end && !group_last)) set_err(w.err, kErrView);
   w.mem_flag[t] = start ? 1 : 0;
 }
 
@@ -585,6 +632,8 @@ __global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
   const long long per_axis = (long long)L.nr * kKinds * L.p1 * L.p1;
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= 2 * per_axis) return;
+  // layout [axis][range][kind][a][b]; a transposed copy ([b][a]) follows the
+  // whole table so that both orientations of a pair read along b
   const int axis = int(t / per_axis);
   long long r = t % per_axis;
   const int b = int(r % L.p1);
@@ -614,6 +663,7 @@ __global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
     acc += wts[m] * shape_pick(L.basis, fp, a, xp) * shape_pick(L.basis, fq, b, xq);
   }
   w.lib[t] = acc;
+  w.lib[2 * per_axis + t - b - a * L.p1 + b * L.p1 + a] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -678,25 +728,18 @@ struct WaveShared {
   int n_src, n_dist, total, u_self;
 };
 
-// Per-wave LDS of the fill only: source prefactors and the staged columns
-// t = 0..T-1 (candidate index u_self + t) of the group's first row
-struct WaveFill {
-  double cA[kWS][4];
-  double cB[kWS][2];
-  int col_c[kWC];
-  unsigned col_p0[kWC];  // packed spec | source << 24
-  unsigned col_p1[kWC];  // packed spec | source << 24, ~0 if none
-};
-
 // Builds row group g's candidate set in this wave's LDS slice. Returns 0 when
 // built and inside the wave path's limits, 1 when the group needs the block
-// path, 2 on an invalid domain (error flag set). With cA/cB non-null the
-// per-source prefactors are computed as well.
-__device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S, int lane, double (*cA)[4],
-                          double (*cB)[2]) {
+// path, 2 on an invalid domain (error flag set). With coef non-null the
+// per-source prefactors are written there (kCoef doubles per source) and the
+// source infos gain their dyadic range ids.
+__device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S, int lane, double* coef) {
   const int n_src = gather_sources(w, g, kWS, S.src_x, S.src_info, lane, kWave);
   if (n_src > kWS) return 1;
-  if (lane == 0) S.u_self = -1;
+  if (lane == 0) {
+    S.n_src = n_src;
+    S.u_self = -1;
+  }
   wave_sync();
   // membership counts -> gather offsets (n_src <= 32)
   int cnt = 0, X = -1;
@@ -715,14 +758,31 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
       S.u.sort.key[r] = ((unsigned long long)(unsigned)w.mem_group[m0 + m] << 20) | (unsigned long long)r;
       S.u.sort.raw_meta[r] = lane | (w.mem_bsoff[m0 + m] << 8);
     }
-    if (cA) {
+    if (coef) {
       const int info = S.src_info[lane];
       const int k = info & 1, rel = (info >> 1) & 3;
       const int E = k == 0 ? w.g_e1[g] : w.g_e2[g];
       const int P = rel == kAnc ? X : E;
       const int Q = rel == kAnc ? E : X;
       int ru, rv;
-      block_coefs(w, P, Q, rel == kLocal, cA[lane], cB[lane], ru, rv);
+      double cA[4], cB[2];
+      block_coefs(w, P, Q, rel == kLocal, cA, cB, ru, rv);
+      double* out = coef + lane * kCoef;
+      out[0] = cA[0];
+      out[1] = cA[1];
+      out[2] = cA[2];
+      out[3] = cA[3];
+      out[4] = cB[0];
+      out[5] = cB[1];
+      // library range bases of the pair and which of the group's Elems it uses
+      // ancestor pairs read the transposed copy, so that the row's spec
+      // always selects the table row and the column's spec the entry in it
+      int* tail = reinterpret_cast<int*>(out + 6);
+      const int tcopy = rel == kAnc ? 2 * w.lib_nr * w.lib_range_stride : 0;
+      tail[0] = tcopy + ru * w.lib_range_stride;
+      tail[1] = tcopy + (w.lib_nr + rv) * w.lib_range_stride;
+      tail[2] = k | (rel << 1);
+      tail[3] = 0;
       S.src_info[lane] = k | (rel << 1) | (ru << 3) | (rv << 17);
     }
   }
@@ -811,7 +871,12 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
   return 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_count_wave(DevView v, Work w) {
+// One wavefront per row group: build the candidate set, then write the row
+// lengths and the group's fixed-stride records for the streaming fill:
+// source prefactors and infos, the packed specs of its rows, and for each
+// candidate column t = 0..T-1 of its first row (candidate index u_self + t)
+// the column and the packed specs of its one or two contributions.
+__global__ void __launch_bounds__(kBlock) k_describe_wave(DevView v, Work w) {
   if (*w.err) return;  // an invalid view was detected: nothing below may trust it
   __shared__ WaveShared shared[kWavesPerBlock];
   const int lane = threadIdx.x % kWave;
@@ -822,20 +887,69 @@ __global__ void __launch_bounds__(kBlock) k_count_wave(DevView v, Work w) {
   // atomics, which span the XCDs' separate L2 caches
   const int stride = gridDim.x * kWavesPerBlock;
   for (int g = blockIdx.x * kWavesPerBlock + wid; g < n_groups; g += stride) {
-    const int st = wave_build(v, w, g, S, lane, nullptr, nullptr);
+    if (g >= w.meta_cap) {  // more row groups than the view's DoF signatures give
+      if (lane == 0) set_err(w.err, kErrView);
+      break;
+    }
+    const long long gs = (long long)g * kWS;
+    const int st = wave_build(v, w, g, S, lane, w.meta_coef + gs * kCoef);
+    const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
+    const int nrows = r1 - r0;
     if (st == 1) {
       if (lane == 0) {
         w.is_large[g] = 1;
         w.large_list[atomicAdd(w.n_large, 1)] = g;
+        w.meta_hdr[g] = make_int4(-1, nrows, 0, 0);
       }
     } else if (st == 0) {
-      if (lane == 0) w.is_large[g] = 0;
-      const int u_self = S.u_self, total = S.total;
-      const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
-      for (int r = r0 + lane; r < r1; r += kWave) w.row_len[r] = (int64_t)(total - (u_self + (r - r0)));
+      const int u_self = S.u_self, n_dist = S.n_dist, n_src = S.n_src;  // set by wave_build
+      const int T = S.total - u_self;
+      if (lane == 0) {
+        w.is_large[g] = 0;
+        w.meta_hdr[g] = make_int4(T, nrows, n_src, 0);
+      }
+      const int e1 = w.g_e1[g], e2 = w.g_e2[g];
+      const long long b1 = v.bs_off[e1];
+      const long long b2 = e2 >= 0 ? v.bs_off[e2] : 0;
+      for (int q = lane; q < nrows; q += kWave) {
+        const int r = r0 + q;
+        w.row_len[r] = (int64_t)(T - q);
+        w.meta_row[(long long)g * kWRows + q] =
+            make_uint2(w.bs_pack[b1 + w.dof_k1[r]], e2 >= 0 ? w.bs_pack[b2 + w.dof_k2[r]] : 0u);
+      }
+      for (int t = lane; t < T; t += kWave) {
+        const int u = u_self + t;
+        int a = 0, b = n_dist;  // cum[a] <= u < cum[b]
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (S.cum[mid] <= u) a = mid; else b = mid;
+        }
+        const int within = u - S.cum[a];
+        const int m0 = S.dmeta0[a], m1 = S.dmeta1[a];
+        uint4 rec;
+        rec.x = unsigned(S.dstart[a] + within);
+        rec.y = w.bs_pack[v.bs_off[S.src_x[m0 & 0xff]] + (m0 >> 8) + within];
+        rec.z = m1 < 0 ? ~0u : w.bs_pack[v.bs_off[S.src_x[m1 & 0xff]] + (m1 >> 8) + within];
+        rec.w = unsigned(m0 & 0xff) | (m1 < 0 ? 0u : unsigned(m1 & 0xff) << 8);
+        w.meta_col[(long long)g * kWC + t] = rec;
+      }
     }
     wave_sync();
   }
+}
+
+// First CSR entry of every row group, and for each chunk of kChunk entries the
+// row group that holds its first entry (one thread per row group).
+__global__ void k_segments(Work w) {
+  if (*w.err) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_groups = n_groups_of(w);
+  if (g >= n_groups) return;
+  const int64_t s0 = w.row_ptr[w.g_start[g]];
+  const int64_t s1 = w.row_ptr[w.g_start[g + 1]];
+  w.seg_start[g] = s0;
+  if (g == n_groups - 1) w.seg_start[n_groups] = s1;
+  for (long long c = (s0 + kChunk - 1) / kChunk; c * kChunk < s1 && c < w.chunk_cap; ++c) w.chunk_first[c] = g;
 }
 
 __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int c, double cc, double l2) {
@@ -846,114 +960,133 @@ __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int
   }
 }
 
-// contribution of one (row spec, column spec) pair through source s
-__device__ __forceinline__ void wave_contrib(const double* __restrict__ lib, const LibShape& L, const WaveShared& S,
-                                             const WaveFill& F, unsigned cpk, uint2 rp, double& cc, double& l2) {
-  const int s = int(cpk >> 24) & (kWS - 1);
-  const int info = S.src_info[s];
-  const int k = info & 1, rel = (info >> 1) & 3;
-  const int ru = (info >> 3) & 0x3fff, rv = (info >> 17) & 0x3fff;
-  const unsigned rpk = k ? rp.y : rp.x;
-  const unsigned cbs = cpk & 0xffffffu;
+// contribution of one (row spec, column spec) pair through source s of the
+// group; the source record (staged in LDS) holds the prefactors, the library
+// range bases and which of the group's Elems (k) and which relation (rel) the
+// pair has
+__device__ __forceinline__ void flat_contrib(const double* __restrict__ lib, unsigned tab,
+                                             const double* __restrict__ coef_g, unsigned s, unsigned cbs, uint2 rp,
+                                             double& cc, double& l2) {
+  const unsigned ci = s * unsigned(kCoef);
+  const int4 tail = reinterpret_cast<const int4*>(coef_g)[ci / 2u + 3u];
+  const unsigned rpk = (tail.z & 1) ? rp.y : rp.x;
+  const bool anc = (tail.z >> 1) == kAnc;
+  const unsigned rd = spec_dir(rpk), cd = spec_dir(cbs);
   // the p side of the reference's pair is the ancestor's spec
-  const unsigned pbs = rel == kAnc ? cbs : rpk;
-  const unsigned qbs = rel == kAnc ? rpk : cbs;
-  pair_values(lib, L, F.cA[s], F.cB[s], ru, rv, pbs, qbs, cc, l2);
+  const unsigned pd = anc ? cd : rd, qd = anc ? rd : cd;
+  const unsigned pair = 2u * pd + qd;
+  const unsigned ix = unsigned(tail.x) + spec_ip1(rpk) + spec_i(cbs);
+  const unsigned iy = unsigned(tail.y) + spec_jp1(rpk) + spec_j(cbs);
+  const double fx = ld32(lib, ix + __umul24(pair, tab));
+  const double fy = ld32(lib, iy + __umul24(3u - pair, tab));
+  cc = coef_g[ci + pair] * fx * fy;
+  // L2Inner (UU, VV only): the TT factor replaces the N N factor's partner on
+  // the axis where the pair has its T functions; read unconditionally
+  const double ftt = ld32(lib, (pd ? ix : iy) + __umul24(unsigned(kTT), tab));
+  const double l2v = (coef_g[ci + 4u + pd] * (pd ? ftt : fx)) * (pd ? fy : ftt);
+  l2 = pd == qd ? l2v : 0.0;
 }
 
 // Entry p of a row group's segment: row q = max{q : off(q) <= p} with
-// off(q) = q T - q (q - 1) / 2, candidate column t = q + p - off(q).
-__device__ __forceinline__ void segment_entry(int p, int T, int nrows, float bq, const WaveShared& S,
-                                              const WaveFill& F, const double* __restrict__ lib, const LibShape& L,
-                                              int& c, double& cc, double& l2) {
-  int q = int((bq - sqrtf(fmaxf(bq * bq - 8.0f * float(p), 0.0f))) * 0.5f);
+// off(q) = q T - q (q - 1) / 2, candidate column t = q + p - off(q). The float
+// root (all operands below 2^24, hardware square root) lands within one row of
+// q; one branch-free step each way fixes it, and the clamps keep (q, t) inside
+// the group's records whatever the rounding.
+__device__ __forceinline__ void segment_entry(int p, int T, int nrows, float bq, int& q, int& t) {
+  q = int((bq - __builtin_amdgcn_sqrtf(fmaxf(bq * bq - 8.0f * float(p), 0.0f))) * 0.5f);
   q = min(max(q, 0), nrows - 1);
-  int oq = q * T - q * (q - 1) / 2;
-  while (oq > p) {  // at most one step: the float root is within one row
-    --q;
-    oq -= T - q;
-  }
-  while (q + 1 < nrows && oq + (T - q) <= p) {
-    oq += T - q;
-    ++q;
-  }
-  const int t = q + (p - oq);
-  const unsigned p0 = F.col_p0[t], p1 = F.col_p1[t];
-  const uint2 rp = S.u.row_pk[q];
-  c = F.col_c[t];
-  wave_contrib(lib, L, S, F, p0, rp, cc, l2);
-  if (p1 != ~0u) {
+  // off(q) = q (2T - q + 1) / 2; both factors stay below 2^24
+  int oq = int(__umul24(unsigned(q), unsigned(2 * T - q + 1)) >> 1);
+  const bool down = oq > p && q > 0;
+  q -= down ? 1 : 0;
+  oq -= down ? T - q : 0;
+  const bool up = q + 1 < nrows && oq + (T - q) <= p;
+  oq += up ? T - q : 0;
+  q += up ? 1 : 0;
+  t = min(max(q + (p - oq), q), T - 1);
+}
+
+// One entry of a row group's segment from the group's records (bases are
+// wave-uniform; offsets are 32-bit).
+__device__ __forceinline__ void flat_entry(const uint4* __restrict__ col_g, const uint2* __restrict__ row_g,
+                                           const double* __restrict__ coef_g, const double* __restrict__ lib,
+                                           unsigned tab, int pl, int T, int nrows, float bq, int& c, double& cc,
+                                           double& l2) {
+  int q, t;
+  segment_entry(pl, T, nrows, bq, q, t);
+  const uint4 rec = ld32(col_g, unsigned(t));
+  const uint2 rp = ld32(row_g, unsigned(q));
+  c = int(rec.x);
+  flat_contrib(lib, tab, coef_g, rec.w & 0xffu, rec.y, rp, cc, l2);
+  if (rec.z != ~0u) {
     double cc1, l21;
-    wave_contrib(lib, L, S, F, p1, rp, cc1, l21);
+    flat_contrib(lib, tab, coef_g, (rec.w >> 8) & 0xffu, rec.z, rp, cc1, l21);
     cc += cc1;
     l2 += l21;
   }
 }
 
-// One wavefront per row group: build the candidate set, stage the group's row
-// specs and its candidate columns in LDS, then write the group's contiguous
-// CSR segment with consecutive lanes on consecutive entries. Entry p of the
-// segment lies in row q = max{q : off(q) <= p}, off(q) = q T - q (q - 1) / 2,
-// where T is the first row's length; its candidate column is q + p - off(q).
-__global__ void __launch_bounds__(kBlock) k_fill_wave(DevView v, Work w, LibShape L, FillOut o) {
-  if (*w.err) return;  // an invalid view was detected: nothing below may trust it
-  __shared__ WaveShared shared[kWavesPerBlock];
-  __shared__ WaveFill fill[kWavesPerBlock];
+// Streaming fill: the CSR is cut into chunks of kChunk consecutive entries;
+// each wavefront takes chunks in turn and writes them with consecutive lanes
+// on consecutive entries, reading the row groups' records (no LDS, no
+// per-group set-up, equal work per chunk). Entries of block-path groups are
+// left to k_fill_large.
+__global__ void __launch_bounds__(kBlock) k_fill_flat(Work w, LibShape L, FillOut o) {
+  if (*w.err) return;
+  // the source records of the row group being written, per wavefront
+  __shared__ double4 src_lds[kWavesPerBlock][kWS * kCoef / 4];
+  const long long nnz = w.row_ptr[w.n_dofs];
+  if (nnz > o.capacity) return;  // the host grows the buffers and runs the fill again
   const int lane = threadIdx.x % kWave;
-  const int wid = threadIdx.x / kWave;
-  WaveShared& S = shared[wid];
-  WaveFill& F = fill[wid];
-  const int n_groups = n_groups_of(w);
+  const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+  const int waves = int((gridDim.x * blockDim.x) / kWave);
+  const long long n_chunks = min((nnz + kChunk - 1) / kChunk, w.chunk_cap);
   const double* __restrict__ lib = w.lib;
-  const int stride = gridDim.x * kWavesPerBlock;
-  for (int g = blockIdx.x * kWavesPerBlock + wid; g < n_groups; g += stride) {
-    if (w.is_large[g]) continue;
-    if (wave_build(v, w, g, S, lane, F.cA, F.cB) != 0) {
-      wave_sync();
-      continue;
-    }
-    const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
-    const int nrows = r1 - r0;
-    const int e1 = w.g_e1[g], e2 = w.g_e2[g];
-    const int n_dist = S.n_dist, u_self = S.u_self;
-    const int T = S.total - u_self;
-    const long long b1 = v.bs_off[e1];
-    const long long b2 = e2 >= 0 ? v.bs_off[e2] : 0;
-    for (int q = lane; q < nrows; q += kWave) {
-      const int r = r0 + q;
-      S.u.row_pk[q] = make_uint2(w.bs_pack[b1 + w.dof_k1[r]], e2 >= 0 ? w.bs_pack[b2 + w.dof_k2[r]] : 0u);
-    }
-    for (int t = lane; t < T; t += kWave) {
-      const int u = u_self + t;
-      int a = 0, b = n_dist;  // cum[a] <= u < cum[b]
-      while (b - a > 1) {
-        const int mid = (a + b) >> 1;
-        if (S.cum[mid] <= u) a = mid; else b = mid;
+  const unsigned tab = unsigned(L.tab);
+  for (long long ch = wave; ch < n_chunks; ch += waves) {
+    const long long pbase = ch * kChunk;
+    const int len = int(min((long long)kChunk, nnz - pbase));
+    int* __restrict__ col_out = o.col + pbase;
+    double* __restrict__ a_out = o.aval + pbase;
+    double* __restrict__ b_out = o.bval + pbase;
+    int g = w.chunk_first[ch];
+    int j0 = 0;
+    while (j0 < len) {
+      const long long S = w.seg_start[g], S1 = w.seg_start[g + 1];
+      const int j1 = int(min((long long)len, S1 - pbase));
+      const int4 hdr = w.meta_hdr[g];
+      if (hdr.x > 0) {
+        const int T = hdr.x, nrows = hdr.y;
+        const int off = int(pbase - S);  // segment position of chunk entry 0
+        const float bq = 2.0f * float(T) + 1.0f;
+        const uint4* col_g = w.meta_col + (size_t)g * kWC;
+        const uint2* row_g = w.meta_row + (size_t)g * kWRows;
+        double4* stage = src_lds[threadIdx.x / kWave];
+        const double4* src_g = reinterpret_cast<const double4*>(w.meta_coef + (size_t)g * kWS * kCoef);
+        for (int i = lane; i < hdr.z * (kCoef / 4); i += kWave) stage[i] = src_g[i];
+        wave_sync();
+        const double* coef_g = reinterpret_cast<const double*>(stage);
+        // two entries per lane and iteration: their loads overlap
+        for (int j = j0 + lane; j < j1; j += 2 * kWave) {
+          int c0, c1 = 0;
+          double cc0, l20, cc1 = 0.0, l21 = 0.0;
+          flat_entry(col_g, row_g, coef_g, lib, tab, j + off, T, nrows, bq, c0, cc0, l20);
+          const bool second = j + kWave < j1;
+          if (second) flat_entry(col_g, row_g, coef_g, lib, tab, j + kWave + off, T, nrows, bq, c1, cc1, l21);
+          col_out[unsigned(j)] = c0;
+          a_out[unsigned(j)] = o.a_is_cc ? cc0 : l20;
+          b_out[unsigned(j)] = o.b_is_cc ? cc0 : l20;
+          if (second) {
+            col_out[unsigned(j + kWave)] = c1;
+            a_out[unsigned(j + kWave)] = o.a_is_cc ? cc1 : l21;
+            b_out[unsigned(j + kWave)] = o.b_is_cc ? cc1 : l21;
+          }
+        }
+        wave_sync();  // every lane is done with the staged records
       }
-      const int within = u - S.cum[a];
-      F.col_c[t] = S.dstart[a] + within;
-      const int m0 = S.dmeta0[a];
-      F.col_p0[t] = w.bs_pack[v.bs_off[S.src_x[m0 & 0xff]] + (m0 >> 8) + within] | (unsigned(m0 & 0xff) << 24);
-      const int m1 = S.dmeta1[a];
-      F.col_p1[t] = m1 < 0 ? ~0u
-                           : (w.bs_pack[v.bs_off[S.src_x[m1 & 0xff]] + (m1 >> 8) + within] |
-                              (unsigned(m1 & 0xff) << 24));
+      j0 = j1;
+      ++g;
     }
-    wave_sync();
-    const long long seg0 = w.row_ptr[r0];
-    const int seg_n = nrows * T - nrows * (nrows - 1) / 2;
-    const float bq = 2.0f * float(T) + 1.0f;
-    // two entries per lane and iteration: their LDS and library loads overlap
-    for (int p = lane; p < seg_n; p += 2 * kWave) {
-      int c0, c1 = 0;
-      double cc0, l20, cc1 = 0.0, l21 = 0.0;
-      segment_entry(p, T, nrows, bq, S, F, lib, L, c0, cc0, l20);
-      const bool second = p + kWave < seg_n;
-      if (second) segment_entry(p + kWave, T, nrows, bq, S, F, lib, L, c1, cc1, l21);
-      write_entry(o, seg0 + p, c0, cc0, l20);
-      if (second) write_entry(o, seg0 + p + kWave, c1, cc1, l21);
-    }
-    wave_sync();
   }
 }
 
@@ -1169,8 +1302,9 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
         const unsigned cbs = w.bs_pack[v.bs_off[S.src_x[sidx]] + off + within];
         const unsigned rbs = S.row_bs[k][lo];
         double cc, l2;
-        if (rel == kAnc) pair_values(w.lib, L, S.cA[sidx], S.cB[sidx], ru, rv, cbs, rbs, cc, l2);
-        else pair_values(w.lib, L, S.cA[sidx], S.cB[sidx], ru, rv, rbs, cbs, cc, l2);
+        const unsigned bx = unsigned(ru * L.range_stride), by = unsigned((L.nr + rv) * L.range_stride);
+        if (rel == kAnc) pair_values(w.lib, unsigned(L.tab), S.cA[sidx], S.cB[sidx], bx, by, cbs, rbs, cc, l2);
+        else pair_values(w.lib, unsigned(L.tab), S.cA[sidx], S.cB[sidx], bx, by, rbs, cbs, cc, l2);
         cc_sum += cc;
         l2_sum += l2;
       }
@@ -1198,9 +1332,25 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// what a recorded assembly graph depends on besides its buffers
+struct GraphKey {
+  unsigned long long domain = 0;
+  int basis = -1, ai = -1, bi = -1, nu = 0, nv = 0;
+  long long capacity = -1;
+  unsigned long long alloc_generation = 0;
+  bool run_large = true;
+  bool operator==(const GraphKey& o) const {
+    return domain == o.domain && basis == o.basis && ai == o.ai && bi == o.bi && nu == o.nu && nv == o.nv &&
+           capacity == o.capacity && alloc_generation == o.alloc_generation && run_large == o.run_large;
+  }
+};
+
+std::atomic<unsigned long long> g_domain_serial{0};
+
 }  // namespace
 
 struct fem2d_device_domain {
+  unsigned long long serial = 0;  // unique per upload (keys recorded graphs)
   int device = 0;
   int cc = 0;
   int n_elems = 0;
@@ -1209,6 +1359,10 @@ struct fem2d_device_domain {
   int max_i = 0, max_j = 0;
   int max_du = 0, max_dv = 0;
   long long nested_cap = 0;  // sum over Elems of 1 + 2 * depth: entries of the nested lists
+  long long n_groups = 0;    // row groups of the view's DoF signatures (sizes the per-group records)
+  // whether any row group needs the block path: a property of the Domain,
+  // learned by the first assembly (-1 unknown, 0 none, 1 some)
+  mutable int large_state = -1;
   // device arrays
   int* parent = nullptr;
   unsigned char* loc = nullptr;
@@ -1236,6 +1390,9 @@ struct fem2d_ctx {
   int n_cu = 0;
   int count_grid = 0;  // resident blocks of the wave kernels on this device
   int fill_grid = 0;
+  bool use_graphs = true;
+  hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};  // the last recorded assembly, in parts
+  GraphKey graph_key;
 };
 
 int fem2d::capi::ctx_device(fem2d_ctx* ctx) { return ctx->device; }
@@ -1255,9 +1412,20 @@ int hip_fail(hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail(_e, #expr);  \
   } while (0)
 
+#define HIP_TRY_AT(expr, where)                                                   \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) return hip_fail(_e, (std::string(#expr) + " " + where).c_str()); \
+  } while (0)
+
+// bumped whenever a device buffer is (re)allocated: recorded graphs hold
+// buffer addresses and are recorded again after any change
+std::atomic<unsigned long long> g_alloc_generation{0};
+
 int ensure(DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (b.bytes >= bytes) return FEM2D_OK;
+  ++g_alloc_generation;
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
@@ -1372,9 +1540,9 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   // the wave kernels assign row groups statically: launch exactly the blocks
   // that are resident at once
   int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_describe_wave, kBlock, 0));
   ctx->count_grid = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_wave, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat, kBlock, 0));
   ctx->fill_grid = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
@@ -1392,6 +1560,8 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
     if (b.p) (void)hipFree(b.p);
   for (DevBuf* b : {&ctx->scan_tmp, &ctx->out_col, &ctx->out_a, &ctx->out_b})
     if (b->p) (void)hipFree(b->p);
+  for (auto& g : ctx->graph_exec)
+    if (g) (void)hipGraphExecDestroy(g);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->h_status) (void)hipHostFree(ctx->h_status);
@@ -1400,6 +1570,15 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
 }
 
 void* fem2d_ctx_stream(fem2d_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int fem2d_ctx_set_graphs(fem2d_ctx* ctx, int enable) {
+  if (!ctx) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  ctx->use_graphs = enable != 0;
+  return FEM2D_OK;
+}
 
 int fem2d_ctx_synchronize(fem2d_ctx* ctx) {
   if (!ctx) {
@@ -1455,6 +1634,7 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
       return FEM2D_ERR_INVALID_VIEW;
     }
   auto* dd = new fem2d_device_domain();
+  dd->serial = ++g_domain_serial;
   dd->device = ctx->device;
   dd->cc = v->cc;
   dd->n_elems = ne;
@@ -1475,6 +1655,20 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
     dd->max_du = std::max(dd->max_du, du);
     dd->max_dv = std::max(dd->max_dv, dv);
     dd->nested_cap += 1 + 2LL * steps;
+  }
+  {
+    // row groups: runs of DoFs whose basis specs lie on the same Elems (spec
+    // lists are visited in Elem order, so the first Elem is the smaller one)
+    std::vector<int> s0(size_t(v->n_dofs), -1), s1(size_t(v->n_dofs), -1);
+    for (int e = 0; e < ne; ++e)
+      for (long long t = v->bs_offsets[e]; t < v->bs_offsets[e + 1]; ++t) {
+        const int d = v->bs_dof[t];
+        if (d < 0 || d >= v->n_dofs) continue;  // rejected by the device checks
+        if (s0[d] < 0) s0[d] = e;
+        else if (s1[d] < 0) s1[d] = e;
+      }
+    for (long long d = 0; d < v->n_dofs; ++d)
+      if (d == 0 || s0[d] != s0[d - 1] || s1[d] != s1[d - 1]) ++dd->n_groups;
   }
   int rc = FEM2D_OK;
   std::vector<long long> off(v->bs_offsets, v->bs_offsets + ne + 1);
@@ -1536,7 +1730,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   const size_t E1 = size_t(std::max(ne, 1)), N1 = size_t(nd + 1), B1 = size_t(nbs + 1);
   if ((rc = slot(ctx, k++, E1, &w.geo)) || (rc = slot(ctx, k++, E1, &w.depth)) ||
       (rc = slot(ctx, k++, E1, &w.n_desc)) || (rc = slot(ctx, k++, E1, &w.desc_cursor)) ||
-      (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_len)) || (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_ptr)) ||
+      (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_ptr)) ||
       (rc = slot(ctx, k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) ||
       (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, N1, &w.slot0)) ||
       (rc = slot(ctx, k++, N1, &w.slot1)) || (rc = slot(ctx, k++, B1, &w.bs_elem)) ||
@@ -1553,15 +1747,26 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       (rc = slot(ctx, k++, N1, &w.is_large)))
     return rc;
   w.n_large = w.err + 1;
+  const size_t G1 = size_t(std::max(1LL, dd->n_groups));
+  w.meta_cap = int(G1);
+  if ((rc = slot(ctx, k++, G1, &w.meta_hdr)) || (rc = slot(ctx, k++, G1 * kWS * kCoef, &w.meta_coef)) ||
+      (rc = slot(ctx, k++, G1 * kWC, &w.meta_col)) ||
+      (rc = slot(ctx, k++, G1 * kWRows, &w.meta_row)) || (rc = slot(ctx, k++, G1 + 1, &w.seg_start)))
+    return rc;
+  const int chunk_slot = k++;
 
   LibShape L;
   L.p1 = std::max(dd->max_i, dd->max_j) + 1;
+  L.tab = L.p1 * L.p1;
+  L.range_stride = kKinds * L.tab;
   L.nr = (1 << (std::max(dd->max_du, dd->max_dv) + 1)) - 1;
   L.nu = nu;
   L.nv = nv;
   L.basis = basis;
   const long long lib_n = 2LL * L.nr * kKinds * L.p1 * L.p1;
-  if ((rc = slot(ctx, k++, size_t(lib_n), &w.lib))) return rc;
+  w.lib_range_stride = L.range_stride;
+  w.lib_nr = L.nr;
+  if ((rc = slot(ctx, k++, size_t(2 * lib_n), &w.lib))) return rc;  // table and its transpose
 
   // scan temporaries
   size_t tmp_bytes = 0, t2 = 0;
@@ -1571,9 +1776,12 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   tmp_bytes = std::max(tmp_bytes, t2);
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.row_len, w.row_ptr, int(nd + 1), ctx->stream));
   tmp_bytes = std::max(tmp_bytes, t2);
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.nested_len, w.nested_ptr, ne + 1, ctx->stream));
+  const hipcub::TransformInputIterator<int, NestedLen, hipcub::CountingInputIterator<int>> nested_len(
+      hipcub::CountingInputIterator<int>(0), NestedLen{w.depth, w.n_desc, ne});
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, nested_len, w.nested_ptr, ne + 1, ctx->stream));
   tmp_bytes = std::max(tmp_bytes, t2);
   if ((rc = ensure(ctx->scan_tmp, tmp_bytes))) return rc;
+  const long long elem_waves_blocks = std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock);
 
   hipStream_t st = ctx->stream;
   // the previous call's readback of the pinned staging buffers has completed
@@ -1599,38 +1807,80 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
 
   const int B = 256;
   const unsigned large_grid = unsigned(std::max(1, ctx->n_cu * 2));
-  HIP_TRY(hipEventRecord(ctx->ev[0], st));
-  k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd, nbs);
-  k_elem_prep<<<grid_for(ne, B), B, 0, st>>>(v, w);
-  k_nested_len<<<grid_for(ne + 1, B), B, 0, st>>>(w, ne);
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.nested_len, w.nested_ptr, ne + 1, st));
-  k_nested_fill<<<grid_for(ne, B), B, 0, st>>>(v, w);
-  k_bs_slots<<<grid_for(nbs, B), B, 0, st>>>(v, w);
-  k_dof_flags<<<grid_for(nd, B), B, 0, st>>>(w);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipcub::DeviceScan::InclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.dof_flag, w.dof_grp, int(nd), st));
-  k_group_starts<<<grid_for(nd, B), B, 0, st>>>(w);
-  k_mem_flags<<<grid_for(nbs, B), B, 0, st>>>(v, w);
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.mem_flag, w.mem_idx, int(nbs + 1), st));
-  k_mem_fill<<<grid_for(std::max<long long>(nbs, ne + 1), B), B, 0, st>>>(v, w);
-  k_gram_lib<<<grid_for(lib_n, B), B, 0, st>>>(w, L, ctx->glq);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[1], st));
+  // the block path runs unless this Domain is known to have no large groups
+  const bool run_large = dd->large_state != 0;
 
-  // symbolic: row lengths per row group, then row_ptr
-  HIP_TRY(hipEventRecord(ctx->ev[4], st));
-  k_count_wave<<<unsigned(ctx->count_grid), kBlock, 0, st>>>(v, w);
-  HIP_TRY(hipEventRecord(ctx->ev[5], st));
-  k_count_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.row_len, w.row_ptr, int(nd + 1), st));
-  HIP_TRY(hipEventRecord(ctx->ev[2], st));
+  // The assembly in three parts: (a) preparation and symbolic phase, (b) the
+  // streaming fill, (c) the block-path fill and the status block. With graphs
+  // each part is recorded once and replayed while the Domain, the options and
+  // the buffers stay the same; the timing events sit between the parts, so the
+  // fill kernel is timed on its own in either mode.
+  auto part_a = [&](const FillOut& o, bool fine_events) -> int {
+    k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd, nbs);
+    k_gram_lib<<<grid_for(lib_n, B), B, 0, st>>>(w, L, ctx->glq);
+    k_elem_prep<<<grid_for(ne, B), B, 0, st>>>(v, w);
+    HIP_TRY_AT(hipGetLastError(), "after the Elem preparation kernels");
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, nested_len, w.nested_ptr, ne + 1, st));
+    k_nested_fill<<<grid_for(ne, B), B, 0, st>>>(v, w);
+    k_bs_slots<<<unsigned(std::min<long long>(elem_waves_blocks, 1 << 16)), kBlock, 0, st>>>(v, w);
+    k_dof_flags<<<grid_for(nd, B), B, 0, st>>>(w);
+    HIP_TRY_AT(hipGetLastError(), "after the nested-list, spec-slot and DoF kernels");
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.dof_flag, w.dof_grp, int(nd), st));
+    k_groups_memberships<<<grid_for(std::max<long long>(nd, nbs), B), B, 0, st>>>(v, w);
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.mem_flag, w.mem_idx, int(nbs + 1), st));
+    k_mem_fill<<<grid_for(std::max<long long>(nbs, ne + 1), B), B, 0, st>>>(v, w);
+    HIP_TRY_AT(hipGetLastError(), "after the row-group kernels");
+    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    // symbolic: per-group candidate sets, row lengths and records, then row_ptr
+    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[4], st));
+    k_describe_wave<<<unsigned(ctx->count_grid), kBlock, 0, st>>>(v, w);
+    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[5], st));
+    if (run_large) k_count_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w);
+    HIP_TRY_AT(hipGetLastError(), "after the describe kernels");
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.row_len, w.row_ptr, int(nd + 1), st));
+    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    if (o.capacity > 0) k_segments<<<grid_for(std::max(1LL, dd->n_groups), B), B, 0, st>>>(w);
+    HIP_TRY_AT(hipGetLastError(), "after the segment kernel");
+    return FEM2D_OK;
+  };
+  // numeric: columns and values; with too small output buffers the fill
+  // writes nothing and the host grows them and runs the assembly again
+  auto part_b = [&](const FillOut& o) -> int {
+    k_fill_flat<<<unsigned(ctx->fill_grid), kBlock, 0, st>>>(w, L, o);
+    HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
+    return FEM2D_OK;
+  };
+  auto part_c = [&](const FillOut& o) -> int {
+    if (o.capacity > 0 && run_large) k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
+    k_finish<<<1, 1, 0, st>>>(w, ctx->d_status);
+    HIP_TRY_AT(hipGetLastError(), "after the status kernel");
+    return FEM2D_OK;
+  };
+  // record one part into ctx->graph_exec[i]
+  auto record = [&](int i, const std::function<int()>& part) -> int {
+    if (ctx->graph_exec[i]) (void)hipGraphExecDestroy(ctx->graph_exec[i]);
+    ctx->graph_exec[i] = nullptr;
+    (void)hipGetLastError();  // no stale error is reported as a recording failure
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int erc = part();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);  // always leave capture mode
+    if (erc != FEM2D_OK || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return erc != FEM2D_OK ? erc : hip_fail(ec, "hipStreamEndCapture");
+    }
+    const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec[i], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      ctx->graph_exec[i] = nullptr;
+      return hip_fail(ei, "hipGraphInstantiate");
+    }
+    return FEM2D_OK;
+  };
 
-  // numeric: columns and values into the output buffers; if they were too
-  // small, the kernels write nothing beyond capacity and the pass is repeated
-  // after growing them
   long long nnz = 0;
   int n_large = 0;
+  bool fine_timing = false;
   for (int attempt = 0; attempt < 2; ++attempt) {
     FillOut o;
     o.col = static_cast<int*>(ctx->out_col.p);
@@ -1639,14 +1889,36 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     o.capacity = ctx->out_capacity;
     o.a_is_cc = ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
     o.b_is_cc = bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
-    HIP_TRY(hipEventRecord(ctx->ev[6], st));
-    if (o.capacity > 0) k_fill_wave<<<unsigned(ctx->fill_grid), kBlock, 0, st>>>(v, w, L, o);
-    HIP_TRY(hipEventRecord(ctx->ev[7], st));
-    if (o.capacity > 0) k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
-    HIP_TRY(hipGetLastError());
-    k_finish<<<1, 1, 0, st>>>(w, ctx->d_status);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    w.chunk_cap = o.capacity / kChunk + 1;
+    if ((rc = slot(ctx, chunk_slot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
+    const bool fill = o.capacity > 0;
+    if (ctx->use_graphs) {
+      const GraphKey key{dd->serial, basis, ai, bi, nu, nv, o.capacity, g_alloc_generation.load(), run_large};
+      if (!ctx->graph_exec[0] || !(key == ctx->graph_key)) {
+        ctx->graph_key = GraphKey{};
+        if ((rc = record(0, [&] { return part_a(o, false); })) ||
+            (fill && (rc = record(1, [&] { return part_b(o); }))) || (rc = record(2, [&] { return part_c(o); })))
+          return rc;
+        ctx->graph_key = key;
+      }
+      HIP_TRY(hipEventRecord(ctx->ev[0], st));
+      HIP_TRY(hipGraphLaunch(ctx->graph_exec[0], st));
+      HIP_TRY(hipEventRecord(ctx->ev[6], st));
+      if (fill) HIP_TRY(hipGraphLaunch(ctx->graph_exec[1], st));
+      HIP_TRY(hipEventRecord(ctx->ev[7], st));
+      HIP_TRY(hipGraphLaunch(ctx->graph_exec[2], st));
+      HIP_TRY(hipEventRecord(ctx->ev[3], st));
+      fine_timing = false;
+    } else {
+      HIP_TRY(hipEventRecord(ctx->ev[0], st));
+      if ((rc = part_a(o, true))) return rc;
+      HIP_TRY(hipEventRecord(ctx->ev[6], st));
+      if (fill && (rc = part_b(o))) return rc;
+      HIP_TRY(hipEventRecord(ctx->ev[7], st));
+      if ((rc = part_c(o))) return rc;
+      HIP_TRY(hipEventRecord(ctx->ev[3], st));
+      fine_timing = true;
+    }
     HIP_TRY(hipStreamSynchronize(st));
     const volatile long long* stat = ctx->h_status;
     if (stat[1]) return err_to_status(stat[1]);
@@ -1665,12 +1937,17 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   }
 
   float t01 = 0, t12 = 0, t23 = 0, t03 = 0, tc = 0, tf = 0;
-  (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
-  (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
-  (void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
   (void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
-  (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
   (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
+  if (fine_timing) {  // per-stage events exist in kernel-by-kernel mode only
+    (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
+    (void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
+    (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
+  } else {
+    (void)hipEventElapsedTime(&t23, ctx->ev[6], ctx->ev[3]);
+  }
+  (void)hipGetLastError();  // a timing query failure is not an assembly failure
   ctx->timing.prep_ms = t01;
   ctx->timing.symbolic_ms = t12;
   ctx->timing.fill_ms = t23;
@@ -1680,6 +1957,12 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   ctx->timing.nnz = nnz;
   ctx->timing.n_groups = ctx->h_status[3];
   ctx->timing.n_large = n_large;
+  if (!run_large && n_large > 0) {
+    // cannot happen for an unchanged Domain; assemble again with the block path
+    dd->large_state = 1;
+    return fem2d_assemble_device(ctx, dd, basis, ai, bi, glq_u, glq_v, precision, out);
+  }
+  dd->large_state = n_large > 0 ? 1 : 0;
 
   out->n = nd;
   out->nnz = nnz;

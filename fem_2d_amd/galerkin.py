@@ -81,6 +81,10 @@ class Context:
         except Exception:
             pass
 
+    def set_graphs(self, enable: bool) -> None:
+        """Record each assembly once as a HIP graph and replay it (default), or launch kernel by kernel."""
+        check(self._lib.fem2d_ctx_set_graphs(self._h, 1 if enable else 0))
+
     def synchronize(self) -> None:
         """Wait for all work on the context's device."""
         check(self._lib.fem2d_ctx_synchronize(self._h))

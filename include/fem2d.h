@@ -126,6 +126,10 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx);
 void* fem2d_ctx_stream(fem2d_ctx* ctx);
 /* device-wide synchronisation of the context's device */
 int fem2d_ctx_synchronize(fem2d_ctx* ctx);
+/* Whether assemblies on this context are recorded once as a HIP graph and
+ * replayed (default 1) or launched kernel by kernel (0). Results are the same;
+ * graphs remove the per-kernel launch gaps of the ~20-kernel pipeline. */
+int fem2d_ctx_set_graphs(fem2d_ctx* ctx, int enable);
 
 /* Replaces galerkin_sample_gep_hcurl (galerkin.rs:33-187).
  * basis: fem2d_basis; a_integral / b_integral: fem2d_integral (the AI / BI
