@@ -10,9 +10,14 @@ of A and B: elem geometry, DoF signatures, row groups, Gram tables, row
 lengths, row_ptr scan and the entry fill. `value` is stored upper-triangular
 nonzeros per second (the length of the reference's BTreeMap per second).
 
-Prints one JSON line. `roofline` covers the entry-fill kernel (the dominant
+Prints one JSON line. The timed steps replay the library's recorded HIP
+graphs. `roofline` covers the entry-fill kernel k_fill_flat (the dominant
 one): achieved = algorithmic bytes (20 B per stored entry: 4 B column + 8 B A
-+ 8 B B) / its HIP-event duration. `cpu_baseline` times the oracle (the C++
++ 8 B B) / its duration, measured with HIP events recorded on the assembly
+stream around the kernel in the same number of kernel-by-kernel steps;
+`traffic` is the HBM bytes per launch from the committed rocprofv3 counter
+summary of that kernel. `host_boundary` times the drop-in entry point with
+host arrays on both sides. `cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
 workload on the host cores.
 """
@@ -84,7 +89,7 @@ def cpu_baseline(w, total_pairs: int, nnz: int, target_s: float):
 def profiled_traffic(workload: str):
     """HBM bytes per launch of the fill kernel on this workload, from the committed rocprofv3
     --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a 1 GiB copy); None if absent."""
-    path = os.path.join(ROOT, "profiles", f"k_fill_wave_{workload}.md")
+    path = os.path.join(ROOT, "profiles", f"k_fill_flat_{workload}.md")
     try:
         for ln in open(path):
             if ln.startswith("traffic_bytes_per_launch:"):
@@ -167,21 +172,33 @@ def main():
     for _ in range(args.warmup):
         out = step()
     # every assembly synchronises its stream before returning; the timed region is
-    # also bracketed by device-wide synchronisation
+    # also bracketed by device-wide synchronisation. Assemblies replay the
+    # recorded HIP graphs of the pipeline (the library's default).
     ctx.synchronize()
     t0 = time.perf_counter()
-    fill_ms, count_ms, total_ms = [], [], []
+    total_ms = []
     for _ in range(args.steps):
         out = step()
-        t = ctx.last_timing()
-        fill_ms.append(t["fill_kernel_ms"])
-        count_ms.append(t["count_kernel_ms"])
-        total_ms.append(t["total_ms"])
+        total_ms.append(ctx.last_timing()["total_ms"])
     ctx.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     nnz = int(out.nnz)
     n = int(out.n)
+    t_last = ctx.last_timing()
+
+    # Per-kernel durations: the same steps launched kernel by kernel, with HIP
+    # events recorded on the assembly stream right before and after the fill
+    # kernel (and the describe kernel).
+    ctx.set_graphs(False)
+    step()
+    fill_ms, count_ms = [], []
+    for _ in range(args.steps):
+        step()
+        t = ctx.last_timing()
+        fill_ms.append(t["fill_kernel_ms"])
+        count_ms.append(t["count_kernel_ms"])
+    ctx.set_graphs(True)
     fill_avg = float(np.mean(fill_ms))
 
     # element pairs visited by the reference's loops (pattern-independent work count)
@@ -223,13 +240,13 @@ def main():
             "glq": list(w.glq),
             "basis": w.basis.__name__,
             "precision": "fp64",
-            "large_groups": int(t["n_large"]),
+            "large_groups": int(t_last["n_large"]),
         },
         "element_pairs_per_s": pairs / (ms_per_step * 1e-3),
-        "device_ms": {"total": float(np.mean(total_ms)), "count_kernel": float(np.mean(count_ms)),
+        "device_ms": {"total": float(np.mean(total_ms)), "describe_kernel": float(np.mean(count_ms)),
                       "fill_kernel": fill_avg},
         "roofline": {
-            "kernel": "k_fill_wave",
+            "kernel": "k_fill_flat",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
