@@ -38,8 +38,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -139,9 +137,8 @@ struct Work {
   int* g_start;  // [G+1]
   int* g_e1;
   int* g_e2;
-  int* mem_flag;  // [n_bs+1] 1 where a membership (run of one row group) starts
-  int* mem_idx;   // [n_bs+1] exclusive scan of mem_flag
-  int* mem_ptr;   // [E+1]
+  int* mem_idx;   // [n_bs+1] memberships before each basis spec; Elem X's are [mem_idx[bs_off[X]], mem_idx[bs_off[X+1]])
+  long long* tile_sum;  // per-tile sums of the prefix sum being computed
   int* mem_group;
   int* mem_bsoff;
   int64_t* row_len;  // [N+1]
@@ -393,8 +390,7 @@ __device__ __forceinline__ void pair_values(const double* __restrict__ lib, unsi
 // Prep kernels
 // ---------------------------------------------------------------------------
 
-__global__ void k_init(Work w, int n_elems, long long n_dofs, long long n_bs) {
-  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void init_body(const Work& w, int n_elems, long long n_dofs, long long t) {
   if (t < n_elems) {
     w.depth[t] = 0;
     w.n_desc[t] = 0;
@@ -407,8 +403,6 @@ __global__ void k_init(Work w, int n_elems, long long n_dofs, long long n_bs) {
   if (t == 0) {
     *w.err = 0;
     *w.n_large = 0;
-    w.mem_flag[n_bs] = 0;
-    w.row_len[n_dofs] = 0;
   }
 }
 
@@ -420,8 +414,7 @@ __device__ __forceinline__ double map_range(double val, double in_min, double in
 // Elem geometry: parametric range from the chain of HRefLocs (elem.rs:191-197,
 // h_refinement.rs:247-279), Jacobian over that range (element.rs:33-50),
 // inverse/determinant (space.rs:138-147) and the Elem's dyadic position.
-__global__ void k_elem_prep(DevView v, Work w) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void elem_prep_body(const DevView& v, const Work& w, int e) {
   if (e >= v.n_elems) return;
   unsigned char locs[kMaxDepth];
   int depth = 0;
@@ -484,19 +477,9 @@ __global__ void k_elem_prep(DevView v, Work w) {
   w.geo[e] = g;
 }
 
-// nested-list length of each Elem: itself, its ancestors, its descendants
-// (read by the scan that places the lists; 0 past the last Elem)
-struct NestedLen {
-  const int* depth;
-  const int* n_desc;
-  int n_elems;
-  __host__ __device__ int operator()(int e) const { return e < n_elems ? 1 + depth[e] + n_desc[e] : 0; }
-};
-
 // Each Elem writes itself and its ancestors into its own list (after the
 // descendant part) and itself into each ancestor's descendant part.
-__global__ void k_nested_fill(DevView v, Work w) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nested_fill_body(const DevView& v, const Work& w, int e) {
   if (e >= v.n_elems) return;
   const int depth = w.depth[e];
   const int own = w.nested_ptr[e] + w.n_desc[e];
@@ -510,10 +493,8 @@ __global__ void k_nested_fill(DevView v, Work w) {
 
 // Per basis spec: owning Elem, packed spec, and the DoF's spec slots (one
 // wavefront per Elem, lanes over its spec list).
-__global__ void k_bs_slots(DevView v, Work w) {
+__device__ __forceinline__ void bs_slots_body(const DevView& v, const Work& w, long long wave, long long waves) {
   const int lane = threadIdx.x % kWave;
-  const long long wave = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave;
-  const long long waves = (gridDim.x * (long long)blockDim.x) / kWave;
   for (long long e = wave; e < v.n_elems; e += waves) {
     const long long t0 = v.bs_off[e], t1 = v.bs_off[e + 1];
     for (long long t = t0 + lane; t < t1; t += kWave) {
@@ -546,81 +527,240 @@ __device__ __forceinline__ void dof_sig(const Work& w, long long d, int& e1, int
   k2 = s1 >= 0 ? int(s1 & 0xffffffff) : -1;
 }
 
-__global__ void k_dof_flags(Work w) {
-  const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (d >= w.n_dofs) return;
-  int e1, k1, e2, k2;
-  dof_sig(w, d, e1, k1, e2, k2);
-  if (e1 < 0 || (e2 >= 0 && e2 == e1)) set_err(w.err, kErrView);  // no spec, or two on one Elem
-  w.dof_e1[d] = e1;
-  w.dof_k1[d] = k1;
-  w.dof_e2[d] = e2;
-  w.dof_k2[d] = k2;
-  int flag = 1;
-  if (d > 0) {
-    int p1, pk1, p2, pk2;
-    dof_sig(w, d - 1, p1, pk1, p2, pk2);
-    flag = (p1 != e1 || p2 != e2) ? 1 : 0;
-  }
-  w.dof_flag[d] = flag;
+// ---------------------------------------------------------------------------
+// Prefix sums with fused producers and consumers (reduce, then scan):
+// Op::load(i) yields item i (no side effects: it runs in both kernels) and
+// Op::store(i, exclusive, item) consumes the result. k_scan_reduce writes the
+// sum of every tile of kScanTile items; k_scan_apply derives each tile's
+// prefix from those sums and scans the tile in kScanRounds striped rounds of
+// kBlock consecutive items (coalesced loads and stores). No block waits on
+// another.
+// ---------------------------------------------------------------------------
+constexpr int kScanRounds = 4;
+constexpr int kScanTile = kBlock * kScanRounds;
+
+__host__ __device__ constexpr long long scan_tiles(long long n) { return (n + kScanTile - 1) / kScanTile; }
+
+// block-wide sum; every thread gets the total (kBlock threads, shared words sw[kWavesPerBlock])
+__device__ __forceinline__ long long block_sum(long long x, long long* sw) {
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+  if (lane == 0) sw[wid] = x;
+  __syncthreads();
+  long long t = 0;
+#pragma unroll
+  for (int i = 0; i < kWavesPerBlock; ++i) t += sw[i];
+  __syncthreads();
+  return t;
 }
 
-// Row-group starts and Elems (threads over DoFs), and each Elem's basis-spec
-// list as runs of whole row groups, "memberships" (threads over specs): the
-// first spec of every run is flagged, and each run must be one complete group
-// (a run starts at a group's first DoF and ends at its last).
-__global__ void k_groups_memberships(DevView v, Work w) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t < w.n_dofs) {
-    const long long d = t;
-    if (w.dof_flag[d]) {
-      const int g = w.dof_grp[d] - 1;
-      w.g_start[g] = int(d);
-      w.g_e1[g] = w.dof_e1[d];
-      w.g_e2[g] = w.dof_e2[d];
+// tile sums, by blocks bid = 0..nblk-1 of the launch's share for this scan
+template <class Op>
+__device__ __forceinline__ void scan_reduce_body(long long n, const Op& op, long long* tile_sum, long long bid,
+                                                 long long nblk) {
+  __shared__ long long sw[kWavesPerBlock];
+  const long long n_tiles = scan_tiles(n);
+  for (long long tile = bid; tile < n_tiles; tile += nblk) {
+    long long sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanRounds; ++k) {
+      const long long i = tile * kScanTile + (long long)k * kBlock + threadIdx.x;
+      if (i < n) sum += op.load(i);
     }
-    if (d == w.n_dofs - 1) w.g_start[w.dof_grp[d]] = int(w.n_dofs);
+    sum = block_sum(sum, sw);
+    if (threadIdx.x == 0) tile_sum[tile] = sum;
   }
-  if (t >= v.n_bs) return;
-  const int e = w.bs_elem[t];
-  const int d = v.bs_dof[t];
-  if (d < 0 || d >= v.n_dofs) {
-    w.mem_flag[t] = 0;
-    return;
-  }
-  const int g = w.dof_grp[d] - 1;
-  const bool first = t == v.bs_off[e];
-  const bool last = t + 1 == v.bs_off[e + 1];
-  bool start = first;
-  if (!first) {
-    const int dp = v.bs_dof[t - 1];
-    start = !(dp >= 0 && dp < v.n_dofs && dp + 1 == d && w.dof_grp[dp] - 1 == g);
-  }
-  bool end = last;
-  if (!last) {
-    const int dn = v.bs_dof[t + 1];
-    end = !(dn >= 0 && dn < v.n_dofs && dn == d + 1 && w.dof_grp[dn] - 1 == g);
-  }
-  const bool group_first = w.dof_flag[d] != 0;
-  const bool group_last = d + 1 == v.n_dofs || w.dof_flag[d + 1] != 0;
-  if ((start && !group_first) || (end && !group_last)) set_err(w.err, kErrView);
-  w.mem_flag[t] = start ? 1 : 0;
 }
 
-__global__ void k_mem_fill(DevView v, Work w) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t < v.n_bs && w.mem_flag[t]) {
-    const int m = w.mem_idx[t];
-    const int e = w.bs_elem[t];
-    w.mem_group[m] = w.dof_grp[v.bs_dof[t]] - 1;
-    w.mem_bsoff[m] = int(t - v.bs_off[e]);
+// All loads of a tile are issued before any result is needed; the rounds are
+// scanned within each wave, and one exchange of wave totals through LDS gives
+// every round its offset.
+template <class Op>
+__device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const long long* tile_sum, long long bid,
+                                                long long nblk) {
+  __shared__ long long sw[kScanRounds][kWavesPerBlock];
+  __shared__ long long sp[kWavesPerBlock];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const long long n_tiles = scan_tiles(n);
+  for (long long tile = bid; tile < n_tiles; tile += nblk) {
+    const long long base = tile * kScanTile + threadIdx.x;
+    long long item[kScanRounds], incl[kScanRounds];
+#pragma unroll
+    for (int k = 0; k < kScanRounds; ++k) {
+      const long long i = base + (long long)k * kBlock;
+      item[k] = i < n ? op.load(i) : 0;
+      incl[k] = item[k];
+    }
+    long long pre = 0;  // sum of the tiles before this one
+    for (long long t = threadIdx.x; t < tile; t += kBlock) pre += tile_sum[t];
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) pre += __shfl_xor(pre, d, kWave);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+#pragma unroll
+      for (int k = 0; k < kScanRounds; ++k) {
+        const long long y = __shfl_up(incl[k], d, kWave);
+        if (lane >= d) incl[k] += y;
+      }
+    }
+    if (lane == kWave - 1) {
+#pragma unroll
+      for (int k = 0; k < kScanRounds; ++k) sw[k][wid] = incl[k];
+    }
+    if (lane == 0) sp[wid] = pre;
+    __syncthreads();
+    long long carry = 0;
+#pragma unroll
+    for (int v = 0; v < kWavesPerBlock; ++v) carry += sp[v];
+#pragma unroll
+    for (int k = 0; k < kScanRounds; ++k) {
+      long long off = carry;
+#pragma unroll
+      for (int v = 0; v < kWavesPerBlock; ++v) {
+        off += v < wid ? sw[k][v] : 0;
+        carry += sw[k][v];
+      }
+      const long long i = base + (long long)k * kBlock;
+      if (i < n) op.store(i, off + incl[k] - item[k], item[k]);
+    }
+    __syncthreads();  // sw and sp are rewritten by the next tile
   }
-  if (t <= v.n_elems) w.mem_ptr[t] = w.mem_idx[v.bs_off[t]];
 }
+
+// DoF signatures -> row groups. Item d is 1 where a row group starts (its
+// pair of Elems differs from DoF d - 1's); the inclusive sum is the group id.
+struct DofGroupScan {
+  Work w;
+  __device__ long long load(long long d) const {
+    if (d == 0) return 1;
+    int e1, k1, e2, k2, p1, pk1, p2, pk2;
+    dof_sig(w, d, e1, k1, e2, k2);
+    dof_sig(w, d - 1, p1, pk1, p2, pk2);
+    return (p1 != e1 || p2 != e2) ? 1 : 0;
+  }
+  __device__ void store(long long d, long long excl, long long flag) const {
+    int e1, k1, e2, k2;
+    dof_sig(w, d, e1, k1, e2, k2);
+    if (e1 < 0 || (e2 >= 0 && e2 == e1)) set_err(w.err, kErrView);  // no spec, or two on one Elem
+    w.dof_e1[d] = e1;
+    w.dof_k1[d] = k1;
+    w.dof_e2[d] = e2;
+    w.dof_k2[d] = k2;
+    const int g1 = int(excl + flag);  // group id + 1
+    w.dof_grp[d] = g1;
+    w.dof_flag[d] = int(flag);
+    if (flag) {
+      w.g_start[g1 - 1] = int(d);
+      w.g_e1[g1 - 1] = e1;
+      w.g_e2[g1 - 1] = e2;
+    }
+    if (d == w.n_dofs - 1) w.g_start[g1] = int(w.n_dofs);
+  }
+};
+
+// Each Elem's basis-spec list as runs of whole row groups ("memberships").
+// Item t is 1 at the first spec of a run; each run must be one complete group
+// (it starts at the group's first DoF and ends at its last). The exclusive sum
+// numbers the memberships; item n_bs (0) gives their total.
+struct MembershipScan {
+  DevView v;
+  Work w;
+  __device__ bool run_start(long long t, int& d, int& g, int& e) const {
+    e = w.bs_elem[t];
+    d = v.bs_dof[t];
+    if (d < 0 || d >= v.n_dofs) return false;  // flagged by k_bs_slots
+    g = w.dof_grp[d] - 1;
+    if (t == v.bs_off[e]) return true;
+    const int dp = v.bs_dof[t - 1];
+    return !(dp >= 0 && dp < v.n_dofs && dp + 1 == d && w.dof_grp[dp] - 1 == g);
+  }
+  __device__ long long load(long long t) const {
+    if (t >= v.n_bs) return 0;
+    int d, g, e;
+    return run_start(t, d, g, e) ? 1 : 0;
+  }
+  __device__ void store(long long t, long long excl, long long start) const {
+    w.mem_idx[t] = int(excl);
+    if (t >= v.n_bs) return;
+    int d, g, e;
+    run_start(t, d, g, e);
+    if (d < 0 || d >= v.n_dofs) return;
+    bool end = t + 1 == v.bs_off[e + 1];
+    if (!end) {
+      const int dn = v.bs_dof[t + 1];
+      end = !(dn >= 0 && dn < v.n_dofs && dn == d + 1 && w.dof_grp[dn] - 1 == g);
+    }
+    const bool group_first = w.dof_flag[d] != 0;
+    const bool group_last = d + 1 == v.n_dofs || w.dof_flag[d + 1] != 0;
+    if ((start && !group_first) || (end && !group_last)) set_err(w.err, kErrView);
+    if (start) {
+      w.mem_group[excl] = g;
+      w.mem_bsoff[excl] = int(t - v.bs_off[e]);
+    }
+  }
+};
+
+// nested-list offsets: item e is the length 1 + depth + descendants of Elem e
+struct NestedScan {
+  Work w;
+  int n_elems;
+  __device__ long long load(long long e) const { return e < n_elems ? 1 + w.depth[e] + w.n_desc[e] : 0; }
+  __device__ void store(long long e, long long excl, long long) const { w.nested_ptr[e] = int(excl); }
+};
+
+// CSR layout by row groups. Item g is the length of group g's segment: row q
+// of a streaming-fill group holds T - q entries, so the segment is
+// nrows T - nrows (nrows - 1) / 2 long; a block-path group sums the row
+// lengths its count kernel wrote. The exclusive sum is the segment's first
+// entry. The store writes the group id for the chunks of kChunk entries that
+// start inside the segment and, for a block-path group, row_ptr of its rows
+// (the streaming fill writes those of the other groups). The item past the
+// last group carries nnz and writes the call's status block (nnz, error bits,
+// block-path groups, row groups).
+struct GroupScan {
+  Work w;
+  long long n_items;  // row groups the host sized the buffers for, plus one
+  long long* status;
+  __device__ long long seg_len(int g, int r0, int nrows, int4 hdr) const {
+    if (hdr.x > 0) return (long long)nrows * hdr.x - (long long)nrows * (nrows - 1) / 2;
+    long long len = 0;
+    for (int q = 0; q < nrows; ++q) len += w.row_len[r0 + q];
+    return len;
+  }
+  __device__ long long load(long long g) const {
+    if (*w.err || g >= n_groups_of(w)) return 0;
+    const int r0 = w.g_start[g];
+    return seg_len(int(g), r0, w.g_start[g + 1] - r0, w.meta_hdr[g]);
+  }
+  __device__ void store(long long g, long long excl, long long len) const {
+    w.seg_start[g] = excl;
+    const int n_groups = n_groups_of(w);
+    if (g == n_items - 1) {
+      w.row_ptr[w.n_dofs] = excl;
+      status[0] = excl;
+      status[1] = *w.err;
+      status[2] = *w.n_large;
+      status[3] = n_groups;
+    }
+    if (*w.err || g >= n_groups) return;  // an invalid view: the group records are not trustworthy
+    const int4 hdr = w.meta_hdr[g];
+    if (hdr.x <= 0) {  // a block-path group (the streaming fill writes the row pointers of the others)
+      const int r0 = w.g_start[g], nrows = w.g_start[g + 1] - r0;
+      long long at = excl;
+      for (int q = 0; q < nrows; ++q) {
+        w.row_ptr[r0 + q] = at;
+        at += w.row_len[r0 + q];
+      }
+    }
+    for (long long c = (excl + kChunk - 1) / kChunk; c * kChunk < excl + len && c < w.chunk_cap; ++c)
+      w.chunk_first[c] = int(g);
+  }
+};
 
 // 1D Gram library: lib[axis][range][kind][a][b] = sum_m w_m f_a(x'_m) g_b(x_m)
 // with x'_m = x_m * s + c mapped into the dyadic sub-range (glq.rs:238-249).
-__global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
+__device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, const GlqArgs& glq, long long block) {
   __shared__ double s_pts[2][kMaxGlq], s_wts[2][kMaxGlq];
   for (int i = threadIdx.x; i < kMaxGlq; i += blockDim.x) {
     s_pts[0][i] = glq.u_pts[i];
@@ -630,7 +770,7 @@ __global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
   }
   __syncthreads();
   const long long per_axis = (long long)L.nr * kKinds * L.p1 * L.p1;
-  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long t = block * blockDim.x + threadIdx.x;
   if (t >= 2 * per_axis) return;
   // layout [axis][range][kind][a][b]; a transposed copy ([b][a]) follows the
   // whole table so that both orientations of a pair read along b
@@ -664,6 +804,71 @@ __global__ void k_gram_lib(Work w, LibShape L, GlqArgs glq) {
   }
   w.lib[t] = acc;
   w.lib[2 * per_axis + t - b - a * L.p1 + b * L.p1 + a] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Preparation launches. Work with no dependence between its parts shares one
+// launch, split by block ranges (each launch costs a few microseconds on its
+// own, more than most of these parts).
+// ---------------------------------------------------------------------------
+
+// counters and spec slots cleared
+__global__ void __launch_bounds__(kBlock) k_init(Work w, int n_elems, long long n_dofs) {
+  init_body(w, n_elems, n_dofs, blockIdx.x * (long long)kBlock + threadIdx.x);
+}
+
+// The Gram library (needed from the describe kernel on; its blocks come first
+// so that they are dispatched first, being the longest); Elem geometry and
+// depth; per-spec owners and the DoFs' spec slots
+__global__ void __launch_bounds__(kBlock) k_gram_elem_slots(DevView v, Work w, unsigned gram_blocks,
+                                                            unsigned prep_blocks, LibShape L, GlqArgs glq) {
+  if (blockIdx.x < gram_blocks) {
+    gram_body(w, L, glq, blockIdx.x);
+  } else if (blockIdx.x < gram_blocks + prep_blocks) {
+    elem_prep_body(v, w, int((blockIdx.x - gram_blocks) * kBlock + threadIdx.x));
+  } else {
+    const unsigned first = gram_blocks + prep_blocks;
+    const long long waves = (long long)(gridDim.x - first) * kWavesPerBlock;
+    bs_slots_body(v, w, (long long)(blockIdx.x - first) * kWavesPerBlock + threadIdx.x / kWave, waves);
+  }
+}
+
+// two independent prefix sums in one launch (first pass, then second pass)
+template <class OpA, class OpB>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce2(long long na, OpA a, long long* ts_a, unsigned nba,
+                                                         long long nb, OpB b, long long* ts_b) {
+  if (blockIdx.x < nba)
+    scan_reduce_body(na, a, ts_a, blockIdx.x, nba);
+  else
+    scan_reduce_body(nb, b, ts_b, blockIdx.x - nba, gridDim.x - nba);
+}
+
+template <class OpA, class OpB>
+__global__ void __launch_bounds__(kBlock) k_scan_apply2(long long na, OpA a, const long long* ts_a, unsigned nba,
+                                                        long long nb, OpB b, const long long* ts_b) {
+  if (blockIdx.x < nba)
+    scan_apply_body(na, a, ts_a, blockIdx.x, nba);
+  else
+    scan_apply_body(nb, b, ts_b, blockIdx.x - nba, gridDim.x - nba);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(long long n, Op op, long long* tile_sum) {
+  scan_reduce_body(n, op, tile_sum, blockIdx.x, gridDim.x);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_scan_apply(long long n, Op op, const long long* tile_sum) {
+  scan_apply_body(n, op, tile_sum, blockIdx.x, gridDim.x);
+}
+
+// the nested lists; the first pass of the membership numbering
+__global__ void __launch_bounds__(kBlock) k_nested_members(DevView v, Work w, unsigned fill_blocks, long long n_mem,
+                                                           MembershipScan mem, long long* ts_mem) {
+  if (blockIdx.x < fill_blocks)
+    nested_fill_body(v, w, int(blockIdx.x * kBlock + threadIdx.x));
+  else
+    scan_reduce_body(n_mem, mem, ts_mem, blockIdx.x - fill_blocks, gridDim.x - fill_blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -745,14 +950,14 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
   int cnt = 0, X = -1;
   if (lane < n_src) {
     X = S.src_x[lane];
-    cnt = w.mem_ptr[X + 1] - w.mem_ptr[X];
+    cnt = w.mem_idx[v.bs_off[X + 1]] - w.mem_idx[v.bs_off[X]];
   }
   const int incl = wave_incl_scan(cnt, lane);
   const int n_raw = __shfl(incl, kWave - 1, kWave);
   if (n_raw > kWR) return 1;
   if (lane < n_src) {
     const int base = incl - cnt;
-    const int m0 = w.mem_ptr[X];
+    const int m0 = w.mem_idx[v.bs_off[X]];
     for (int m = 0; m < cnt; ++m) {
       const int r = base + m;
       S.u.sort.key[r] = ((unsigned long long)(unsigned)w.mem_group[m0 + m] << 20) | (unsigned long long)r;
@@ -913,7 +1118,6 @@ __global__ void __launch_bounds__(kBlock) k_describe_wave(DevView v, Work w) {
       const long long b2 = e2 >= 0 ? v.bs_off[e2] : 0;
       for (int q = lane; q < nrows; q += kWave) {
         const int r = r0 + q;
-        w.row_len[r] = (int64_t)(T - q);
         w.meta_row[(long long)g * kWRows + q] =
             make_uint2(w.bs_pack[b1 + w.dof_k1[r]], e2 >= 0 ? w.bs_pack[b2 + w.dof_k2[r]] : 0u);
       }
@@ -936,20 +1140,6 @@ __global__ void __launch_bounds__(kBlock) k_describe_wave(DevView v, Work w) {
     }
     wave_sync();
   }
-}
-
-// First CSR entry of every row group, and for each chunk of kChunk entries the
-// row group that holds its first entry (one thread per row group).
-__global__ void k_segments(Work w) {
-  if (*w.err) return;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n_groups = n_groups_of(w);
-  if (g >= n_groups) return;
-  const int64_t s0 = w.row_ptr[w.g_start[g]];
-  const int64_t s1 = w.row_ptr[w.g_start[g + 1]];
-  w.seg_start[g] = s0;
-  if (g == n_groups - 1) w.seg_start[n_groups] = s1;
-  for (long long c = (s0 + kChunk - 1) / kChunk; c * kChunk < s1 && c < w.chunk_cap; ++c) w.chunk_first[c] = g;
 }
 
 __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int c, double cc, double l2) {
@@ -1037,6 +1227,18 @@ __global__ void __launch_bounds__(kBlock) k_fill_flat(Work w, LibShape L, FillOu
   __shared__ double4 src_lds[kWavesPerBlock][kWS * kCoef / 4];
   const long long nnz = w.row_ptr[w.n_dofs];
   if (nnz > o.capacity) return;  // the host grows the buffers and runs the fill again
+  // row pointers of the streaming-fill groups' rows: row q of group g starts
+  // q T - q (q - 1) / 2 entries into the group's segment (the layout scan
+  // writes those of block-path groups)
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < w.n_dofs;
+       r += (long long)gridDim.x * blockDim.x) {
+    const int g = w.dof_grp[r] - 1;
+    const int T = w.meta_hdr[g].x;
+    if (T > 0) {
+      const long long q = r - w.g_start[g];
+      w.row_ptr[r] = w.seg_start[g] + q * T - q * (q - 1) / 2;
+    }
+  }
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
   const int waves = int((gridDim.x * blockDim.x) / kWave);
@@ -1132,7 +1334,7 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
     for (int s = 0; s < S.n_src; ++s) {
       S.raw_cnt[s] = tot;
       const int X = S.src_x[s];
-      tot += w.mem_ptr[X + 1] - w.mem_ptr[X];
+      tot += w.mem_idx[v.bs_off[X + 1]] - w.mem_idx[v.bs_off[X]];
     }
     S.raw_cnt[S.n_src] = tot;
     if (tot > kMaxRaw) S.fail = 1;
@@ -1150,7 +1352,7 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
   const int n_src = S.n_src, n_raw = S.n_raw;
   for (int s = tid; s < n_src; s += blockDim.x) {
     const int X = S.src_x[s];
-    const int m0 = w.mem_ptr[X], m1 = w.mem_ptr[X + 1];
+    const int m0 = w.mem_idx[v.bs_off[X]], m1 = w.mem_idx[v.bs_off[X + 1]];
     int r = S.raw_cnt[s];
     for (int m = m0; m < m1; ++m, ++r) {
       S.key[r] = ((unsigned long long)(unsigned)w.mem_group[m] << 20) | (unsigned long long)r;
@@ -1314,14 +1516,6 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
   }
 }
 
-// Status of the call into the context's mapped pinned block: nnz, error bits,
-// large-group count, row groups. The host reads it after the stream sync.
-__global__ void k_finish(Work w, long long* status) {
-  status[0] = w.row_ptr[w.n_dofs];
-  status[1] = *w.err;
-  status[2] = *w.n_large;
-  status[3] = n_groups_of(w);
-}
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -1380,7 +1574,6 @@ struct fem2d_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<DevBuf> bufs;  // grow-only work buffers, by slot
-  DevBuf scan_tmp;
   DevBuf out_col, out_a, out_b;
   long long out_capacity = 0;
   long long* h_status = nullptr;  // mapped pinned status block: nnz, err, n_large, n_groups
@@ -1558,7 +1751,7 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& b : ctx->bufs)
     if (b.p) (void)hipFree(b.p);
-  for (DevBuf* b : {&ctx->scan_tmp, &ctx->out_col, &ctx->out_a, &ctx->out_b})
+  for (DevBuf* b : {&ctx->out_col, &ctx->out_a, &ctx->out_b})
     if (b->p) (void)hipFree(b->p);
   for (auto& g : ctx->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -1739,8 +1932,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       (rc = slot(ctx, k++, N1, &w.dof_k2)) || (rc = slot(ctx, k++, N1, &w.dof_flag)) ||
       (rc = slot(ctx, k++, N1, &w.dof_grp)) || (rc = slot(ctx, k++, N1, &w.g_start)) ||
       (rc = slot(ctx, k++, N1, &w.g_e1)) || (rc = slot(ctx, k++, N1, &w.g_e2)) ||
-      (rc = slot(ctx, k++, B1, &w.mem_flag)) || (rc = slot(ctx, k++, B1, &w.mem_idx)) ||
-      (rc = slot(ctx, k++, size_t(ne + 1), &w.mem_ptr)) || (rc = slot(ctx, k++, B1, &w.mem_group)) ||
+      (rc = slot(ctx, k++, B1, &w.mem_idx)) || (rc = slot(ctx, k++, B1, &w.mem_group)) ||
       (rc = slot(ctx, k++, B1, &w.mem_bsoff)) || (rc = slot(ctx, k++, N1, &w.row_len)) ||
       (rc = slot(ctx, k++, N1, &w.row_ptr)) || 
       (rc = slot(ctx, k++, size_t(8), &w.err)) || (rc = slot(ctx, k++, N1, &w.large_list)) ||
@@ -1768,20 +1960,18 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   w.lib_nr = L.nr;
   if ((rc = slot(ctx, k++, size_t(2 * lib_n), &w.lib))) return rc;  // table and its transpose
 
-  // scan temporaries
-  size_t tmp_bytes = 0, t2 = 0;
-  HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, t2, w.dof_flag, w.dof_grp, int(nd), ctx->stream));
-  tmp_bytes = std::max(tmp_bytes, t2);
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.mem_flag, w.mem_idx, int(nbs + 1), ctx->stream));
-  tmp_bytes = std::max(tmp_bytes, t2);
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, w.row_len, w.row_ptr, int(nd + 1), ctx->stream));
-  tmp_bytes = std::max(tmp_bytes, t2);
-  const hipcub::TransformInputIterator<int, NestedLen, hipcub::CountingInputIterator<int>> nested_len(
-      hipcub::CountingInputIterator<int>(0), NestedLen{w.depth, w.n_desc, ne});
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, nested_len, w.nested_ptr, ne + 1, ctx->stream));
-  tmp_bytes = std::max(tmp_bytes, t2);
-  if ((rc = ensure(ctx->scan_tmp, tmp_bytes))) return rc;
-  const long long elem_waves_blocks = std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock);
+  // tile sums of the four prefix sums, one region each (two run side by side)
+  const long long n_grp_items = (long long)G1 + 1;
+  const long long tiles_nested = scan_tiles(ne + 1), tiles_dof = scan_tiles(nd), tiles_mem = scan_tiles(nbs + 1),
+                  tiles_grp = scan_tiles(n_grp_items);
+  if ((rc = slot(ctx, k++, size_t(tiles_nested + tiles_dof + tiles_mem + tiles_grp), &w.tile_sum))) return rc;
+  long long* ts_nested = w.tile_sum;
+  long long* ts_dof = ts_nested + tiles_nested;
+  long long* ts_mem = ts_dof + tiles_dof;
+  long long* ts_grp = ts_mem + tiles_mem;
+  auto scan_grid = [&](long long tiles) { return unsigned(std::max(1LL, std::min(tiles, 8LL * ctx->n_cu))); };
+  const unsigned slot_blocks =
+      unsigned(std::min<long long>(std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock), 1 << 16));
 
   hipStream_t st = ctx->stream;
   // the previous call's readback of the pinned staging buffers has completed
@@ -1810,26 +2000,27 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   // the block path runs unless this Domain is known to have no large groups
   const bool run_large = dd->large_state != 0;
 
-  // The assembly in three parts: (a) preparation and symbolic phase, (b) the
-  // streaming fill, (c) the block-path fill and the status block. With graphs
+  // The assembly in three parts: (a) preparation and symbolic phase, ending
+  // with the CSR layout and the status block, (b) the streaming fill, (c) the
+  // block-path fill (only for Domains with large row groups). With graphs
   // each part is recorded once and replayed while the Domain, the options and
   // the buffers stay the same; the timing events sit between the parts, so the
   // fill kernel is timed on its own in either mode.
   auto part_a = [&](const FillOut& o, bool fine_events) -> int {
-    k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd, nbs);
-    k_gram_lib<<<grid_for(lib_n, B), B, 0, st>>>(w, L, ctx->glq);
-    k_elem_prep<<<grid_for(ne, B), B, 0, st>>>(v, w);
+    k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd);
+    const unsigned gram_blocks = grid_for(lib_n, B), prep_blocks = grid_for(ne, B);
+    k_gram_elem_slots<<<gram_blocks + prep_blocks + slot_blocks, B, 0, st>>>(v, w, gram_blocks, prep_blocks, L,
+                                                                              ctx->glq);
     HIP_TRY_AT(hipGetLastError(), "after the Elem preparation kernels");
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, nested_len, w.nested_ptr, ne + 1, st));
-    k_nested_fill<<<grid_for(ne, B), B, 0, st>>>(v, w);
-    k_bs_slots<<<unsigned(std::min<long long>(elem_waves_blocks, 1 << 16)), kBlock, 0, st>>>(v, w);
-    k_dof_flags<<<grid_for(nd, B), B, 0, st>>>(w);
-    HIP_TRY_AT(hipGetLastError(), "after the nested-list, spec-slot and DoF kernels");
-    HIP_TRY(hipcub::DeviceScan::InclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.dof_flag, w.dof_grp, int(nd), st));
-    k_groups_memberships<<<grid_for(std::max<long long>(nd, nbs), B), B, 0, st>>>(v, w);
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.mem_flag, w.mem_idx, int(nbs + 1), st));
-    k_mem_fill<<<grid_for(std::max<long long>(nbs, ne + 1), B), B, 0, st>>>(v, w);
-    HIP_TRY_AT(hipGetLastError(), "after the row-group kernels");
+    // nested-list offsets and row groups side by side
+    const unsigned gn = scan_grid(tiles_nested), gd = scan_grid(tiles_dof), gm = scan_grid(tiles_mem);
+    k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{w}, ts_dof);
+    k_scan_apply2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{w}, ts_dof);
+    // nested lists beside the first pass of the membership numbering
+    const unsigned fill_blocks = grid_for(ne, B);
+    k_nested_members<<<fill_blocks + gm, kBlock, 0, st>>>(v, w, fill_blocks, nbs + 1, MembershipScan{v, w}, ts_mem);
+    k_scan_apply<<<gm, kBlock, 0, st>>>(nbs + 1, MembershipScan{v, w}, ts_mem);
+    HIP_TRY_AT(hipGetLastError(), "after the nested-list and row-group kernels");
     if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[1], st));
     // symbolic: per-group candidate sets, row lengths and records, then row_ptr
     if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[4], st));
@@ -1837,10 +2028,12 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[5], st));
     if (run_large) k_count_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w);
     HIP_TRY_AT(hipGetLastError(), "after the describe kernels");
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, w.row_len, w.row_ptr, int(nd + 1), st));
+    // CSR layout and the status block
+    const GroupScan gs{w, n_grp_items, ctx->d_status};
+    k_scan_reduce<<<scan_grid(tiles_grp), kBlock, 0, st>>>(n_grp_items, gs, ts_grp);
+    k_scan_apply<<<scan_grid(tiles_grp), kBlock, 0, st>>>(n_grp_items, gs, ts_grp);
+    HIP_TRY_AT(hipGetLastError(), "after the CSR layout scan");
     if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[2], st));
-    if (o.capacity > 0) k_segments<<<grid_for(std::max(1LL, dd->n_groups), B), B, 0, st>>>(w);
-    HIP_TRY_AT(hipGetLastError(), "after the segment kernel");
     return FEM2D_OK;
   };
   // numeric: columns and values; with too small output buffers the fill
@@ -1851,9 +2044,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     return FEM2D_OK;
   };
   auto part_c = [&](const FillOut& o) -> int {
-    if (o.capacity > 0 && run_large) k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
-    k_finish<<<1, 1, 0, st>>>(w, ctx->d_status);
-    HIP_TRY_AT(hipGetLastError(), "after the status kernel");
+    k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
+    HIP_TRY_AT(hipGetLastError(), "after the block-path fill kernel");
     return FEM2D_OK;
   };
   // record one part into ctx->graph_exec[i]
@@ -1892,12 +2084,14 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     w.chunk_cap = o.capacity / kChunk + 1;
     if ((rc = slot(ctx, chunk_slot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
     const bool fill = o.capacity > 0;
+    const bool tail = fill && run_large;  // part (c) has work
     if (ctx->use_graphs) {
       const GraphKey key{dd->serial, basis, ai, bi, nu, nv, o.capacity, g_alloc_generation.load(), run_large};
       if (!ctx->graph_exec[0] || !(key == ctx->graph_key)) {
         ctx->graph_key = GraphKey{};
         if ((rc = record(0, [&] { return part_a(o, false); })) ||
-            (fill && (rc = record(1, [&] { return part_b(o); }))) || (rc = record(2, [&] { return part_c(o); })))
+            (fill && (rc = record(1, [&] { return part_b(o); }))) ||
+            (tail && (rc = record(2, [&] { return part_c(o); }))))
           return rc;
         ctx->graph_key = key;
       }
@@ -1906,7 +2100,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       HIP_TRY(hipEventRecord(ctx->ev[6], st));
       if (fill) HIP_TRY(hipGraphLaunch(ctx->graph_exec[1], st));
       HIP_TRY(hipEventRecord(ctx->ev[7], st));
-      HIP_TRY(hipGraphLaunch(ctx->graph_exec[2], st));
+      if (tail) HIP_TRY(hipGraphLaunch(ctx->graph_exec[2], st));
       HIP_TRY(hipEventRecord(ctx->ev[3], st));
       fine_timing = false;
     } else {
@@ -1915,7 +2109,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       HIP_TRY(hipEventRecord(ctx->ev[6], st));
       if (fill && (rc = part_b(o))) return rc;
       HIP_TRY(hipEventRecord(ctx->ev[7], st));
-      if ((rc = part_c(o))) return rc;
+      if (tail && (rc = part_c(o))) return rc;
       HIP_TRY(hipEventRecord(ctx->ev[3], st));
       fine_timing = true;
     }
