@@ -71,6 +71,12 @@ constexpr int kMaxSrc = 128;
 constexpr int kMaxRaw = 1024;
 constexpr int kMaxDist = 1024;
 constexpr int kMaxRows = 1024;
+// streaming fill: 8 wavefronts per block share one LDS copy of the Gram
+// library (up to kLibLdsMax values); three blocks per CU (48 KB of LDS each)
+// keep 6 wavefronts per SIMD
+constexpr int kFillBlock = 512;
+constexpr int kFillWavesPerSimd = 6;
+constexpr int kLibLdsMax = 4096;
 constexpr int kMaxDepth = 24;      // h-refinement depth below a base Elem
 constexpr int kMaxRelDepth = 12;   // dyadic halvings per axis covered by the Gram library
 constexpr int kMaxOrder = 20;      // MAX_POLYNOMIAL_ORDER (mesh.rs:42)
@@ -772,8 +778,7 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
   const long long per_axis = (long long)L.nr * kKinds * L.p1 * L.p1;
   long long t = block * blockDim.x + threadIdx.x;
   if (t >= 2 * per_axis) return;
-  // layout [axis][range][kind][a][b]; a transposed copy ([b][a]) follows the
-  // whole table so that both orientations of a pair read along b
+  // layout [axis][range][kind][a][b]
   const int axis = int(t / per_axis);
   long long r = t % per_axis;
   const int b = int(r % L.p1);
@@ -803,7 +808,6 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
     acc += wts[m] * shape_pick(L.basis, fp, a, xp) * shape_pick(L.basis, fq, b, xq);
   }
   w.lib[t] = acc;
-  w.lib[2 * per_axis + t - b - a * L.p1 + b * L.p1 + a] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -980,12 +984,9 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
       out[4] = cB[0];
       out[5] = cB[1];
       // library range bases of the pair and which of the group's Elems it uses
-      // ancestor pairs read the transposed copy, so that the row's spec
-      // always selects the table row and the column's spec the entry in it
       int* tail = reinterpret_cast<int*>(out + 6);
-      const int tcopy = rel == kAnc ? 2 * w.lib_nr * w.lib_range_stride : 0;
-      tail[0] = tcopy + ru * w.lib_range_stride;
-      tail[1] = tcopy + (w.lib_nr + rv) * w.lib_range_stride;
+      tail[0] = ru * w.lib_range_stride;
+      tail[1] = (w.lib_nr + rv) * w.lib_range_stride;
       tail[2] = k | (rel << 1);
       tail[3] = 0;
       S.src_info[lane] = k | (rel << 1) | (ru << 3) | (rv << 17);
@@ -1153,7 +1154,7 @@ __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int
 // contribution of one (row spec, column spec) pair through source s of the
 // group; the source record (staged in LDS) holds the prefactors, the library
 // range bases and which of the group's Elems (k) and which relation (rel) the
-// pair has
+// pair has. lib is the Gram library in global memory or its copy in LDS.
 __device__ __forceinline__ void flat_contrib(const double* __restrict__ lib, unsigned tab,
                                              const double* __restrict__ coef_g, unsigned s, unsigned cbs, uint2 rp,
                                              double& cc, double& l2) {
@@ -1165,8 +1166,9 @@ __device__ __forceinline__ void flat_contrib(const double* __restrict__ lib, uns
   // the p side of the reference's pair is the ancestor's spec
   const unsigned pd = anc ? cd : rd, qd = anc ? rd : cd;
   const unsigned pair = 2u * pd + qd;
-  const unsigned ix = unsigned(tail.x) + spec_ip1(rpk) + spec_i(cbs);
-  const unsigned iy = unsigned(tail.y) + spec_jp1(rpk) + spec_j(cbs);
+  // table row a = p's order, entry b = q's order
+  const unsigned ix = unsigned(tail.x) + (anc ? spec_ip1(cbs) + spec_i(rpk) : spec_ip1(rpk) + spec_i(cbs));
+  const unsigned iy = unsigned(tail.y) + (anc ? spec_jp1(cbs) + spec_j(rpk) : spec_jp1(rpk) + spec_j(cbs));
   const double fx = ld32(lib, ix + __umul24(pair, tab));
   const double fy = ld32(lib, iy + __umul24(3u - pair, tab));
   cc = coef_g[ci + pair] * fx * fy;
@@ -1196,23 +1198,37 @@ __device__ __forceinline__ void segment_entry(int p, int T, int nrows, float bq,
   t = min(max(q + (p - oq), q), T - 1);
 }
 
-// One entry of a row group's segment from the group's records (bases are
-// wave-uniform; offsets are 32-bit).
-__device__ __forceinline__ void flat_entry(const uint4* __restrict__ col_g, const uint2* __restrict__ row_g,
-                                           const double* __restrict__ coef_g, const double* __restrict__ lib,
-                                           unsigned tab, int pl, int T, int nrows, float bq, int& c, double& cc,
-                                           double& l2) {
-  int q, t;
-  segment_entry(pl, T, nrows, bq, q, t);
-  const uint4 rec = ld32(col_g, unsigned(t));
-  const uint2 rp = ld32(row_g, unsigned(q));
-  c = int(rec.x);
-  flat_contrib(lib, tab, coef_g, rec.w & 0xffu, rec.y, rp, cc, l2);
-  if (rec.z != ~0u) {
+// Two entries of a row group's segment from the group's records (bases are
+// wave-uniform; offsets are 32-bit). The two are computed side by side with
+// no branch between them, so that their loads are in flight together; the
+// second contribution (both specs of the row and of the column on the same
+// two Elems, a few entries in a hundred) follows.
+__device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, const uint2* __restrict__ row_g,
+                                            const double* __restrict__ coef_g, const double* __restrict__ lib,
+                                            unsigned tab, int pa, int pb, int T, int nrows, float bq, int& ca,
+                                            double& cca, double& l2a, int& cb, double& ccb, double& l2b) {
+  int qa, ta, qb, tb;
+  segment_entry(pa, T, nrows, bq, qa, ta);
+  segment_entry(pb, T, nrows, bq, qb, tb);
+  const uint4 ra = ld32(col_g, unsigned(ta));
+  const uint4 rb = ld32(col_g, unsigned(tb));
+  const uint2 rpa = ld32(row_g, unsigned(qa));
+  const uint2 rpb = ld32(row_g, unsigned(qb));
+  ca = int(ra.x);
+  cb = int(rb.x);
+  flat_contrib(lib, tab, coef_g, ra.w & 0xffu, ra.y, rpa, cca, l2a);
+  flat_contrib(lib, tab, coef_g, rb.w & 0xffu, rb.y, rpb, ccb, l2b);
+  if (ra.z != ~0u) {
     double cc1, l21;
-    flat_contrib(lib, tab, coef_g, (rec.w >> 8) & 0xffu, rec.z, rp, cc1, l21);
-    cc += cc1;
-    l2 += l21;
+    flat_contrib(lib, tab, coef_g, (ra.w >> 8) & 0xffu, ra.z, rpa, cc1, l21);
+    cca += cc1;
+    l2a += l21;
+  }
+  if (rb.z != ~0u) {
+    double cc1, l21;
+    flat_contrib(lib, tab, coef_g, (rb.w >> 8) & 0xffu, rb.z, rpb, cc1, l21);
+    ccb += cc1;
+    l2b += l21;
   }
 }
 
@@ -1221,12 +1237,20 @@ __device__ __forceinline__ void flat_entry(const uint4* __restrict__ col_g, cons
 // on consecutive entries, reading the row groups' records (no LDS, no
 // per-group set-up, equal work per chunk). Entries of block-path groups are
 // left to k_fill_large.
-__global__ void __launch_bounds__(kBlock) k_fill_flat(Work w, LibShape L, FillOut o) {
+template <bool kLibInLds>
+__global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
   if (*w.err) return;
   // the source records of the row group being written, per wavefront
-  __shared__ double4 src_lds[kWavesPerBlock][kWS * kCoef / 4];
+  __shared__ double4 src_lds[kFillBlock / kWave][kWS * kCoef / 4];
+  // the block's copy of the Gram library (when it fits)
+  __shared__ double lib_lds[kLibInLds ? kLibLdsMax : 1];
   const long long nnz = w.row_ptr[w.n_dofs];
   if (nnz > o.capacity) return;  // the host grows the buffers and runs the fill again
+  if (kLibInLds) {
+    const int n_lib = 2 * L.nr * L.range_stride;
+    for (int i = threadIdx.x; i < n_lib; i += blockDim.x) lib_lds[i] = w.lib[i];
+    __syncthreads();
+  }
   // row pointers of the streaming-fill groups' rows: row q of group g starts
   // q T - q (q - 1) / 2 entries into the group's segment (the layout scan
   // writes those of block-path groups)
@@ -1243,7 +1267,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_flat(Work w, LibShape L, FillOu
   const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
   const int waves = int((gridDim.x * blockDim.x) / kWave);
   const long long n_chunks = min((nnz + kChunk - 1) / kChunk, w.chunk_cap);
-  const double* __restrict__ lib = w.lib;
+  const double* __restrict__ lib = kLibInLds ? lib_lds : w.lib;
   const unsigned tab = unsigned(L.tab);
   for (long long ch = wave; ch < n_chunks; ch += waves) {
     const long long pbase = ch * kChunk;
@@ -1268,13 +1292,14 @@ __global__ void __launch_bounds__(kBlock) k_fill_flat(Work w, LibShape L, FillOu
         for (int i = lane; i < hdr.z * (kCoef / 4); i += kWave) stage[i] = src_g[i];
         wave_sync();
         const double* coef_g = reinterpret_cast<const double*>(stage);
-        // two entries per lane and iteration: their loads overlap
+        // two entries per lane and iteration; past the end of the group's
+        // part of the chunk the second one repeats the first and is not stored
         for (int j = j0 + lane; j < j1; j += 2 * kWave) {
-          int c0, c1 = 0;
-          double cc0, l20, cc1 = 0.0, l21 = 0.0;
-          flat_entry(col_g, row_g, coef_g, lib, tab, j + off, T, nrows, bq, c0, cc0, l20);
+          int c0, c1;
+          double cc0, l20, cc1, l21;
           const bool second = j + kWave < j1;
-          if (second) flat_entry(col_g, row_g, coef_g, lib, tab, j + kWave + off, T, nrows, bq, c1, cc1, l21);
+          flat_entry2(col_g, row_g, coef_g, lib, tab, j + off, (second ? j + kWave : j) + off, T, nrows, bq, c0, cc0,
+                      l20, c1, cc1, l21);
           col_out[unsigned(j)] = c0;
           a_out[unsigned(j)] = o.a_is_cc ? cc0 : l20;
           b_out[unsigned(j)] = o.b_is_cc ? cc0 : l20;
@@ -1582,7 +1607,8 @@ struct fem2d_ctx {
   fem2d_timing timing{};
   int n_cu = 0;
   int count_grid = 0;  // resident blocks of the wave kernels on this device
-  int fill_grid = 0;
+  int fill_grid = 0;      // resident blocks of the streaming fill (library in global memory)
+  int fill_grid_lds = 0;  // the same with the library in LDS
   bool use_graphs = true;
   hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};  // the last recorded assembly, in parts
   GraphKey graph_key;
@@ -1735,8 +1761,10 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_describe_wave, kBlock, 0));
   ctx->count_grid = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat, kBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false>, kFillBlock, 0));
   ctx->fill_grid = std::max(1, per_cu) * ctx->n_cu;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true>, kFillBlock, 0));
+  ctx->fill_grid_lds = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fill_large),
@@ -1958,7 +1986,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   const long long lib_n = 2LL * L.nr * kKinds * L.p1 * L.p1;
   w.lib_range_stride = L.range_stride;
   w.lib_nr = L.nr;
-  if ((rc = slot(ctx, k++, size_t(2 * lib_n), &w.lib))) return rc;  // table and its transpose
+  if ((rc = slot(ctx, k++, size_t(lib_n), &w.lib))) return rc;
 
   // tile sums of the four prefix sums, one region each (two run side by side)
   const long long n_grp_items = (long long)G1 + 1;
@@ -2039,7 +2067,10 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   // numeric: columns and values; with too small output buffers the fill
   // writes nothing and the host grows them and runs the assembly again
   auto part_b = [&](const FillOut& o) -> int {
-    k_fill_flat<<<unsigned(ctx->fill_grid), kBlock, 0, st>>>(w, L, o);
+    if (lib_n <= kLibLdsMax)
+      k_fill_flat<true><<<unsigned(ctx->fill_grid_lds), kFillBlock, 0, st>>>(w, L, o);
+    else
+      k_fill_flat<false><<<unsigned(ctx->fill_grid), kFillBlock, 0, st>>>(w, L, o);
     HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
     return FEM2D_OK;
   };
