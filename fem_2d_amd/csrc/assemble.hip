@@ -904,6 +904,14 @@ __device__ __forceinline__ int gather_sources(const Work& w, int g, int cap, int
 // Wave path: one wavefront per row group
 // ---------------------------------------------------------------------------
 
+// a wave-uniform value loaded through the vector path, moved to a scalar register
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long long uniform(long long x) {
+  const unsigned long long u = (unsigned long long)x;
+  return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(int(u >> 32)) << 32) |
+                     (unsigned)__builtin_amdgcn_readfirstlane(int(u & 0xffffffffu)));
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1275,12 +1283,17 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
     int* __restrict__ col_out = o.col + pbase;
     double* __restrict__ a_out = o.aval + pbase;
     double* __restrict__ b_out = o.bval + pbase;
-    int g = w.chunk_first[ch];
+    // per-group values are wave-uniform: keep them (and the record bases) in
+    // scalar registers
+    int g = uniform(w.chunk_first[ch]);
     int j0 = 0;
     while (j0 < len) {
-      const long long S = w.seg_start[g], S1 = w.seg_start[g + 1];
+      const long long S = uniform((long long)w.seg_start[g]), S1 = uniform((long long)w.seg_start[g + 1]);
       const int j1 = int(min((long long)len, S1 - pbase));
-      const int4 hdr = w.meta_hdr[g];
+      int4 hdr = w.meta_hdr[g];
+      hdr.x = uniform(hdr.x);
+      hdr.y = uniform(hdr.y);
+      hdr.z = uniform(hdr.z);
       if (hdr.x > 0) {
         const int T = hdr.x, nrows = hdr.y;
         const int off = int(pbase - S);  // segment position of chunk entry 0
