@@ -365,6 +365,11 @@ __device__ __forceinline__ unsigned spec_jp1(unsigned x) { return (x >> 20) & 51
 // element `idx` of a uniform base with a 32-bit byte offset (lets the compiler
 // use the scalar-base + 32-bit vector-offset addressing of global loads)
 template <class T>
+__device__ __forceinline__ void st32(T* __restrict__ base, unsigned idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(T))) = v;
+}
+
+template <class T>
 __device__ __forceinline__ T ld32(const T* __restrict__ base, unsigned idx) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * unsigned(sizeof(T)));
 }
@@ -1245,7 +1250,8 @@ __device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, con
 // on consecutive entries, reading the row groups' records (no LDS, no
 // per-group set-up, equal work per chunk). Entries of block-path groups are
 // left to k_fill_large.
-template <bool kLibInLds>
+// kACurlCurl / kBCurlCurl: A and B take the CurlCurl values (else L2Inner)
+template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl>
 __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
   if (*w.err) return;
   // the source records of the row group being written, per wavefront
@@ -1280,9 +1286,10 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
   for (long long ch = wave; ch < n_chunks; ch += waves) {
     const long long pbase = ch * kChunk;
     const int len = int(min((long long)kChunk, nnz - pbase));
-    int* __restrict__ col_out = o.col + pbase;
-    double* __restrict__ a_out = o.aval + pbase;
-    double* __restrict__ b_out = o.bval + pbase;
+    const long long pb = uniform(pbase);
+    int* __restrict__ col_out = o.col + pb;
+    double* __restrict__ a_out = o.aval + pb;
+    double* __restrict__ b_out = o.bval + pb;
     // per-group values are wave-uniform: keep them (and the record bases) in
     // scalar registers
     int g = uniform(w.chunk_first[ch]);
@@ -1313,13 +1320,13 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
           const bool second = j + kWave < j1;
           flat_entry2(col_g, row_g, coef_g, lib, tab, j + off, (second ? j + kWave : j) + off, T, nrows, bq, c0, cc0,
                       l20, c1, cc1, l21);
-          col_out[unsigned(j)] = c0;
-          a_out[unsigned(j)] = o.a_is_cc ? cc0 : l20;
-          b_out[unsigned(j)] = o.b_is_cc ? cc0 : l20;
+          st32(col_out, unsigned(j), c0);
+          st32(a_out, unsigned(j), kACurlCurl ? cc0 : l20);
+          st32(b_out, unsigned(j), kBCurlCurl ? cc0 : l20);
           if (second) {
-            col_out[unsigned(j + kWave)] = c1;
-            a_out[unsigned(j + kWave)] = o.a_is_cc ? cc1 : l21;
-            b_out[unsigned(j + kWave)] = o.b_is_cc ? cc1 : l21;
+            st32(col_out, unsigned(j + kWave), c1);
+            st32(a_out, unsigned(j + kWave), kACurlCurl ? cc1 : l21);
+            st32(b_out, unsigned(j + kWave), kBCurlCurl ? cc1 : l21);
           }
         }
         wave_sync();  // every lane is done with the staged records
@@ -1328,6 +1335,22 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
       ++g;
     }
   }
+}
+
+// one instance per library placement and integral pair
+template <bool kLibInLds>
+void launch_fill_lib(bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
+                     const FillOut& o) {
+  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  else if (a_cc) k_fill_flat<kLibInLds, true, true><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  else k_fill_flat<kLibInLds, false, false><<<grid, kFillBlock, 0, st>>>(w, L, o);
+}
+
+void launch_fill(bool lds, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
+                 const FillOut& o) {
+  if (lds) launch_fill_lib<true>(a_cc, b_cc, grid, st, w, L, o);
+  else launch_fill_lib<false>(a_cc, b_cc, grid, st, w, L, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -1774,9 +1797,9 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_describe_wave, kBlock, 0));
   ctx->count_grid = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false>, kFillBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false>, kFillBlock, 0));
   ctx->fill_grid = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true>, kFillBlock, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false>, kFillBlock, 0));
   ctx->fill_grid_lds = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
@@ -2080,10 +2103,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   // numeric: columns and values; with too small output buffers the fill
   // writes nothing and the host grows them and runs the assembly again
   auto part_b = [&](const FillOut& o) -> int {
-    if (lib_n <= kLibLdsMax)
-      k_fill_flat<true><<<unsigned(ctx->fill_grid_lds), kFillBlock, 0, st>>>(w, L, o);
-    else
-      k_fill_flat<false><<<unsigned(ctx->fill_grid), kFillBlock, 0, st>>>(w, L, o);
+    const bool lds = lib_n <= kLibLdsMax;
+    const unsigned grid = unsigned(lds ? ctx->fill_grid_lds : ctx->fill_grid);
+    launch_fill(lds, o.a_is_cc != 0, o.b_is_cc != 0, grid, st, w, L, o);
     HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
     return FEM2D_OK;
   };
