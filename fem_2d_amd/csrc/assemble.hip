@@ -364,9 +364,10 @@ __device__ __forceinline__ unsigned spec_jp1(unsigned x) { return (x >> 20) & 51
 
 // element `idx` of a uniform base with a 32-bit byte offset (lets the compiler
 // use the scalar-base + 32-bit vector-offset addressing of global loads)
+// streaming store of the output (written once, never read back by the kernel)
 template <class T>
 __device__ __forceinline__ void st32(T* __restrict__ base, unsigned idx, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(T))) = v;
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(T))));
 }
 
 template <class T>
