@@ -1647,7 +1647,7 @@ struct fem2d_ctx {
   int fill_grid = 0;      // resident blocks of the streaming fill (library in global memory)
   int fill_grid_lds = 0;  // the same with the library in LDS
   bool use_graphs = true;
-  hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};  // the last recorded assembly, in parts
+  hipGraphExec_t graph_exec[1] = {nullptr};  // the last recorded assembly
   GraphKey graph_key;
 };
 
@@ -2067,10 +2067,11 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
 
   // The assembly in three parts: (a) preparation and symbolic phase, ending
   // with the CSR layout and the status block, (b) the streaming fill, (c) the
-  // block-path fill (only for Domains with large row groups). With graphs
-  // each part is recorded once and replayed while the Domain, the options and
-  // the buffers stay the same; the timing events sit between the parts, so the
-  // fill kernel is timed on its own in either mode.
+  // block-path fill (only for Domains with large row groups). With graphs the
+  // three are recorded as one graph, replayed while the Domain, the options
+  // and the buffers stay the same, and timed end to end; kernel by kernel,
+  // events between the stages also time the stages and the describe and fill
+  // kernels on their own.
   auto part_a = [&](const FillOut& o, bool fine_events) -> int {
     k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd);
     const unsigned gram_blocks = grid_for(lib_n, B), prep_blocks = grid_for(ne, B);
@@ -2115,7 +2116,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     HIP_TRY_AT(hipGetLastError(), "after the block-path fill kernel");
     return FEM2D_OK;
   };
-  // record one part into ctx->graph_exec[i]
+  // record the work of parts() into ctx->graph_exec[i]
   auto record = [&](int i, const std::function<int()>& part) -> int {
     if (ctx->graph_exec[i]) (void)hipGraphExecDestroy(ctx->graph_exec[i]);
     ctx->graph_exec[i] = nullptr;
@@ -2156,18 +2157,18 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       const GraphKey key{dd->serial, basis, ai, bi, nu, nv, o.capacity, g_alloc_generation.load(), run_large};
       if (!ctx->graph_exec[0] || !(key == ctx->graph_key)) {
         ctx->graph_key = GraphKey{};
-        if ((rc = record(0, [&] { return part_a(o, false); })) ||
-            (fill && (rc = record(1, [&] { return part_b(o); }))) ||
-            (tail && (rc = record(2, [&] { return part_c(o); }))))
+        if ((rc = record(0, [&] {
+               int prc = part_a(o, false);
+               if (prc == FEM2D_OK && fill) prc = part_b(o);
+               if (prc == FEM2D_OK && tail) prc = part_c(o);
+               return prc;
+             })))
           return rc;
         ctx->graph_key = key;
       }
+      // one graph for the whole assembly; only its end points are timed
       HIP_TRY(hipEventRecord(ctx->ev[0], st));
       HIP_TRY(hipGraphLaunch(ctx->graph_exec[0], st));
-      HIP_TRY(hipEventRecord(ctx->ev[6], st));
-      if (fill) HIP_TRY(hipGraphLaunch(ctx->graph_exec[1], st));
-      HIP_TRY(hipEventRecord(ctx->ev[7], st));
-      if (tail) HIP_TRY(hipGraphLaunch(ctx->graph_exec[2], st));
       HIP_TRY(hipEventRecord(ctx->ev[3], st));
       fine_timing = false;
     } else {
@@ -2199,14 +2200,12 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
 
   float t01 = 0, t12 = 0, t23 = 0, t03 = 0, tc = 0, tf = 0;
   (void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
-  (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
   if (fine_timing) {  // per-stage events exist in kernel-by-kernel mode only
+    (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
     (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
     (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
     (void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
     (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
-  } else {
-    (void)hipEventElapsedTime(&t23, ctx->ev[6], ctx->ev[3]);
   }
   (void)hipGetLastError();  // a timing query failure is not an assembly failure
   ctx->timing.prep_ms = t01;

@@ -180,7 +180,10 @@ int fem2d_gep_csr_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep_cs
                                               const char* prefix);
 
 /* Per-stage device time of the last assembly on this context, measured with
- * HIP events recorded on the context's stream (milliseconds). */
+ * HIP events recorded on the context's stream (milliseconds). An assembly that
+ * replays its recorded HIP graph (the default, fem2d_ctx_set_graphs) is timed
+ * end to end only: total_ms; the stage and kernel times are those of
+ * kernel-by-kernel assemblies and read 0 otherwise. */
 typedef struct fem2d_timing {
   double total_ms;    /* first to last event of the assembly */
   double prep_ms;     /* elem geometry, DoF signatures, row groups, Gram tables */
