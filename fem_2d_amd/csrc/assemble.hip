@@ -937,6 +937,9 @@ __device__ __forceinline__ int wave_incl_scan(int x, int lane) {
 struct WaveShared {
   int src_x[kWS];
   int src_info[kWS];  // k | rel << 1 | ru << 3 | rv << 17
+  int src_bs[kWS];    // first basis spec of each source (bs_offsets)
+  int src_m0[kWS];    // first membership of each source
+  int raw_base[kWS];  // first raw candidate of each source
   union {
     struct {
       unsigned long long key[kWR];  // row group << 20 | raw index
@@ -968,19 +971,30 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
   int cnt = 0, X = -1;
   if (lane < n_src) {
     X = S.src_x[lane];
-    cnt = w.mem_idx[v.bs_off[X + 1]] - w.mem_idx[v.bs_off[X]];
+    const long long b0 = v.bs_off[X];
+    const int m0 = w.mem_idx[b0];
+    cnt = w.mem_idx[v.bs_off[X + 1]] - m0;
+    S.src_bs[lane] = int(b0);
+    S.src_m0[lane] = m0;
   }
   const int incl = wave_incl_scan(cnt, lane);
   const int n_raw = __shfl(incl, kWave - 1, kWave);
   if (n_raw > kWR) return 1;
-  if (lane < n_src) {
-    const int base = incl - cnt;
-    const int m0 = w.mem_idx[v.bs_off[X]];
-    for (int m = 0; m < cnt; ++m) {
-      const int r = base + m;
-      S.u.sort.key[r] = ((unsigned long long)(unsigned)w.mem_group[m0 + m] << 20) | (unsigned long long)r;
-      S.u.sort.raw_meta[r] = lane | (w.mem_bsoff[m0 + m] << 8);
+  if (lane < n_src) S.raw_base[lane] = incl - cnt;
+  wave_sync();
+  // raw candidates (one membership of one source each) gathered across lanes;
+  // the source of item r is the last one whose first item is at most r
+  for (int r = lane; r < n_raw; r += kWave) {
+    int a = 0, b = n_src;  // raw_base[a] <= r < raw_base[b] (raw_base[n_src] = n_raw)
+    while (b - a > 1) {
+      const int mid = (a + b) >> 1;
+      if (S.raw_base[mid] <= r) a = mid; else b = mid;
     }
+    const int m = S.src_m0[a] + (r - S.raw_base[a]);
+    S.u.sort.key[r] = ((unsigned long long)(unsigned)w.mem_group[m] << 20) | (unsigned long long)r;
+    S.u.sort.raw_meta[r] = a | (w.mem_bsoff[m] << 8);
+  }
+  if (lane < n_src) {
     if (coef) {
       const int info = S.src_info[lane];
       const int k = info & 1, rel = (info >> 1) & 3;
@@ -1147,8 +1161,8 @@ __global__ void __launch_bounds__(kBlock) k_describe_wave(DevView v, Work w) {
         const int m0 = S.dmeta0[a], m1 = S.dmeta1[a];
         uint4 rec;
         rec.x = unsigned(S.dstart[a] + within);
-        rec.y = w.bs_pack[v.bs_off[S.src_x[m0 & 0xff]] + (m0 >> 8) + within];
-        rec.z = m1 < 0 ? ~0u : w.bs_pack[v.bs_off[S.src_x[m1 & 0xff]] + (m1 >> 8) + within];
+        rec.y = w.bs_pack[S.src_bs[m0 & 0xff] + (m0 >> 8) + within];
+        rec.z = m1 < 0 ? ~0u : w.bs_pack[S.src_bs[m1 & 0xff] + (m1 >> 8) + within];
         rec.w = unsigned(m0 & 0xff) | (m1 < 0 ? 0u : unsigned(m1 & 0xff) << 8);
         w.meta_col[(long long)g * kWC + t] = rec;
       }
