@@ -39,6 +39,7 @@ class DomainView(C.Structure):
         ("bs_i", C.POINTER(C.c_uint8)),
         ("bs_j", C.POINTER(C.c_uint8)),
         ("bs_dof", C.POINTER(C.c_int32)),
+        ("dof_bs", C.POINTER(C.c_int32)),
     ]
 
 

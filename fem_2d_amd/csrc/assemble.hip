@@ -119,6 +119,7 @@ struct DevView {
   const unsigned char* bs_i;
   const unsigned char* bs_j;
   const int* bs_dof;
+  const int* dof_bs;  // [2 n_dofs] each DoF's first and second basis spec (flat index), -1 for none
   int p1;  // orders per axis in the Gram library (max expansion order + 1)
 };
 
@@ -130,19 +131,18 @@ struct Work {
   int* nested_ptr;   // [E+1] exclusive scan of the lengths 1 + depth + n_desc
   int* nested;       // per Elem: its descendants (any order), itself, its ancestors (parent first); X << 2 | rel
   int4* dyad;  // (du, iu, dv, iv): position of the Elem inside its base Elem
-  long long* slot0;  // per DoF: (elem << 32 | index) of its basis specs
-  long long* slot1;
   int* bs_elem;       // per basis spec: owning Elem
   unsigned* bs_pack;  // per basis spec, packed by pack_spec
   int* dof_e1;
   int* dof_e2;
-  int* dof_k1;
-  int* dof_k2;
+  int* dof_t1;  // the DoF's basis spec on its first / second Elem (flat index; dof_t2 -1 for none)
+  int* dof_t2;
   int* dof_flag;
   int* dof_grp;  // inclusive scan of dof_flag
   int* g_start;  // [G+1]
   int* g_e1;
   int* g_e2;
+  int2* spec_run;  // [n_bs] per basis spec: row group if a membership starts there (else -1), index in the Elem's list
   int* mem_idx;   // [n_bs+1] memberships before each basis spec; Elem X's are [mem_idx[bs_off[X]], mem_idx[bs_off[X+1]])
   long long* tile_sum;  // per-tile sums of the prefix sum being computed
   int* mem_group;
@@ -408,10 +408,6 @@ __device__ __forceinline__ void init_body(const Work& w, int n_elems, long long 
     w.n_desc[t] = 0;
     w.desc_cursor[t] = 0;
   }
-  if (t < n_dofs) {
-    w.slot0[t] = -1;
-    w.slot1[t] = -1;
-  }
   if (t == 0) {
     *w.err = 0;
     *w.n_large = 0;
@@ -503,9 +499,9 @@ __device__ __forceinline__ void nested_fill_body(const DevView& v, const Work& w
   }
 }
 
-// Per basis spec: owning Elem, packed spec, and the DoF's spec slots (one
-// wavefront per Elem, lanes over its spec list).
-__device__ __forceinline__ void bs_slots_body(const DevView& v, const Work& w, long long wave, long long waves) {
+// Per basis spec: owning Elem and packed spec (one wavefront per Elem, lanes
+// over its spec list); the spec must be one of its DoF's listed specs.
+__device__ __forceinline__ void spec_body(const DevView& v, const Work& w, long long wave, long long waves) {
   const int lane = threadIdx.x % kWave;
   for (long long e = wave; e < v.n_elems; e += waves) {
     const long long t0 = v.bs_off[e], t1 = v.bs_off[e + 1];
@@ -514,29 +510,44 @@ __device__ __forceinline__ void bs_slots_body(const DevView& v, const Work& w, l
       w.bs_pack[t] = pack_spec(v.bs_dir[t], v.bs_i[t], v.bs_j[t], v.p1);
       const int d = v.bs_dof[t];
       // H(curl) specs are U or V directed with orders inside the Gram library
-      if (d < 0 || d >= v.n_dofs || v.bs_dir[t] > FEM2D_DIR_V || v.bs_i[t] >= v.p1 || v.bs_j[t] >= v.p1) {
+      if (d < 0 || d >= v.n_dofs || v.bs_dir[t] > FEM2D_DIR_V || v.bs_i[t] >= v.p1 || v.bs_j[t] >= v.p1 ||
+          (v.dof_bs[2LL * d] != t && v.dof_bs[2LL * d + 1] != t))
         set_err(w.err, kErrView);
-        continue;
-      }
-      const long long val = (e << 32) | (t - t0);
-      if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot0[d]), ~0ull, (unsigned long long)val) != ~0ull)
-        if (atomicCAS(reinterpret_cast<unsigned long long*>(&w.slot1[d]), ~0ull, (unsigned long long)val) != ~0ull)
-          set_err(w.err, kErrContrib);
     }
   }
 }
 
-__device__ __forceinline__ void dof_sig(const Work& w, long long d, int& e1, int& k1, int& e2, int& k2) {
-  long long s0 = w.slot0[d], s1 = w.slot1[d];
-  if (s1 >= 0 && (s1 >> 32) < (s0 >> 32)) {
-    const long long t = s0;
-    s0 = s1;
-    s1 = t;
+// Per DoF: its first listed spec exists, the second is absent or another
+// spec, and both name this DoF (with the per-spec check above, the DoFs'
+// spec lists and the specs' DoF ids describe the same relation).
+__device__ __forceinline__ void dof_check_body(const DevView& v, const Work& w, long long d) {
+  if (d >= v.n_dofs) return;
+  const int ta = v.dof_bs[2 * d], tb = v.dof_bs[2 * d + 1];
+  const bool ok_a = ta >= 0 && ta < v.n_bs && v.bs_dof[ta] == d;
+  const bool ok_b = tb == -1 || (tb >= 0 && tb < v.n_bs && tb != ta && v.bs_dof[tb] == d);
+  if (!ok_a || !ok_b) set_err(w.err, kErrView);
+}
+
+// A DoF's signature: the Elems of its specs in increasing order and the specs
+// (flat indices); -1 where absent. Out-of-range spec indices (an invalid view,
+// flagged by dof_check_body) read as absent.
+__device__ __forceinline__ void dof_sig(const DevView& v, const Work& w, long long d, int& e1, int& t1, int& e2,
+                                        int& t2) {
+  int ta = v.dof_bs[2 * d], tb = v.dof_bs[2 * d + 1];
+  if (ta < 0 || ta >= v.n_bs) ta = -1;
+  if (tb < 0 || tb >= v.n_bs) tb = -1;
+  int ea = ta >= 0 ? w.bs_elem[ta] : -1, eb = tb >= 0 ? w.bs_elem[tb] : -1;
+  if (ea < 0 || (eb >= 0 && eb < ea)) {
+    const int te = ta, ee = ea;
+    ta = tb;
+    ea = eb;
+    tb = te;
+    eb = ee;
   }
-  e1 = s0 >= 0 ? int(s0 >> 32) : -1;
-  k1 = s0 >= 0 ? int(s0 & 0xffffffff) : -1;
-  e2 = s1 >= 0 ? int(s1 >> 32) : -1;
-  k2 = s1 >= 0 ? int(s1 & 0xffffffff) : -1;
+  e1 = ea;
+  t1 = ta;
+  e2 = eb;
+  t2 = tb;
 }
 
 // ---------------------------------------------------------------------------
@@ -642,23 +653,37 @@ __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const
 
 // DoF signatures -> row groups. Item d is 1 where a row group starts (its
 // pair of Elems differs from DoF d - 1's); the inclusive sum is the group id.
+// The store also marks, for each spec of DoF d, whether a membership starts
+// there: a membership is a run of an Elem's spec list that is one whole row
+// group (it starts at the group's first DoF and ends at its last).
 struct DofGroupScan {
+  DevView v;
   Work w;
   __device__ long long load(long long d) const {
     if (d == 0) return 1;
-    int e1, k1, e2, k2, p1, pk1, p2, pk2;
-    dof_sig(w, d, e1, k1, e2, k2);
-    dof_sig(w, d - 1, p1, pk1, p2, pk2);
+    int e1, t1, e2, t2, p1, pt1, p2, pt2;
+    dof_sig(v, w, d, e1, t1, e2, t2);
+    dof_sig(v, w, d - 1, p1, pt1, p2, pt2);
     return (p1 != e1 || p2 != e2) ? 1 : 0;
   }
+  // spec t (on Elem e) of DoF d in group g; first / last: d is the group's
+  // first / last DoF. The run continues from spec t - 1 when that spec is
+  // DoF d - 1's and d is not a group's first, and on to t + 1 likewise.
+  __device__ void mark_spec(long long d, int t, int e, int g, bool first, bool last) const {
+    const long long b0 = v.bs_off[e], b1 = v.bs_off[e + 1];
+    const bool start = t == b0 || v.bs_dof[t - 1] != d - 1 || first;
+    const bool end = t + 1 == b1 || v.bs_dof[t + 1] != d + 1 || last;
+    if ((start && !first) || (end && !last)) set_err(w.err, kErrView);  // a run that is not a whole group
+    w.spec_run[t] = make_int2(start ? g : -1, int(t - b0));
+  }
   __device__ void store(long long d, long long excl, long long flag) const {
-    int e1, k1, e2, k2;
-    dof_sig(w, d, e1, k1, e2, k2);
+    int e1, t1, e2, t2;
+    dof_sig(v, w, d, e1, t1, e2, t2);
     if (e1 < 0 || (e2 >= 0 && e2 == e1)) set_err(w.err, kErrView);  // no spec, or two on one Elem
     w.dof_e1[d] = e1;
-    w.dof_k1[d] = k1;
+    w.dof_t1[d] = t1;
     w.dof_e2[d] = e2;
-    w.dof_k2[d] = k2;
+    w.dof_t2[d] = t2;
     const int g1 = int(excl + flag);  // group id + 1
     w.dof_grp[d] = g1;
     w.dof_flag[d] = int(flag);
@@ -668,47 +693,30 @@ struct DofGroupScan {
       w.g_e2[g1 - 1] = e2;
     }
     if (d == w.n_dofs - 1) w.g_start[g1] = int(w.n_dofs);
+    bool last = d + 1 == w.n_dofs;  // DoF d + 1 starts another group
+    if (!last) {
+      int n1, nt1, n2, nt2;
+      dof_sig(v, w, d + 1, n1, nt1, n2, nt2);
+      last = n1 != e1 || n2 != e2;
+    }
+    if (e1 >= 0) mark_spec(d, t1, e1, g1 - 1, flag != 0, last);
+    if (e2 >= 0) mark_spec(d, t2, e2, g1 - 1, flag != 0, last);
   }
 };
 
-// Each Elem's basis-spec list as runs of whole row groups ("memberships").
-// Item t is 1 at the first spec of a run; each run must be one complete group
-// (it starts at the group's first DoF and ends at its last). The exclusive sum
-// numbers the memberships; item n_bs (0) gives their total.
+// Each Elem's basis-spec list as memberships (runs marked by DofGroupScan).
+// Item t is 1 at the first spec of a membership; the exclusive sum numbers
+// the memberships; item n_bs (0) gives their total.
 struct MembershipScan {
   DevView v;
   Work w;
-  __device__ bool run_start(long long t, int& d, int& g, int& e) const {
-    e = w.bs_elem[t];
-    d = v.bs_dof[t];
-    if (d < 0 || d >= v.n_dofs) return false;  // flagged by k_bs_slots
-    g = w.dof_grp[d] - 1;
-    if (t == v.bs_off[e]) return true;
-    const int dp = v.bs_dof[t - 1];
-    return !(dp >= 0 && dp < v.n_dofs && dp + 1 == d && w.dof_grp[dp] - 1 == g);
-  }
-  __device__ long long load(long long t) const {
-    if (t >= v.n_bs) return 0;
-    int d, g, e;
-    return run_start(t, d, g, e) ? 1 : 0;
-  }
+  __device__ long long load(long long t) const { return t < v.n_bs && w.spec_run[t].x >= 0 ? 1 : 0; }
   __device__ void store(long long t, long long excl, long long start) const {
     w.mem_idx[t] = int(excl);
-    if (t >= v.n_bs) return;
-    int d, g, e;
-    run_start(t, d, g, e);
-    if (d < 0 || d >= v.n_dofs) return;
-    bool end = t + 1 == v.bs_off[e + 1];
-    if (!end) {
-      const int dn = v.bs_dof[t + 1];
-      end = !(dn >= 0 && dn < v.n_dofs && dn == d + 1 && w.dof_grp[dn] - 1 == g);
-    }
-    const bool group_first = w.dof_flag[d] != 0;
-    const bool group_last = d + 1 == v.n_dofs || w.dof_flag[d + 1] != 0;
-    if ((start && !group_first) || (end && !group_last)) set_err(w.err, kErrView);
     if (start) {
-      w.mem_group[excl] = g;
-      w.mem_bsoff[excl] = int(t - v.bs_off[e]);
+      const int2 r = w.spec_run[t];
+      w.mem_group[excl] = r.x;
+      w.mem_bsoff[excl] = r.y;
     }
   }
 };
@@ -829,17 +837,21 @@ __global__ void __launch_bounds__(kBlock) k_init(Work w, int n_elems, long long 
 
 // The Gram library (needed from the describe kernel on; its blocks come first
 // so that they are dispatched first, being the longest); Elem geometry and
-// depth; per-spec owners and the DoFs' spec slots
-__global__ void __launch_bounds__(kBlock) k_gram_elem_slots(DevView v, Work w, unsigned gram_blocks,
-                                                            unsigned prep_blocks, LibShape L, GlqArgs glq) {
+// depth; per-spec owners and packed specs; the DoFs' spec lists checked
+__global__ void __launch_bounds__(kBlock) k_gram_elem_specs(DevView v, Work w, unsigned gram_blocks,
+                                                            unsigned prep_blocks, unsigned spec_blocks, LibShape L,
+                                                            GlqArgs glq) {
   if (blockIdx.x < gram_blocks) {
     gram_body(w, L, glq, blockIdx.x);
   } else if (blockIdx.x < gram_blocks + prep_blocks) {
     elem_prep_body(v, w, int((blockIdx.x - gram_blocks) * kBlock + threadIdx.x));
-  } else {
+  } else if (blockIdx.x < gram_blocks + prep_blocks + spec_blocks) {
     const unsigned first = gram_blocks + prep_blocks;
-    const long long waves = (long long)(gridDim.x - first) * kWavesPerBlock;
-    bs_slots_body(v, w, (long long)(blockIdx.x - first) * kWavesPerBlock + threadIdx.x / kWave, waves);
+    spec_body(v, w, (long long)(blockIdx.x - first) * kWavesPerBlock + threadIdx.x / kWave,
+              (long long)spec_blocks * kWavesPerBlock);
+  } else {
+    const unsigned first = gram_blocks + prep_blocks + spec_blocks;
+    dof_check_body(v, w, (long long)(blockIdx.x - first) * kBlock + threadIdx.x);
   }
 }
 
@@ -1142,13 +1154,10 @@ __global__ void __launch_bounds__(kBlock) k_describe_wave(DevView v, Work w) {
         w.is_large[g] = 0;
         w.meta_hdr[g] = make_int4(T, nrows, n_src, 0);
       }
-      const int e1 = w.g_e1[g], e2 = w.g_e2[g];
-      const long long b1 = v.bs_off[e1];
-      const long long b2 = e2 >= 0 ? v.bs_off[e2] : 0;
       for (int q = lane; q < nrows; q += kWave) {
         const int r = r0 + q;
-        w.meta_row[(long long)g * kWRows + q] =
-            make_uint2(w.bs_pack[b1 + w.dof_k1[r]], e2 >= 0 ? w.bs_pack[b2 + w.dof_k2[r]] : 0u);
+        const int t2 = w.dof_t2[r];
+        w.meta_row[(long long)g * kWRows + q] = make_uint2(w.bs_pack[w.dof_t1[r]], t2 >= 0 ? w.bs_pack[t2] : 0u);
       }
       for (int t = lane; t < T; t += kWave) {
         const int u = u_self + t;
@@ -1548,8 +1557,8 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
       S.row_rel[q] = int(row_ptr[r0 + q] - seg0);
       if (q < nrows) {
         const int r = r0 + q;
-        S.row_bs[0][q] = w.bs_pack[v.bs_off[e_k[0]] + w.dof_k1[r]];
-        S.row_bs[1][q] = e_k[1] >= 0 ? w.bs_pack[v.bs_off[e_k[1]] + w.dof_k2[r]] : 0u;
+        S.row_bs[0][q] = w.bs_pack[w.dof_t1[r]];
+        S.row_bs[1][q] = e_k[1] >= 0 ? w.bs_pack[w.dof_t2[r]] : 0u;
       }
     }
     __syncthreads();
@@ -1643,6 +1652,7 @@ struct fem2d_device_domain {
   unsigned char* bs_i = nullptr;
   unsigned char* bs_j = nullptr;
   int* bs_dof = nullptr;
+  int* dof_bs = nullptr;
 };
 
 struct fem2d_ctx {
@@ -1888,7 +1898,7 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
   HIP_TRY(hipSetDevice(ctx->device));
   const int ne = v->n_elems;
   const long long nbs = ne > 0 ? v->bs_offsets[ne] : 0;
-  if (nbs > 0 && (!v->bs_dir || !v->bs_i || !v->bs_j || !v->bs_dof)) {
+  if ((nbs > 0 && (!v->bs_dir || !v->bs_i || !v->bs_j || !v->bs_dof)) || (v->n_dofs > 0 && !v->dof_bs)) {
     set_error("domain view has null basis-spec arrays", 0);
     return FEM2D_ERR_INVALID_VIEW;
   }
@@ -1950,7 +1960,8 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
       (rc = upload(v->materials, size_t(2 * ne), &dd->mat)) ||
       (rc = upload(off.data(), off.size(), &dd->bs_off)) ||
       (rc = upload(v->bs_dir, size_t(nbs), &dd->bs_dir)) || (rc = upload(v->bs_i, size_t(nbs), &dd->bs_i)) ||
-      (rc = upload(v->bs_j, size_t(nbs), &dd->bs_j)) || (rc = upload(v->bs_dof, size_t(nbs), &dd->bs_dof))) {
+      (rc = upload(v->bs_j, size_t(nbs), &dd->bs_j)) || (rc = upload(v->bs_dof, size_t(nbs), &dd->bs_dof)) ||
+      (rc = upload(v->dof_bs, size_t(2 * v->n_dofs), &dd->dof_bs))) {
     fem2d_device_domain_free(dd);
     return rc;
   }
@@ -1962,7 +1973,7 @@ void fem2d_device_domain_free(fem2d_device_domain* dd) {
   if (!dd) return;
   (void)hipSetDevice(dd->device);
   for (void* p : {(void*)dd->parent, (void*)dd->loc, (void*)dd->box, (void*)dd->mat, (void*)dd->bs_off,
-                  (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof})
+                  (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof, (void*)dd->dof_bs})
     if (p) (void)hipFree(p);
   delete dd;
 }
@@ -2004,11 +2015,11 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       (rc = slot(ctx, k++, E1, &w.n_desc)) || (rc = slot(ctx, k++, E1, &w.desc_cursor)) ||
       (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_ptr)) ||
       (rc = slot(ctx, k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) ||
-      (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, N1, &w.slot0)) ||
-      (rc = slot(ctx, k++, N1, &w.slot1)) || (rc = slot(ctx, k++, B1, &w.bs_elem)) ||
+      (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, B1, &w.spec_run)) ||
+      (rc = slot(ctx, k++, B1, &w.bs_elem)) ||
       (rc = slot(ctx, k++, B1, &w.bs_pack)) || (rc = slot(ctx, k++, N1, &w.dof_e1)) ||
-      (rc = slot(ctx, k++, N1, &w.dof_e2)) || (rc = slot(ctx, k++, N1, &w.dof_k1)) ||
-      (rc = slot(ctx, k++, N1, &w.dof_k2)) || (rc = slot(ctx, k++, N1, &w.dof_flag)) ||
+      (rc = slot(ctx, k++, N1, &w.dof_e2)) || (rc = slot(ctx, k++, N1, &w.dof_t1)) ||
+      (rc = slot(ctx, k++, N1, &w.dof_t2)) || (rc = slot(ctx, k++, N1, &w.dof_flag)) ||
       (rc = slot(ctx, k++, N1, &w.dof_grp)) || (rc = slot(ctx, k++, N1, &w.g_start)) ||
       (rc = slot(ctx, k++, N1, &w.g_e1)) || (rc = slot(ctx, k++, N1, &w.g_e2)) ||
       (rc = slot(ctx, k++, B1, &w.mem_idx)) || (rc = slot(ctx, k++, B1, &w.mem_group)) ||
@@ -2072,6 +2083,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   v.bs_i = dd->bs_i;
   v.bs_j = dd->bs_j;
   v.bs_dof = dd->bs_dof;
+  v.dof_bs = dd->dof_bs;
   v.p1 = L.p1;
 
   const int B = 256;
@@ -2087,15 +2099,15 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   // events between the stages also time the stages and the describe and fill
   // kernels on their own.
   auto part_a = [&](const FillOut& o, bool fine_events) -> int {
-    k_init<<<grid_for(std::max<long long>(ne, nd), B), B, 0, st>>>(w, ne, nd);
+    k_init<<<grid_for(ne, B), B, 0, st>>>(w, ne, nd);
     const unsigned gram_blocks = grid_for(lib_n, B), prep_blocks = grid_for(ne, B);
-    k_gram_elem_slots<<<gram_blocks + prep_blocks + slot_blocks, B, 0, st>>>(v, w, gram_blocks, prep_blocks, L,
-                                                                              ctx->glq);
+    k_gram_elem_specs<<<gram_blocks + prep_blocks + slot_blocks + grid_for(nd, B), B, 0, st>>>(
+        v, w, gram_blocks, prep_blocks, slot_blocks, L, ctx->glq);
     HIP_TRY_AT(hipGetLastError(), "after the Elem preparation kernels");
     // nested-list offsets and row groups side by side
     const unsigned gn = scan_grid(tiles_nested), gd = scan_grid(tiles_dof), gm = scan_grid(tiles_mem);
-    k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{w}, ts_dof);
-    k_scan_apply2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{w}, ts_dof);
+    k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{v, w}, ts_dof);
+    k_scan_apply2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{v, w}, ts_dof);
     // nested lists beside the first pass of the membership numbering
     const unsigned fill_blocks = grid_for(ne, B);
     k_nested_members<<<fill_blocks + gm, kBlock, 0, st>>>(v, w, fill_blocks, nbs + 1, MembershipScan{v, w}, ts_mem);

@@ -28,6 +28,7 @@ struct fem2d_domain {
   std::vector<int64_t> bs_offsets;
   std::vector<uint8_t> bs_dir, bs_i, bs_j;
   std::vector<int32_t> bs_dof;
+  std::vector<int32_t> dof_bs;
 };
 
 namespace fem2d {
@@ -414,6 +415,15 @@ static void flatten(fem2d_domain* dom) {
       dom->bs_dof[t] = bs.dof_id;
       ++t;
     }
+  // each DoF's basis-spec addresses as flat spec indices
+  dom->dof_bs.assign(2 * d.dofs.size(), -1);
+  for (size_t f = 0; f < d.dofs.size(); ++f) {
+    const DoF& dof = d.dofs[f];
+    for (size_t k = 0; k < dof.basis_specs.size() && k < 2; ++k) {
+      const BSAddress& a = dof.basis_specs[k];
+      dom->dof_bs[2 * f + k] = int32_t(dom->bs_offsets[size_t(a.elem_id)] + a.elem_idx);
+    }
+  }
 }
 
 int fem2d_domain_from_mesh(const fem2d_mesh* m, int32_t cc, fem2d_domain** out) {
@@ -468,6 +478,7 @@ int fem2d_domain_get_view(const fem2d_domain* d, fem2d_domain_view* out) {
   out->bs_i = d->bs_i.data();
   out->bs_j = d->bs_j.data();
   out->bs_dof = d->bs_dof.data();
+  out->dof_bs = d->dof_bs.data();
   return FEM2D_OK;
 }
 

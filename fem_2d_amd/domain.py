@@ -116,7 +116,8 @@ class Domain:
                 materials=arr(v.materials, 2 * ne, np.float64).reshape(-1, 2),
                 orders=arr(v.orders, 2 * ne, np.uint8).reshape(-1, 2),
                 bs_offsets=off, bs_dir=arr(v.bs_dir, nbs, np.uint8), bs_i=arr(v.bs_i, nbs, np.uint8),
-                bs_j=arr(v.bs_j, nbs, np.uint8), bs_dof=arr(v.bs_dof, nbs, np.int32))
+                bs_j=arr(v.bs_j, nbs, np.uint8), bs_dof=arr(v.bs_dof, nbs, np.int32),
+                dof_bs=arr(v.dof_bs, 2 * int(v.n_dofs), np.int32))
         return self._arrays
 
     def local_basis_specs(self, elem_id: int) -> List[BasisSpec]:

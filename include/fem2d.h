@@ -101,6 +101,10 @@ typedef struct fem2d_domain_view {
   const uint8_t* bs_i;        /* [n_bs] BasisSpec::i */
   const uint8_t* bs_j;        /* [n_bs] BasisSpec::j */
   const int32_t* bs_dof;      /* [n_bs] BasisSpec::dof_id */
+  const int32_t* dof_bs;      /* [2*n_dofs] DoF::basis_specs (dof.rs:11-28): the flat index
+                                 bs_offsets[elem_id] + elem_idx of each DoF's first and second
+                                 BSAddress, -1 for a DoF with one (H1 node DoFs, with four,
+                                 list their first two; the H(curl) assembly rejects them) */
 } fem2d_domain_view;
 
 /* GEP { a, b } (linalg.rs:29-34) as upper-triangular CSR: one pattern shared by

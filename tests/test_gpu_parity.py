@@ -112,7 +112,8 @@ def _view_from_arrays(a):
     keep = {}
     for name, ctype in [("parent", C.c_int32), ("loc", C.c_uint8), ("element_box", C.c_double),
                         ("materials", C.c_double), ("orders", C.c_uint8), ("bs_offsets", C.c_int64),
-                        ("bs_dir", C.c_uint8), ("bs_i", C.c_uint8), ("bs_j", C.c_uint8), ("bs_dof", C.c_int32)]:
+                        ("bs_dir", C.c_uint8), ("bs_i", C.c_uint8), ("bs_j", C.c_uint8), ("bs_dof", C.c_int32),
+                        ("dof_bs", C.c_int32)]:
         keep[name] = np.ascontiguousarray(a[name].ravel())
         setattr(view, name, keep[name].ctypes.data_as(C.POINTER(ctype)))
     view.cc = 0
