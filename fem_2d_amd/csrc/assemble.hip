@@ -143,6 +143,7 @@ struct Work {
   int* g_e1;
   int* g_e2;
   int2* spec_run;  // [n_bs] per basis spec: row group if a membership starts there (else -1), index in the Elem's list
+  unsigned* spec_pos;  // [n_bs] index in the Elem's list | last in the list << 31
   int* mem_idx;   // [n_bs+1] memberships before each basis spec; Elem X's are [mem_idx[bs_off[X]], mem_idx[bs_off[X+1]])
   long long* tile_sum;  // per-tile sums of the prefix sum being computed
   int* mem_group;
@@ -508,6 +509,7 @@ __device__ __forceinline__ void spec_body(const DevView& v, const Work& w, long 
     for (long long t = t0 + lane; t < t1; t += kWave) {
       w.bs_elem[t] = int(e);
       w.bs_pack[t] = pack_spec(v.bs_dir[t], v.bs_i[t], v.bs_j[t], v.p1);
+      w.spec_pos[t] = unsigned(t - t0) | (t + 1 == t1 ? 0x80000000u : 0u);
       const int d = v.bs_dof[t];
       // H(curl) specs are U or V directed with orders inside the Gram library
       if (d < 0 || d >= v.n_dofs || v.bs_dir[t] > FEM2D_DIR_V || v.bs_i[t] >= v.p1 || v.bs_j[t] >= v.p1 ||
@@ -526,6 +528,26 @@ __device__ __forceinline__ void dof_check_body(const DevView& v, const Work& w, 
   const bool ok_a = ta >= 0 && ta < v.n_bs && v.bs_dof[ta] == d;
   const bool ok_b = tb == -1 || (tb >= 0 && tb < v.n_bs && tb != ta && v.bs_dof[tb] == d);
   if (!ok_a || !ok_b) set_err(w.err, kErrView);
+}
+
+// One of a DoF's basis specs (flat index t, -1 for none): its Elem, its place
+// in the Elem's list (spec_pos) and the DoFs of its neighbours in that list.
+struct SpecInfo {
+  int t, e;
+  unsigned pos;
+  int prev_d, next_d;
+};
+
+__device__ __forceinline__ SpecInfo spec_info(const DevView& v, const Work& w, int t) {
+  const bool ok = t >= 0 && t < v.n_bs;
+  const int tt = ok ? t : 0;  // every load in bounds
+  SpecInfo s;
+  s.t = ok ? t : -1;
+  s.e = ok ? w.bs_elem[tt] : -1;
+  s.pos = w.spec_pos[tt];
+  s.prev_d = tt > 0 ? v.bs_dof[tt - 1] : -1;
+  s.next_d = tt + 1 < v.n_bs ? v.bs_dof[tt + 1] : -1;
+  return s;
 }
 
 // A DoF's signature: the Elems of its specs in increasing order and the specs
@@ -552,8 +574,9 @@ __device__ __forceinline__ void dof_sig(const DevView& v, const Work& w, long lo
 
 // ---------------------------------------------------------------------------
 // Prefix sums with fused producers and consumers (reduce, then scan):
-// Op::load(i) yields item i (no side effects: it runs in both kernels) and
-// Op::store(i, exclusive, item) consumes the result. k_scan_reduce writes the
+// Op::load(i) yields item i (no side effects: it runs in both kernels), a
+// count or a struct whose field n is the count and whose other fields carry
+// what the store needs, and Op::store(i, exclusive, item) consumes the result. k_scan_reduce writes the
 // sum of every tile of kScanTile items; k_scan_apply derives each tile's
 // prefix from those sums and scans the tile in kScanRounds striped rounds of
 // kBlock consecutive items (coalesced loads and stores). No block waits on
@@ -563,6 +586,12 @@ constexpr int kScanRounds = 4;
 constexpr int kScanTile = kBlock * kScanRounds;
 
 __host__ __device__ constexpr long long scan_tiles(long long n) { return (n + kScanTile - 1) / kScanTile; }
+
+__device__ __forceinline__ long long item_count(long long x) { return x; }
+template <class I>
+__device__ __forceinline__ long long item_count(const I& x) {
+  return x.n;
+}
 
 // block-wide sum; every thread gets the total (kBlock threads, shared words sw[kWavesPerBlock])
 __device__ __forceinline__ long long block_sum(long long x, long long* sw) {
@@ -589,7 +618,7 @@ __device__ __forceinline__ void scan_reduce_body(long long n, const Op& op, long
 #pragma unroll
     for (int k = 0; k < kScanRounds; ++k) {
       const long long i = tile * kScanTile + (long long)k * kBlock + threadIdx.x;
-      if (i < n) sum += op.load(i);
+      if (i < n) sum += item_count(op.load(i));
     }
     sum = block_sum(sum, sw);
     if (threadIdx.x == 0) tile_sum[tile] = sum;
@@ -608,12 +637,14 @@ __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const
   const long long n_tiles = scan_tiles(n);
   for (long long tile = bid; tile < n_tiles; tile += nblk) {
     const long long base = tile * kScanTile + threadIdx.x;
-    long long item[kScanRounds], incl[kScanRounds];
+    using Item = decltype(op.load(0));
+    Item item[kScanRounds];
+    long long incl[kScanRounds];
 #pragma unroll
     for (int k = 0; k < kScanRounds; ++k) {
       const long long i = base + (long long)k * kBlock;
-      item[k] = i < n ? op.load(i) : 0;
-      incl[k] = item[k];
+      item[k] = i < n ? op.load(i) : Item{};
+      incl[k] = item_count(item[k]);
     }
     long long pre = 0;  // sum of the tiles before this one
     for (long long t = threadIdx.x; t < tile; t += kBlock) pre += tile_sum[t];
@@ -645,7 +676,7 @@ __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const
         carry += sw[k][v];
       }
       const long long i = base + (long long)k * kBlock;
-      if (i < n) op.store(i, off + incl[k] - item[k], item[k]);
+      if (i < n) op.store(i, off + incl[k] - item_count(item[k]), item[k]);
     }
     __syncthreads();  // sw and sp are rewritten by the next tile
   }
@@ -659,48 +690,61 @@ __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const
 struct DofGroupScan {
   DevView v;
   Work w;
-  __device__ long long load(long long d) const {
-    if (d == 0) return 1;
-    int e1, t1, e2, t2, p1, pt1, p2, pt2;
-    dof_sig(v, w, d, e1, t1, e2, t2);
-    dof_sig(v, w, d - 1, p1, pt1, p2, pt2);
-    return (p1 != e1 || p2 != e2) ? 1 : 0;
+  struct Item {
+    long long n;     // 1 where a row group starts
+    SpecInfo a, b;   // the DoF's specs, on its first and second Elem
+    bool last;       // the next DoF starts another group (or there is none)
+  };
+  __device__ Item load(long long d) const {
+    Item it;
+    it.a = spec_info(v, w, v.dof_bs[2 * d]);
+    it.b = spec_info(v, w, v.dof_bs[2 * d + 1]);
+    if (it.a.e < 0 || (it.b.e >= 0 && it.b.e < it.a.e)) {
+      const SpecInfo x = it.a;
+      it.a = it.b;
+      it.b = x;
+    }
+    int p1, pt1, p2, pt2;
+    it.n = 1;
+    if (d > 0) {
+      dof_sig(v, w, d - 1, p1, pt1, p2, pt2);
+      it.n = (p1 != it.a.e || p2 != it.b.e) ? 1 : 0;
+    }
+    it.last = true;
+    if (d + 1 < v.n_dofs) {
+      dof_sig(v, w, d + 1, p1, pt1, p2, pt2);
+      it.last = p1 != it.a.e || p2 != it.b.e;
+    }
+    return it;
   }
-  // spec t (on Elem e) of DoF d in group g; first / last: d is the group's
-  // first / last DoF. The run continues from spec t - 1 when that spec is
-  // DoF d - 1's and d is not a group's first, and on to t + 1 likewise.
-  __device__ void mark_spec(long long d, int t, int e, int g, bool first, bool last) const {
-    const long long b0 = v.bs_off[e], b1 = v.bs_off[e + 1];
-    const bool start = t == b0 || v.bs_dof[t - 1] != d - 1 || first;
-    const bool end = t + 1 == b1 || v.bs_dof[t + 1] != d + 1 || last;
+  // spec s of DoF d in group g; first: d is the group's first DoF. The run
+  // continues from the previous spec of the Elem's list when that spec is DoF
+  // d - 1's and d is not a group's first, and on to the next one likewise.
+  __device__ void mark_spec(long long d, const SpecInfo& s, int g, bool first, bool last) const {
+    const int k = int(s.pos & 0x7fffffffu);
+    const bool start = k == 0 || s.prev_d != d - 1 || first;
+    const bool end = (s.pos >> 31) != 0u || s.next_d != d + 1 || last;
     if ((start && !first) || (end && !last)) set_err(w.err, kErrView);  // a run that is not a whole group
-    w.spec_run[t] = make_int2(start ? g : -1, int(t - b0));
+    w.spec_run[s.t] = make_int2(start ? g : -1, k);
   }
-  __device__ void store(long long d, long long excl, long long flag) const {
-    int e1, t1, e2, t2;
-    dof_sig(v, w, d, e1, t1, e2, t2);
+  __device__ void store(long long d, long long excl, const Item& it) const {
+    const int e1 = it.a.e, e2 = it.b.e;
     if (e1 < 0 || (e2 >= 0 && e2 == e1)) set_err(w.err, kErrView);  // no spec, or two on one Elem
     w.dof_e1[d] = e1;
-    w.dof_t1[d] = t1;
+    w.dof_t1[d] = it.a.t;
     w.dof_e2[d] = e2;
-    w.dof_t2[d] = t2;
-    const int g1 = int(excl + flag);  // group id + 1
+    w.dof_t2[d] = it.b.t;
+    const int g1 = int(excl + it.n);  // group id + 1
     w.dof_grp[d] = g1;
-    w.dof_flag[d] = int(flag);
-    if (flag) {
+    w.dof_flag[d] = int(it.n);
+    if (it.n) {
       w.g_start[g1 - 1] = int(d);
       w.g_e1[g1 - 1] = e1;
       w.g_e2[g1 - 1] = e2;
     }
     if (d == w.n_dofs - 1) w.g_start[g1] = int(w.n_dofs);
-    bool last = d + 1 == w.n_dofs;  // DoF d + 1 starts another group
-    if (!last) {
-      int n1, nt1, n2, nt2;
-      dof_sig(v, w, d + 1, n1, nt1, n2, nt2);
-      last = n1 != e1 || n2 != e2;
-    }
-    if (e1 >= 0) mark_spec(d, t1, e1, g1 - 1, flag != 0, last);
-    if (e2 >= 0) mark_spec(d, t2, e2, g1 - 1, flag != 0, last);
+    if (e1 >= 0) mark_spec(d, it.a, g1 - 1, it.n != 0, it.last);
+    if (e2 >= 0) mark_spec(d, it.b, g1 - 1, it.n != 0, it.last);
   }
 };
 
@@ -710,13 +754,21 @@ struct DofGroupScan {
 struct MembershipScan {
   DevView v;
   Work w;
-  __device__ long long load(long long t) const { return t < v.n_bs && w.spec_run[t].x >= 0 ? 1 : 0; }
-  __device__ void store(long long t, long long excl, long long start) const {
+  struct Item {
+    long long n;
+    int2 run;
+  };
+  __device__ Item load(long long t) const {
+    Item it;
+    it.run = t < v.n_bs ? w.spec_run[t] : make_int2(-1, 0);
+    it.n = it.run.x >= 0 ? 1 : 0;
+    return it;
+  }
+  __device__ void store(long long t, long long excl, const Item& it) const {
     w.mem_idx[t] = int(excl);
-    if (start) {
-      const int2 r = w.spec_run[t];
-      w.mem_group[excl] = r.x;
-      w.mem_bsoff[excl] = r.y;
+    if (it.n) {
+      w.mem_group[excl] = it.run.x;
+      w.mem_bsoff[excl] = it.run.y;
     }
   }
 };
@@ -2015,7 +2067,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       (rc = slot(ctx, k++, E1, &w.n_desc)) || (rc = slot(ctx, k++, E1, &w.desc_cursor)) ||
       (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_ptr)) ||
       (rc = slot(ctx, k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) ||
-      (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, B1, &w.spec_run)) ||
+      (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, B1, &w.spec_run)) || (rc = slot(ctx, k++, B1, &w.spec_pos)) ||
       (rc = slot(ctx, k++, B1, &w.bs_elem)) ||
       (rc = slot(ctx, k++, B1, &w.bs_pack)) || (rc = slot(ctx, k++, N1, &w.dof_e1)) ||
       (rc = slot(ctx, k++, N1, &w.dof_e2)) || (rc = slot(ctx, k++, N1, &w.dof_t1)) ||
