@@ -65,7 +65,7 @@ constexpr int kWD = 128;           // distinct row groups in one candidate set
 constexpr int kWC = 256;           // candidate columns from the group's first row on
 constexpr int kWRows = 128;        // rows per row group
 constexpr int kCoef = 8;           // doubles per source record: CurlCurl prefactors (4), L2Inner (2), padding
-constexpr int kChunk = 1024;       // CSR entries per work item of the streaming fill
+constexpr int kChunk = 256;        // CSR entries per work item of the streaming fill
 // block path (one block per row group)
 constexpr int kMaxSrc = 128;
 constexpr int kMaxRaw = 1024;
