@@ -283,9 +283,10 @@ __device__ void kol(int n, double x, double& N, double& Nd, double& T, double& T
   Td = sd * c;
 }
 
-__device__ __forceinline__ double shape_pick(int basis, int which, int n, double x) {
+template <int kBasis>
+__device__ __forceinline__ double shape_pick(int which, int n, double x) {
   double N, Nd, T, Td;
-  if (basis == FEM2D_BASIS_KOL) kol(n, x, N, Nd, T, Td);
+  if (kBasis == FEM2D_BASIS_KOL) kol(n, x, N, Nd, T, Td);
   else hier_poly(n, x, N, Nd, T, Td);
   switch (which) {
     case 0: return N;
@@ -576,16 +577,27 @@ __device__ __forceinline__ void dof_sig(const DevView& v, const Work& w, long lo
 // Prefix sums with fused producers and consumers (reduce, then scan):
 // Op::load(i) yields item i (no side effects: it runs in both kernels), a
 // count or a struct whose field n is the count and whose other fields carry
-// what the store needs, and Op::store(i, exclusive, item) consumes the result. k_scan_reduce writes the
-// sum of every tile of kScanTile items; k_scan_apply derives each tile's
-// prefix from those sums and scans the tile in kScanRounds striped rounds of
-// kBlock consecutive items (coalesced loads and stores). No block waits on
-// another.
+// what the store needs, and Op::store(i, exclusive, item) consumes the
+// result. k_scan_reduce writes the sum of every tile of kBlock * R items (R
+// rounds, ScanRounds); k_scan_apply derives each tile's prefix from those
+// sums and scans the tile in R striped rounds of kBlock consecutive items
+// (coalesced loads and stores). No block waits on another.
 // ---------------------------------------------------------------------------
-constexpr int kScanRounds = 4;
-constexpr int kScanTile = kBlock * kScanRounds;
+constexpr int kScanRounds = 4;  // rounds per tile unless the operation sets its own kRounds
 
-__host__ __device__ constexpr long long scan_tiles(long long n) { return (n + kScanTile - 1) / kScanTile; }
+// rounds per tile of an operation: Op::kRounds where declared
+template <class Op, class = void>
+struct ScanRounds {
+  static constexpr int value = kScanRounds;
+};
+template <class Op>
+struct ScanRounds<Op, decltype(void(Op::kRounds))> {
+  static constexpr int value = Op::kRounds;
+};
+
+__host__ __device__ constexpr long long scan_tiles(long long n, int rounds = kScanRounds) {
+  return (n + (long long)kBlock * rounds - 1) / ((long long)kBlock * rounds);
+}
 
 __device__ __forceinline__ long long item_count(long long x) { return x; }
 template <class I>
@@ -612,12 +624,13 @@ template <class Op>
 __device__ __forceinline__ void scan_reduce_body(long long n, const Op& op, long long* tile_sum, long long bid,
                                                  long long nblk) {
   __shared__ long long sw[kWavesPerBlock];
-  const long long n_tiles = scan_tiles(n);
+  constexpr int R = ScanRounds<Op>::value;
+  const long long n_tiles = scan_tiles(n, R);
   for (long long tile = bid; tile < n_tiles; tile += nblk) {
     long long sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanRounds; ++k) {
-      const long long i = tile * kScanTile + (long long)k * kBlock + threadIdx.x;
+    for (int k = 0; k < R; ++k) {
+      const long long i = tile * (long long)(kBlock * R) + (long long)k * kBlock + threadIdx.x;
       if (i < n) sum += item_count(op.load(i));
     }
     sum = block_sum(sum, sw);
@@ -631,17 +644,18 @@ __device__ __forceinline__ void scan_reduce_body(long long n, const Op& op, long
 template <class Op>
 __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const long long* tile_sum, long long bid,
                                                 long long nblk) {
-  __shared__ long long sw[kScanRounds][kWavesPerBlock];
+  constexpr int R = ScanRounds<Op>::value;
+  __shared__ long long sw[R][kWavesPerBlock];
   __shared__ long long sp[kWavesPerBlock];
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  const long long n_tiles = scan_tiles(n);
+  const long long n_tiles = scan_tiles(n, R);
   for (long long tile = bid; tile < n_tiles; tile += nblk) {
-    const long long base = tile * kScanTile + threadIdx.x;
+    const long long base = tile * (long long)(kBlock * R) + threadIdx.x;
     using Item = decltype(op.load(0));
-    Item item[kScanRounds];
-    long long incl[kScanRounds];
+    Item item[R];
+    long long incl[R];
 #pragma unroll
-    for (int k = 0; k < kScanRounds; ++k) {
+    for (int k = 0; k < R; ++k) {
       const long long i = base + (long long)k * kBlock;
       item[k] = i < n ? op.load(i) : Item{};
       incl[k] = item_count(item[k]);
@@ -653,14 +667,14 @@ __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
 #pragma unroll
-      for (int k = 0; k < kScanRounds; ++k) {
+      for (int k = 0; k < R; ++k) {
         const long long y = __shfl_up(incl[k], d, kWave);
         if (lane >= d) incl[k] += y;
       }
     }
     if (lane == kWave - 1) {
 #pragma unroll
-      for (int k = 0; k < kScanRounds; ++k) sw[k][wid] = incl[k];
+      for (int k = 0; k < R; ++k) sw[k][wid] = incl[k];
     }
     if (lane == 0) sp[wid] = pre;
     __syncthreads();
@@ -668,7 +682,7 @@ __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const
 #pragma unroll
     for (int v = 0; v < kWavesPerBlock; ++v) carry += sp[v];
 #pragma unroll
-    for (int k = 0; k < kScanRounds; ++k) {
+    for (int k = 0; k < R; ++k) {
       long long off = carry;
 #pragma unroll
       for (int v = 0; v < kWavesPerBlock; ++v) {
@@ -791,6 +805,7 @@ struct NestedScan {
 // last group carries nnz and writes the call's status block (nnz, error bits,
 // block-path groups, row groups).
 struct GroupScan {
+  static constexpr int kRounds = 1;  // one group per thread: the store's chunk loop is the long part
   Work w;
   long long n_items;  // row groups the host sized the buffers for, plus one
   long long* status;
@@ -832,6 +847,7 @@ struct GroupScan {
 
 // 1D Gram library: lib[axis][range][kind][a][b] = sum_m w_m f_a(x'_m) g_b(x_m)
 // with x'_m = x_m * s + c mapped into the dyadic sub-range (glq.rs:238-249).
+template <int kBasis>
 __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, const GlqArgs& glq, long long block) {
   __shared__ double s_pts[2][kMaxGlq], s_wts[2][kMaxGlq];
   for (int i = threadIdx.x; i < kMaxGlq; i += blockDim.x) {
@@ -871,7 +887,7 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
   for (int m = 0; m < np; ++m) {
     const double xq = pts[m];
     const double xp = rid == 0 ? xq : xq * s + c;
-    acc += wts[m] * shape_pick(L.basis, fp, a, xp) * shape_pick(L.basis, fq, b, xq);
+    acc += wts[m] * shape_pick<kBasis>(fp, a, xp) * shape_pick<kBasis>(fq, b, xq);
   }
   w.lib[t] = acc;
 }
@@ -894,7 +910,9 @@ __global__ void __launch_bounds__(kBlock) k_gram_elem_specs(DevView v, Work w, u
                                                             unsigned prep_blocks, unsigned spec_blocks, LibShape L,
                                                             GlqArgs glq) {
   if (blockIdx.x < gram_blocks) {
-    gram_body(w, L, glq, blockIdx.x);
+    // one instance per shape-function space (no evaluation of the other's recurrence)
+    if (L.basis == FEM2D_BASIS_KOL) gram_body<FEM2D_BASIS_KOL>(w, L, glq, blockIdx.x);
+    else gram_body<FEM2D_BASIS_HIERPOLY>(w, L, glq, blockIdx.x);
   } else if (blockIdx.x < gram_blocks + prep_blocks) {
     elem_prep_body(v, w, int((blockIdx.x - gram_blocks) * kBlock + threadIdx.x));
   } else if (blockIdx.x < gram_blocks + prep_blocks + spec_blocks) {
@@ -2105,7 +2123,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   // tile sums of the four prefix sums, one region each (two run side by side)
   const long long n_grp_items = (long long)G1 + 1;
   const long long tiles_nested = scan_tiles(ne + 1), tiles_dof = scan_tiles(nd), tiles_mem = scan_tiles(nbs + 1),
-                  tiles_grp = scan_tiles(n_grp_items);
+                  tiles_grp = scan_tiles(n_grp_items, ScanRounds<GroupScan>::value);
   if ((rc = slot(ctx, k++, size_t(tiles_nested + tiles_dof + tiles_mem + tiles_grp), &w.tile_sum))) return rc;
   long long* ts_nested = w.tile_sum;
   long long* ts_dof = ts_nested + tiles_nested;
