@@ -19,7 +19,9 @@ stream around the kernel in the same number of kernel-by-kernel steps;
 summary of that kernel. `host_boundary` times the drop-in entry point with
 host arrays on both sides. `cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
-workload on the host cores.
+workload on the host cores. `mixed_precision` repeats the measurement with the values
+stored as FP32 (FEM2D_PRECISION_FP32: FP64 arithmetic, 12 B per stored entry) and states its
+largest relative entry error against the FP64 values of the whole matrix.
 """
 
 from __future__ import annotations
@@ -99,6 +101,70 @@ def profiled_traffic(workload: str):
     return None
 
 
+def _max_rel_errors(g32, g64):
+    """Largest relative entry error of the FP32-stored values against the FP64 ones, per matrix,
+    with the floor sqrt(|d_r d_c|) of the parity definition (entries that cancel to rounding)."""
+    import numpy as np
+
+    rows = g64.rows()
+    out = {}
+    for name, got, ref in (("A", g32.a_val, g64.a_val), ("B", g32.b_val, g64.b_val)):
+        diag = np.zeros(g64.n)
+        on = rows == g64.col
+        diag[rows[on]] = np.abs(ref[on])
+        scale = np.maximum(np.abs(ref), np.sqrt(diag[rows] * diag[g64.col]))
+        err = np.abs(got - ref)
+        ok = scale > 0
+        out[name] = float(np.max(err[ok] / scale[ok])) if ok.any() else 0.0
+    return out
+
+
+def mixed_precision(dd, w, args, nnz: int, ms_fp64: float, fill_fp64_ms: float):
+    """The same assembly with FEM2D_PRECISION_FP32: FP64 arithmetic, the A and B values stored
+    as FP32 (12 B per stored entry instead of 20). Timed like the FP64 steps (graph replays;
+    the fill kernel kernel by kernel); its values are compared with the FP64 GPU values of the
+    whole matrix."""
+    import numpy as np
+
+    import fem_2d_amd as F
+
+    ctx = dd.ctx
+
+    def step(precision):
+        return dd.assemble(w.glq, w.basis, F.CurlCurl, F.L2Inner, precision=precision)
+
+    g64 = dd.download(step(F.FP64))
+    for _ in range(args.warmup):
+        step(F.FP32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step(F.FP32)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    ctx.set_graphs(False)
+    step(F.FP32)
+    fill_ms = []
+    for _ in range(args.steps):
+        step(F.FP32)
+        fill_ms.append(ctx.last_timing()["fill_kernel_ms"])
+    ctx.set_graphs(True)
+    g32 = dd.download(step(F.FP32))
+    assert int(out.nnz) == nnz and g32.nnz == nnz
+    fill = float(np.mean(fill_ms))
+    return {
+        "precision": "fp32 values, fp64 arithmetic (FEM2D_PRECISION_FP32)",
+        "value": nnz / (ms * 1e-3),
+        "unit": "nonzeros/s",
+        "ms_per_step": ms,
+        "speedup_vs_fp64": ms_fp64 / ms,
+        "fill_kernel_ms": fill,
+        "fill_speedup_vs_fp64": fill_fp64_ms / fill,
+        "fill_achieved_GBs": 12 * nnz / (fill * 1e-3) / 1e9,
+        "max_rel_entry_error_vs_fp64": _max_rel_errors(g32, g64),
+    }
+
+
 def host_boundary(w, nnz: int, calls: int = 3):
     """The drop-in entry point with host arrays on both sides: view upload, assembly and the
     CSR download into host memory (PCIe-inclusive)."""
@@ -137,6 +203,8 @@ def main():
     ap.add_argument("--no-host-boundary", action="store_true",
                     help="skip timing the host-array entry point (upload + assembly + download)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the FP32-storage (mixed precision) assembly and its error check")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -200,6 +268,7 @@ def main():
         count_ms.append(t["count_kernel_ms"])
     ctx.set_graphs(True)
     fill_avg = float(np.mean(fill_ms))
+    mixed = None if args.no_mixed else mixed_precision(dd, w, args, nnz, ms_per_step, fill_avg)
 
     # element pairs visited by the reference's loops (pattern-independent work count)
     a = w.domain.arrays()
@@ -257,6 +326,8 @@ def main():
         },
     }
     line["roofline"]["traffic"] = profiled_traffic(w.name)
+    if mixed is not None:
+        line["mixed_precision"] = mixed
     if not args.no_host_boundary:
         line["host_boundary"] = host_boundary(w, nnz)
     if not args.no_cpu_baseline:

@@ -9,16 +9,16 @@ GPU.
 from .domain import BasisDir, BasisSpec, ContinuityCondition, Domain
 from .errors import (EmptyDOFSet, Fem2dError, GalerkinSamplingError, HRefError, InvalidGLQSettings, MeshAccessError,
                      MeshPanic, NativeError, PRefError, WrongContinuityCondition)
-from .galerkin import (GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace, HierCurlIntegral,
-                       HierPoly, KOLShapeFn, L2Inner, ShapeFn, default_context, galerkin_sample_gep_hcurl,
-                       gauss_quadrature_points)
+from .galerkin import (FP32, FP64, GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace,
+                       HierCurlIntegral, HierPoly, KOLShapeFn, L2Inner, ShapeFn, default_context,
+                       galerkin_sample_gep_hcurl, gauss_quadrature_points)
 from .mesh import (MAX_POLYNOMIAL_ORDER, MIN_EDGE_LENGTH, Edge, Elem, HLevels, HRef, Mesh, Node, Orders, PolyOrders,
                    PRef)
 
 __all__ = [
     "BasisDir", "BasisSpec", "ContinuityCondition", "Domain", "EmptyDOFSet", "Fem2dError", "GalerkinSamplingError",
     "HRefError", "InvalidGLQSettings", "MeshAccessError", "MeshPanic", "NativeError", "PRefError",
-    "WrongContinuityCondition", "GEP", "MIN_GLQ_ORDER", "Context", "CurlCurl", "DeviceDomain", "HierCurlBasisFnSpace",
+    "WrongContinuityCondition", "FP32", "FP64", "GEP", "MIN_GLQ_ORDER", "Context", "CurlCurl", "DeviceDomain", "HierCurlBasisFnSpace",
     "HierCurlIntegral", "HierPoly", "KOLShapeFn", "L2Inner", "ShapeFn", "default_context",
     "galerkin_sample_gep_hcurl", "gauss_quadrature_points", "MAX_POLYNOMIAL_ORDER", "MIN_EDGE_LENGTH", "Edge", "Elem",
     "HLevels", "HRef", "Mesh", "Node", "Orders", "PolyOrders", "PRef",

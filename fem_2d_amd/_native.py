@@ -54,8 +54,16 @@ class GepCsr(C.Structure):
     ]
 
 
+PRECISION_FP64 = 0
+PRECISION_FP32 = 1
+
+
 class GepDevice(C.Structure):
-    _fields_ = GepCsr._fields_
+    _fields_ = GepCsr._fields_ + [
+        ("precision", C.c_int32),
+        ("a_val32", C.POINTER(C.c_float)),
+        ("b_val32", C.POINTER(C.c_float)),
+    ]
 
 
 class Timing(C.Structure):

@@ -188,11 +188,28 @@ struct LibShape {
 
 struct FillOut {
   int* col;
-  double* aval;
+  double* aval;  // FP64 values (these and the FP32 ones share the output buffers)
   double* bval;
+  float* aval32;  // FP32 values
+  float* bval32;
   long long capacity;  // entries the output buffers hold
   int a_is_cc, b_is_cc;
+  int fp32;
 };
+
+// the value arrays of one precision
+template <class V>
+__device__ __forceinline__ V* out_a(const FillOut& o);
+template <>
+__device__ __forceinline__ double* out_a<double>(const FillOut& o) { return o.aval; }
+template <>
+__device__ __forceinline__ float* out_a<float>(const FillOut& o) { return o.aval32; }
+template <class V>
+__device__ __forceinline__ V* out_b(const FillOut& o);
+template <>
+__device__ __forceinline__ double* out_b<double>(const FillOut& o) { return o.bval; }
+template <>
+__device__ __forceinline__ float* out_b<float>(const FillOut& o) { return o.bval32; }
 
 __device__ __forceinline__ void set_err(int* err, int bits) { atomicOr(err, bits); }
 
@@ -1253,8 +1270,14 @@ __global__ void __launch_bounds__(kBlock) k_describe_wave(DevView v, Work w) {
 __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int c, double cc, double l2) {
   if (pos >= 0 && pos < o.capacity) {
     o.col[pos] = c;
-    o.aval[pos] = o.a_is_cc ? cc : l2;
-    o.bval[pos] = o.b_is_cc ? cc : l2;
+    const double a = o.a_is_cc ? cc : l2, b = o.b_is_cc ? cc : l2;
+    if (o.fp32) {
+      o.aval32[pos] = float(a);
+      o.bval32[pos] = float(b);
+    } else {
+      o.aval[pos] = a;
+      o.bval[pos] = b;
+    }
   }
 }
 
@@ -1340,12 +1363,16 @@ __device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, con
 }
 
 // Streaming fill: the CSR is cut into chunks of kChunk consecutive entries;
-// each wavefront takes chunks in turn and writes them with consecutive lanes
-// on consecutive entries, reading the row groups' records (no LDS, no
-// per-group set-up, equal work per chunk). Entries of block-path groups are
-// left to k_fill_large.
-// kACurlCurl / kBCurlCurl: A and B take the CurlCurl values (else L2Inner)
-template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl>
+// wavefront v takes chunks v, v + waves, ..., so that the wavefronts write
+// neighbouring chunks at any time. For each row group a chunk meets, the
+// wavefront stages the group's source records in LDS and writes the entries
+// with consecutive lanes on consecutive entries (coalesced, non-temporal
+// stores), reading the group's column and row records and the Gram library (a
+// per-block LDS copy when it fits). Entries of block-path groups are left to
+// k_fill_large. kACurlCurl / kBCurlCurl: A and B take the CurlCurl values
+// (else L2Inner); V: the stored precision (double, or float for
+// FEM2D_PRECISION_FP32; the arithmetic is FP64 either way).
+template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl, class V>
 __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
   if (*w.err) return;
   // the source records of the row group being written, per wavefront
@@ -1382,8 +1409,8 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
     const int len = int(min((long long)kChunk, nnz - pbase));
     const long long pb = uniform(pbase);
     int* __restrict__ col_out = o.col + pb;
-    double* __restrict__ a_out = o.aval + pb;
-    double* __restrict__ b_out = o.bval + pb;
+    V* __restrict__ a_out = out_a<V>(o) + pb;
+    V* __restrict__ b_out = out_b<V>(o) + pb;
     // per-group values are wave-uniform: keep them (and the record bases) in
     // scalar registers
     int g = uniform(w.chunk_first[ch]);
@@ -1415,12 +1442,12 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
           flat_entry2(col_g, row_g, coef_g, lib, tab, j + off, (second ? j + kWave : j) + off, T, nrows, bq, c0, cc0,
                       l20, c1, cc1, l21);
           st32(col_out, unsigned(j), c0);
-          st32(a_out, unsigned(j), kACurlCurl ? cc0 : l20);
-          st32(b_out, unsigned(j), kBCurlCurl ? cc0 : l20);
+          st32(a_out, unsigned(j), V(kACurlCurl ? cc0 : l20));
+          st32(b_out, unsigned(j), V(kBCurlCurl ? cc0 : l20));
           if (second) {
             st32(col_out, unsigned(j + kWave), c1);
-            st32(a_out, unsigned(j + kWave), kACurlCurl ? cc1 : l21);
-            st32(b_out, unsigned(j + kWave), kBCurlCurl ? cc1 : l21);
+            st32(a_out, unsigned(j + kWave), V(kACurlCurl ? cc1 : l21));
+            st32(b_out, unsigned(j + kWave), V(kBCurlCurl ? cc1 : l21));
           }
         }
         wave_sync();  // every lane is done with the staged records
@@ -1431,20 +1458,25 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
   }
 }
 
-// one instance per library placement and integral pair
-template <bool kLibInLds>
+// one instance per library placement, integral pair and stored precision
+template <bool kLibInLds, class V>
 void launch_fill_lib(bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
                      const FillOut& o) {
-  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false><<<grid, kFillBlock, 0, st>>>(w, L, o);
-  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true><<<grid, kFillBlock, 0, st>>>(w, L, o);
-  else if (a_cc) k_fill_flat<kLibInLds, true, true><<<grid, kFillBlock, 0, st>>>(w, L, o);
-  else k_fill_flat<kLibInLds, false, false><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  else if (a_cc) k_fill_flat<kLibInLds, true, true, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  else k_fill_flat<kLibInLds, false, false, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
 }
 
 void launch_fill(bool lds, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
                  const FillOut& o) {
-  if (lds) launch_fill_lib<true>(a_cc, b_cc, grid, st, w, L, o);
-  else launch_fill_lib<false>(a_cc, b_cc, grid, st, w, L, o);
+  if (lds) {
+    if (o.fp32) launch_fill_lib<true, float>(a_cc, b_cc, grid, st, w, L, o);
+    else launch_fill_lib<true, double>(a_cc, b_cc, grid, st, w, L, o);
+  } else {
+    if (o.fp32) launch_fill_lib<false, float>(a_cc, b_cc, grid, st, w, L, o);
+    else launch_fill_lib<false, double>(a_cc, b_cc, grid, st, w, L, o);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1688,9 +1720,11 @@ struct GraphKey {
   long long capacity = -1;
   unsigned long long alloc_generation = 0;
   bool run_large = true;
+  int precision = -1;
   bool operator==(const GraphKey& o) const {
     return domain == o.domain && basis == o.basis && ai == o.ai && bi == o.bi && nu == o.nu && nv == o.nv &&
-           capacity == o.capacity && alloc_generation == o.alloc_generation && run_large == o.run_large;
+           capacity == o.capacity && alloc_generation == o.alloc_generation && run_large == o.run_large &&
+           precision == o.precision;
   }
 };
 
@@ -1837,9 +1871,9 @@ int check_common(int cc, long long n_dofs, int glq_u, int glq_v, int basis, int 
     set_error("unknown integral", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
-  if (precision != FEM2D_PRECISION_FP64) {
-    set_error("only FEM2D_PRECISION_FP64 is provided", 0);
-    return FEM2D_ERR_UNSUPPORTED;
+  if (precision != FEM2D_PRECISION_FP64 && precision != FEM2D_PRECISION_FP32) {
+    set_error("unknown precision", 0);
+    return FEM2D_ERR_INVALID_ARG;
   }
   return FEM2D_OK;
 }
@@ -1892,9 +1926,11 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_describe_wave, kBlock, 0));
   ctx->count_grid = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false>, kFillBlock, 0));
+  HIP_TRY(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, double>, kFillBlock, 0));
   ctx->fill_grid = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false>, kFillBlock, 0));
+  HIP_TRY(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, double>, kFillBlock, 0));
   ctx->fill_grid_lds = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
@@ -2242,6 +2278,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     o.col = static_cast<int*>(ctx->out_col.p);
     o.aval = static_cast<double*>(ctx->out_a.p);
     o.bval = static_cast<double*>(ctx->out_b.p);
+    o.aval32 = static_cast<float*>(ctx->out_a.p);
+    o.bval32 = static_cast<float*>(ctx->out_b.p);
+    o.fp32 = precision == FEM2D_PRECISION_FP32 ? 1 : 0;
     o.capacity = ctx->out_capacity;
     o.a_is_cc = ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
     o.b_is_cc = bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
@@ -2250,7 +2289,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     const bool fill = o.capacity > 0;
     const bool tail = fill && run_large;  // part (c) has work
     if (ctx->use_graphs) {
-      const GraphKey key{dd->serial, basis, ai, bi, nu, nv, o.capacity, g_alloc_generation.load(), run_large};
+      const GraphKey key{dd->serial, basis,     ai,       bi, nu, nv, o.capacity, g_alloc_generation.load(),
+                         run_large,  precision};
       if (!ctx->graph_exec[0] || !(key == ctx->graph_key)) {
         ctx->graph_key = GraphKey{};
         if ((rc = record(0, [&] {
@@ -2324,8 +2364,12 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   out->nnz = nnz;
   out->row_ptr = w.row_ptr;
   out->col = static_cast<int32_t*>(ctx->out_col.p);
-  out->a_val = static_cast<double*>(ctx->out_a.p);
-  out->b_val = static_cast<double*>(ctx->out_b.p);
+  const bool fp32 = precision == FEM2D_PRECISION_FP32;
+  out->precision = precision;
+  out->a_val = fp32 ? nullptr : static_cast<double*>(ctx->out_a.p);
+  out->b_val = fp32 ? nullptr : static_cast<double*>(ctx->out_b.p);
+  out->a_val32 = fp32 ? static_cast<float*>(ctx->out_a.p) : nullptr;
+  out->b_val32 = fp32 ? static_cast<float*>(ctx->out_b.p) : nullptr;
   return FEM2D_OK;
 }
 
@@ -2352,12 +2396,32 @@ int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d
   if (src->nnz > 0) {
     HIP_TRY(hipMemcpyAsync(out->col, src->col, size_t(src->nnz) * sizeof(int32_t), hipMemcpyDeviceToHost,
                            ctx->stream));
-    HIP_TRY(hipMemcpyAsync(out->a_val, src->a_val, size_t(src->nnz) * sizeof(double), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    HIP_TRY(hipMemcpyAsync(out->b_val, src->b_val, size_t(src->nnz) * sizeof(double), hipMemcpyDeviceToHost,
-                           ctx->stream));
+    if (src->precision == FEM2D_PRECISION_FP32) {
+      // FP32 values land in the front half of the double arrays and are widened in place
+      HIP_TRY(hipMemcpyAsync(out->a_val, src->a_val32, size_t(src->nnz) * sizeof(float), hipMemcpyDeviceToHost,
+                             ctx->stream));
+      HIP_TRY(hipMemcpyAsync(out->b_val, src->b_val32, size_t(src->nnz) * sizeof(float), hipMemcpyDeviceToHost,
+                             ctx->stream));
+    } else {
+      HIP_TRY(hipMemcpyAsync(out->a_val, src->a_val, size_t(src->nnz) * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+      HIP_TRY(hipMemcpyAsync(out->b_val, src->b_val, size_t(src->nnz) * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+    }
   }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (src->precision == FEM2D_PRECISION_FP32) {
+    // from the last entry down, each double covers floats it has already read
+    for (double* v : {out->a_val, out->b_val}) {
+      unsigned char* bytes = reinterpret_cast<unsigned char*>(v);
+      for (long long i = src->nnz - 1; i >= 0; --i) {
+        float x;
+        std::memcpy(&x, bytes + 4 * i, sizeof(float));
+        const double y = double(x);
+        std::memcpy(bytes + 8 * i, &y, sizeof(double));
+      }
+    }
+  }
   return FEM2D_OK;
 }
 

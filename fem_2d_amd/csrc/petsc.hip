@@ -86,8 +86,10 @@ __global__ void k_petsc_rows(long long n, long long nnz, const int64_t* __restri
 
 // Full rows in big-endian words: lower part from the sorted strict entries,
 // then the stored upper row (one wavefront per row).
+// V: the GEP's stored precision (FP32 values are widened to the format's f64)
+template <class V>
 __global__ void k_petsc_fill(long long n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                             const double* __restrict__ a, const double* __restrict__ b,
+                             const V* __restrict__ a, const V* __restrict__ b,
                              const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ row_of,
                              const int64_t* __restrict__ lower_start, uint32_t* __restrict__ col_be,
                              uint64_t* __restrict__ a_be, uint64_t* __restrict__ b_be) {
@@ -109,8 +111,8 @@ __global__ void k_petsc_fill(long long n, const int64_t* __restrict__ row_ptr, c
         c = uint32_t(col[k]);
       }
       col_be[out0 + j] = be32(c);
-      a_be[out0 + j] = be64(a[k]);
-      b_be[out0 + j] = be64(b[k]);
+      a_be[out0 + j] = be64(double(a[k]));
+      b_be[out0 + j] = be64(double(b[k]));
     }
   }
 }
@@ -225,8 +227,12 @@ int fem2d_gep_device_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep
   PETSC_TRY(m.alloc(&a_be, size_t(full)));
   PETSC_TRY(m.alloc(&b_be, size_t(full)));
   if (n > 0) {
-    k_petsc_fill<<<std::max(1u, row_waves), block, 0, st>>>(n, gep->row_ptr, gep->col, gep->a_val, gep->b_val, sidx,
-                                                            row_of, lower_start, col_be, a_be, b_be);
+    if (gep->precision == FEM2D_PRECISION_FP32)
+      k_petsc_fill<<<std::max(1u, row_waves), block, 0, st>>>(n, gep->row_ptr, gep->col, gep->a_val32, gep->b_val32,
+                                                              sidx, row_of, lower_start, col_be, a_be, b_be);
+    else
+      k_petsc_fill<<<std::max(1u, row_waves), block, 0, st>>>(n, gep->row_ptr, gep->col, gep->a_val, gep->b_val, sidx,
+                                                              row_of, lower_start, col_be, a_be, b_be);
     PETSC_TRY(hipGetLastError());
   }
   std::vector<uint32_t> counts(static_cast<size_t>(n)), cols(static_cast<size_t>(full));

@@ -18,6 +18,11 @@ from .errors import check
 
 MIN_GLQ_ORDER = 4  # galerkin.rs:13
 
+# precision of the assembled values (fem2d_precision): FP64 is the reference's; FP32
+# keeps the FP64 arithmetic and stores the A and B values rounded to FP32
+FP64 = N.PRECISION_FP64
+FP32 = N.PRECISION_FP32
+
 
 # -- generic arguments of galerkin_sample_gep_hcurl ------------------------
 class HierCurlBasisFnSpace:
@@ -206,16 +211,18 @@ def _glq(glq_grid_dim: Optional[Sequence[int]]):
 
 
 def galerkin_sample_gep_hcurl(domain: Domain, glq_grid_dim: Optional[Sequence[int]] = None,
-                              BSpace=HierPoly, AI=CurlCurl, BI=L2Inner, ctx: Optional[Context] = None) -> GEP:
+                              BSpace=HierPoly, AI=CurlCurl, BI=L2Inner, ctx: Optional[Context] = None,
+                              precision: int = FP64) -> GEP:
     """Fill A (``AI``) and B (``BI``) over the Domain's basis space on the GPU (galerkin.rs:33-187).
 
-    Raises WrongContinuityCondition, EmptyDOFSet or InvalidGLQSettings in the reference's order."""
+    Raises WrongContinuityCondition, EmptyDOFSet or InvalidGLQSettings in the reference's order.
+    ``precision`` FP32 stores the values rounded to FP32 (returned widened to float64)."""
     ctx = ctx or default_context()
     gu, gv = _glq(glq_grid_dim)
     out = N.GepCsr()
     lib = N.load()
     check(lib.fem2d_galerkin_sample_gep_hcurl(ctx._h, C.byref(domain.view()), BSpace.code, AI.code, BI.code, gu, gv,
-                                              0, C.byref(out)))
+                                              int(precision), C.byref(out)))
     try:
         return GEP._from_csr(out)
     finally:
@@ -240,12 +247,13 @@ class DeviceDomain:
         except Exception:
             pass
 
-    def assemble(self, glq_grid_dim=None, BSpace=HierPoly, AI=CurlCurl, BI=L2Inner) -> N.GepDevice:
+    def assemble(self, glq_grid_dim=None, BSpace=HierPoly, AI=CurlCurl, BI=L2Inner, precision: int = FP64
+                 ) -> N.GepDevice:
         """Assemble into context-owned device buffers (valid until the next assembly)."""
         gu, gv = _glq(glq_grid_dim)
         out = N.GepDevice()
-        check(self._lib.fem2d_assemble_device(self.ctx._h, self._h, BSpace.code, AI.code, BI.code, gu, gv, 0,
-                                              C.byref(out)))
+        check(self._lib.fem2d_assemble_device(self.ctx._h, self._h, BSpace.code, AI.code, BI.code, gu, gv,
+                                              int(precision), C.byref(out)))
         return out
 
     def print_to_petsc_binary_files(self, dev: N.GepDevice, dir: str, prefix: str) -> None:

@@ -64,9 +64,12 @@ enum fem2d_basis { FEM2D_BASIS_HIERPOLY = 0, FEM2D_BASIS_KOL = 1 };
 /* HierCurlIntegral implementations (integrals.rs): CurlCurl and L2Inner */
 enum fem2d_integral { FEM2D_INTEGRAL_CURLCURL = 0, FEM2D_INTEGRAL_L2INNER = 1 };
 
-/* Arithmetic of the assembly. Only FP64 is provided: the separable (sum-factorised)
- * contraction leaves the FP64 path bound by HBM writes of the FP64 output. */
-enum fem2d_precision { FEM2D_PRECISION_FP64 = 0 };
+/* Precision of the assembled values. FP64: the reference's. FP32 (mixed): the same
+ * FP64 arithmetic, the A and B values stored rounded to FP32, so a stored entry
+ * takes 12 bytes of HBM instead of 20 (the pattern is the same). The separable
+ * (sum-factorised) contraction leaves the assembly bound by these writes. Host
+ * results (fem2d_gep_csr) hold FP32 values widened to double. */
+enum fem2d_precision { FEM2D_PRECISION_FP64 = 0, FEM2D_PRECISION_FP32 = 1 };
 
 /* BasisDir (basis_spec.rs:259-268) */
 enum fem2d_basis_dir { FEM2D_DIR_U = 0, FEM2D_DIR_V = 1, FEM2D_DIR_W = 2 };
@@ -159,8 +162,11 @@ typedef struct fem2d_gep_device {
   int64_t nnz;
   int64_t* row_ptr;
   int32_t* col;
-  double* a_val;
+  double* a_val;     /* FEM2D_PRECISION_FP64 values (NULL for FP32) */
   double* b_val;
+  int32_t precision; /* fem2d_precision of the values */
+  float* a_val32;    /* FEM2D_PRECISION_FP32 values (NULL for FP64) */
+  float* b_val32;
 } fem2d_gep_device;
 
 int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* domain, fem2d_device_domain** out);

@@ -51,7 +51,8 @@ def test_boundary_errors_before_device_work():
     assert lib.fem2d_galerkin_sample_gep_hcurl(None, C.byref(v), 0, 0, 1, 3, 8, 0, C.byref(out)) == N.ERR_INVALID_GLQ
     assert lib.fem2d_galerkin_sample_gep_hcurl(None, C.byref(v), 0, 0, 1, 0, 0, 0, C.byref(out)) == N.ERR_INVALID_ARG
     assert lib.fem2d_galerkin_sample_gep_hcurl(None, C.byref(v), 7, 0, 1, 8, 8, 0, C.byref(out)) == N.ERR_INVALID_ARG
-    assert lib.fem2d_galerkin_sample_gep_hcurl(None, C.byref(v), 0, 0, 1, 8, 8, 5, C.byref(out)) == N.ERR_UNSUPPORTED
+    # precision codes: FEM2D_PRECISION_FP64 (0) and FEM2D_PRECISION_FP32 (1)
+    assert lib.fem2d_galerkin_sample_gep_hcurl(None, C.byref(v), 0, 0, 1, 8, 8, 5, C.byref(out)) == N.ERR_INVALID_ARG
 
 
 def test_context_without_device_fails_loudly():
