@@ -77,6 +77,8 @@ constexpr int kMaxRows = 1024;
 constexpr int kFillBlock = 512;
 constexpr int kFillWavesPerSimd = 6;
 constexpr int kLibLdsMax = 4096;
+// latency-bound kernels: at most 64 VGPRs, so that 8 wavefronts per SIMD are resident
+constexpr int kLatencyWavesPerSimd = 8;
 constexpr int kMaxDepth = 24;      // h-refinement depth below a base Elem
 constexpr int kMaxRelDepth = 12;   // dyadic halvings per axis covered by the Gram library
 constexpr int kMaxOrder = 20;      // MAX_POLYNOMIAL_ORDER (mesh.rs:42)
@@ -719,6 +721,7 @@ __device__ __forceinline__ void scan_apply_body(long long n, const Op& op, const
 // there: a membership is a run of an Elem's spec list that is one whole row
 // group (it starts at the group's first DoF and ends at its last).
 struct DofGroupScan {
+  static constexpr int kRounds = 1;  // the items are large: one per thread keeps 8 wavefronts per SIMD
   DevView v;
   Work w;
   struct Item {
@@ -1209,7 +1212,7 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
 // source prefactors and infos, the packed specs of its rows, and for each
 // candidate column t = 0..T-1 of its first row (candidate index u_self + t)
 // the column and the packed specs of its one or two contributions.
-__global__ void __launch_bounds__(kBlock) k_describe_wave(DevView v, Work w) {
+__global__ void __launch_bounds__(kBlock, kLatencyWavesPerSimd) k_describe_wave(DevView v, Work w) {
   if (*w.err) return;  // an invalid view was detected: nothing below may trust it
   __shared__ WaveShared shared[kWavesPerBlock];
   const int lane = threadIdx.x % kWave;
@@ -1769,6 +1772,7 @@ struct fem2d_ctx {
   long long* h_status = nullptr;  // mapped pinned status block: nnz, err, n_large, n_groups
   long long* d_status = nullptr;  // its device address
   GlqArgs glq{};
+  int glq_n[2] = {0, 0};  // point counts glq holds
   fem2d_timing timing{};
   int n_cu = 0;
   int count_grid = 0;  // resident blocks of the wave kernels on this device
@@ -2158,7 +2162,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
 
   // tile sums of the four prefix sums, one region each (two run side by side)
   const long long n_grp_items = (long long)G1 + 1;
-  const long long tiles_nested = scan_tiles(ne + 1), tiles_dof = scan_tiles(nd), tiles_mem = scan_tiles(nbs + 1),
+  const long long tiles_nested = scan_tiles(ne + 1), tiles_dof = scan_tiles(nd, ScanRounds<DofGroupScan>::value),
+                  tiles_mem = scan_tiles(nbs + 1),
                   tiles_grp = scan_tiles(n_grp_items, ScanRounds<GroupScan>::value);
   if ((rc = slot(ctx, k++, size_t(tiles_nested + tiles_dof + tiles_mem + tiles_grp), &w.tile_sum))) return rc;
   long long* ts_nested = w.tile_sum;
@@ -2172,9 +2177,14 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   hipStream_t st = ctx->stream;
   // the previous call's readback of the pinned staging buffers has completed
   // (every call ends with a stream synchronisation)
-  ctx->glq = GlqArgs{};
-  fem2d::gauss_legendre_nodes(nu, ctx->glq.u_pts, ctx->glq.u_wts);
-  fem2d::gauss_legendre_nodes(nv, ctx->glq.v_pts, ctx->glq.v_wts);
+  // Gauss-Legendre nodes and weights per axis (a function of the point counts only)
+  if (ctx->glq_n[0] != nu || ctx->glq_n[1] != nv) {
+    ctx->glq = GlqArgs{};
+    fem2d::gauss_legendre_nodes(nu, ctx->glq.u_pts, ctx->glq.u_wts);
+    fem2d::gauss_legendre_nodes(nv, ctx->glq.v_pts, ctx->glq.v_wts);
+    ctx->glq_n[0] = nu;
+    ctx->glq_n[1] = nv;
+  }
 
   DevView v;
   v.n_elems = ne;
