@@ -256,7 +256,7 @@ __device__ void hier_poly(int n, double x, double& N, double& Nd, double& T, dou
 // T_n = c_n * sum_{k<=n} q_{n,k} L_k with q_{n,n} = 1 and
 // q_{n,k} = -(2k+1) / (n(n-1)/2) for k < n, k = n (mod 2); c_2..c_8 are the
 // Euclidean normalisation constants of hierarchical_basis_fns.rs:206-209.
-__device__ void kol(int n, double x, double& N, double& Nd, double& T, double& Td) {
+__device__ __noinline__ void kol(int n, double x, double& N, double& Nd, double& T, double& Td) {
   // L_k and L'_k are produced by the recurrence and folded into the T sum in
   // increasing k (no per-thread arrays)
   const bool bubble = n >= 2;
@@ -877,18 +877,19 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
     s_wts[1][i] = glq.v_wts[i];
   }
   __syncthreads();
-  const long long per_axis = (long long)L.nr * kKinds * L.p1 * L.p1;
-  long long t = block * blockDim.x + threadIdx.x;
-  if (t >= 2 * per_axis) return;
-  // layout [axis][range][kind][a][b]
-  const int axis = int(t / per_axis);
-  long long r = t % per_axis;
-  const int b = int(r % L.p1);
-  r /= L.p1;
-  const int a = int(r % L.p1);
-  r /= L.p1;
-  const int kind = int(r % kKinds);
-  const int rid = int(r / kKinds);
+  // layout [axis][range][kind][a][b]; the library holds fewer than 2^31 values
+  const unsigned per_axis = unsigned(L.nr) * unsigned(kKinds * L.p1 * L.p1);
+  const unsigned t = unsigned(block) * blockDim.x + threadIdx.x;
+  if (block >= (2LL * per_axis + blockDim.x - 1) / blockDim.x || t >= 2u * per_axis) return;
+  const unsigned p1 = unsigned(L.p1);
+  const int axis = t >= per_axis ? 1 : 0;
+  unsigned r = t - (axis ? per_axis : 0u);
+  const int b = int(r % p1);
+  r /= p1;
+  const int a = int(r % p1);
+  r /= p1;
+  const int kind = int(r % unsigned(kKinds));
+  const int rid = int(r / unsigned(kKinds));
   double s, c;
   range_of(rid, s, c);
   const int np = axis == 0 ? L.nu : L.nv;
@@ -904,6 +905,7 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
     default: fp = 2; fq = 2; break;
   }
   double acc = 0.0;
+#pragma unroll 1
   for (int m = 0; m < np; ++m) {
     const double xq = pts[m];
     const double xp = rid == 0 ? xq : xq * s + c;
