@@ -1124,26 +1124,23 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
       S.src_info[lane] = k | (rel << 1) | (ru << 3) | (rv << 17);
     }
   }
-  int np2 = 1;
-  while (np2 < n_raw) np2 <<= 1;
-  for (int r = n_raw + lane; r < np2; r += kWave) S.u.sort.key[r] = ~0ull;
-  wave_sync();
-  // bitonic sort by (row group, gather order)
-  for (int kk = 2; kk <= np2; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < np2; i += kWave) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const unsigned long long a = S.u.sort.key[i], b = S.u.sort.key[ixj];
-          if ((a > b) == ((i & kk) == 0)) {
-            S.u.sort.key[i] = b;
-            S.u.sort.key[ixj] = a;
-          }
-        }
-      }
-      wave_sync();
-    }
+  wave_sync();  // every raw key is in LDS
+  // rank sort by (row group, gather order): the keys are distinct (they carry
+  // the raw index), so key i belongs at #{k : key_k < key_i}. Each lane ranks
+  // its two keys against every key (LDS broadcasts), then all write in place.
+  static_assert(kWR == 2 * kWave, "two raw candidates per lane");
+  unsigned long long mine[2];
+  int rank[2] = {0, 0};
+  for (int h = 0; h < 2; ++h) mine[h] = lane + h * kWave < n_raw ? S.u.sort.key[lane + h * kWave] : ~0ull;
+  for (int k = 0; k < n_raw; ++k) {
+    const unsigned long long kk = S.u.sort.key[k];
+    rank[0] += kk < mine[0] ? 1 : 0;
+    rank[1] += kk < mine[1] ? 1 : 0;
   }
+  wave_sync();  // every lane has read every key
+  for (int h = 0; h < 2; ++h)
+    if (lane + h * kWave < n_raw) S.u.sort.key[rank[h]] = mine[h];
+  wave_sync();
   // dedupe: first pass writes every new row group, second the second sources
   int running = 0;
   int bad = 0;
