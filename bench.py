@@ -17,7 +17,8 @@ one): achieved = algorithmic bytes (20 B per stored entry: 4 B column + 8 B A
 stream around the kernel in the same number of kernel-by-kernel steps;
 `traffic` is the HBM bytes per launch from the committed rocprofv3 counter
 summary of that kernel. `host_boundary` times the drop-in entry point with
-host arrays on both sides. `cpu_baseline` times the oracle (the C++
+host arrays on both sides. `domain_build` times the host Domain construction
+from the workload's Mesh, data-parallel and sequential. `cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
 workload on the host cores. `mixed_precision` repeats the measurement with the values
 stored as FP32 (FEM2D_PRECISION_FP32: FP64 arithmetic, 12 B per stored entry) and states its
@@ -193,6 +194,31 @@ def host_boundary(w, nnz: int, calls: int = 3):
             "entry_point": "fem2d_galerkin_sample_gep_hcurl (host view in, host CSR out)"}
 
 
+def domain_build(w, calls: int = 3):
+    """Host Domain construction from the workload's Mesh (Domain::from_mesh, domain.rs:69-159):
+    the data-parallel builder against the sequential one (the reference's construction order);
+    both produce the same Domain. Mean over `calls` builds, after one untimed build of each."""
+    import fem_2d_amd as F
+
+    mesh = w.domain.mesh
+    out = {}
+    for name, build in (("parallel", lambda: F.Domain.from_mesh(mesh, F.ContinuityCondition.HCurl)),
+                        ("serial", lambda: F.Domain.from_mesh_serial(mesh, F.ContinuityCondition.HCurl))):
+        d = build()
+        n = d.num_dofs
+        del d
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            d = build()
+            del d
+        out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / calls
+    out["n_dofs"] = n
+    out["threads"] = min(16, os.cpu_count() or 1)
+    out["speedup"] = out["serial_ms"] / out["parallel_ms"]
+    out["dofs_per_s"] = n / (out["parallel_ms"] * 1e-3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +229,8 @@ def main():
     ap.add_argument("--no-host-boundary", action="store_true",
                     help="skip timing the host-array entry point (upload + assembly + download)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-domain-build", action="store_true",
+                    help="skip timing the host Domain construction (parallel and sequential builders)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the FP32-storage (mixed precision) assembly and its error check")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -333,6 +361,8 @@ def main():
         line["mixed_precision"] = mixed
     if not args.no_host_boundary:
         line["host_boundary"] = host_boundary(w, nnz)
+    if not args.no_domain_build:
+        line["domain_build"] = domain_build(w)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, pairs, nnz, args.cpu_seconds)
     print(json.dumps(line), flush=True)

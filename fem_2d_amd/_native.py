@@ -162,6 +162,8 @@ SIGNATURES = {
     "fem2d_mesh_set_global_expansion_orders": (C.c_int, [vp, i32, i32]),
     "fem2d_mesh_set_expansion_orders": (C.c_int, [vp, P(i32), P(i32), P(i32), i64]),
     "fem2d_domain_from_mesh": (C.c_int, [vp, i32, P(vp)]),
+    "fem2d_domain_from_mesh_threads": (C.c_int, [vp, i32, i32, P(vp)]),
+    "fem2d_domain_from_mesh_serial": (C.c_int, [vp, i32, P(vp)]),
     "fem2d_domain_blank": (C.c_int, [i32, P(vp)]),
     "fem2d_domain_free": (None, [vp]),
     "fem2d_domain_num_dofs": (C.c_int, [vp, P(i64)]),

@@ -10,6 +10,7 @@
 #include "capi_internal.hpp"
 #include "domain.hpp"
 #include "mesh.hpp"
+#include "parallel.hpp"
 
 using namespace fem2d;
 
@@ -26,9 +27,10 @@ struct fem2d_domain {
   std::vector<double> materials;
   std::vector<uint8_t> orders;
   std::vector<int64_t> bs_offsets;
-  std::vector<uint8_t> bs_dir, bs_i, bs_j;
-  std::vector<int32_t> bs_dof;
-  std::vector<int32_t> dof_bs;
+  // sized unwritten and filled by parallel loops
+  fem2d::uninit_vector<uint8_t> bs_dir, bs_i, bs_j;
+  fem2d::uninit_vector<int32_t> bs_dof;
+  fem2d::uninit_vector<int32_t> dof_bs;
 };
 
 namespace fem2d {
@@ -376,7 +378,7 @@ int fem2d_mesh_set_expansion_orders(fem2d_mesh* m, const int32_t* ids, const int
 // Domain
 // ---------------------------------------------------------------------------
 
-static void flatten(fem2d_domain* dom) {
+static void flatten(fem2d_domain* dom, int threads) {
   const Domain& d = dom->d;
   const Mesh& m = d.mesh;
   const size_t ne = m.elems.size();
@@ -386,53 +388,73 @@ static void flatten(fem2d_domain* dom) {
   dom->materials.resize(2 * ne);
   dom->orders.resize(2 * ne);
   dom->bs_offsets.assign(ne + 1, 0);
-  for (size_t e = 0; e < ne; ++e) {
-    const Elem& el = m.elems[e];
-    const Element& elt = m.elements[el.element];
-    dom->parent[e] = el.parent_id();
-    dom->loc[e] = el.ancestors.empty() ? 0 : uint8_t(el.ancestors.back().second);
-    dom->element_box[4 * e + 0] = elt.points[0].x;
-    dom->element_box[4 * e + 1] = elt.points[0].y;
-    dom->element_box[4 * e + 2] = elt.points[3].x;
-    dom->element_box[4 * e + 3] = elt.points[3].y;
-    dom->materials[2 * e + 0] = elt.materials.eps_re;
-    dom->materials[2 * e + 1] = elt.materials.mu_re;
-    dom->orders[2 * e + 0] = el.poly_orders.ni;
-    dom->orders[2 * e + 1] = el.poly_orders.nj;
-    dom->bs_offsets[e + 1] = dom->bs_offsets[e] + int64_t(d.basis_specs[e].size());
-  }
+  for (size_t e = 0; e < ne; ++e) dom->bs_offsets[e + 1] = dom->bs_offsets[e] + int64_t(d.basis_specs[e].size());
   const size_t nbs = size_t(dom->bs_offsets[ne]);
   dom->bs_dir.resize(nbs);
   dom->bs_i.resize(nbs);
   dom->bs_j.resize(nbs);
   dom->bs_dof.resize(nbs);
-  size_t t = 0;
-  for (size_t e = 0; e < ne; ++e)
-    for (const BasisSpec& bs : d.basis_specs[e]) {
-      dom->bs_dir[t] = uint8_t(bs.dir);
-      dom->bs_i[t] = bs.i;
-      dom->bs_j[t] = bs.j;
-      dom->bs_dof[t] = bs.dof_id;
-      ++t;
+  parallel_ranges(ne, threads, 512, [&](size_t b, size_t en) {
+    for (size_t e = b; e < en; ++e) {
+      const Elem& el = m.elems[e];
+      const Element& elt = m.elements[el.element];
+      dom->parent[e] = el.parent_id();
+      dom->loc[e] = el.ancestors.empty() ? 0 : uint8_t(el.ancestors.back().second);
+      dom->element_box[4 * e + 0] = elt.points[0].x;
+      dom->element_box[4 * e + 1] = elt.points[0].y;
+      dom->element_box[4 * e + 2] = elt.points[3].x;
+      dom->element_box[4 * e + 3] = elt.points[3].y;
+      dom->materials[2 * e + 0] = elt.materials.eps_re;
+      dom->materials[2 * e + 1] = elt.materials.mu_re;
+      dom->orders[2 * e + 0] = el.poly_orders.ni;
+      dom->orders[2 * e + 1] = el.poly_orders.nj;
+      size_t t = size_t(dom->bs_offsets[e]);
+      for (const BasisSpec& bs : d.basis_specs[e]) {
+        dom->bs_dir[t] = uint8_t(bs.dir);
+        dom->bs_i[t] = bs.i;
+        dom->bs_j[t] = bs.j;
+        dom->bs_dof[t] = bs.dof_id;
+        ++t;
+      }
     }
+  });
   // each DoF's basis-spec addresses as flat spec indices
-  dom->dof_bs.assign(2 * d.dofs.size(), -1);
-  for (size_t f = 0; f < d.dofs.size(); ++f) {
-    const DoF& dof = d.dofs[f];
-    for (size_t k = 0; k < dof.basis_specs.size() && k < 2; ++k) {
-      const BSAddress& a = dof.basis_specs[k];
-      dom->dof_bs[2 * f + k] = int32_t(dom->bs_offsets[size_t(a.elem_id)] + a.elem_idx);
+  dom->dof_bs.resize(2 * d.dofs.size());
+  parallel_ranges(d.dofs.size(), threads, 8192, [&](size_t b, size_t e) {
+    for (size_t f = b; f < e; ++f) {
+      const DoF& dof = d.dofs[f];
+      for (size_t k = 0; k < 2; ++k) {
+        const BSAddress* a = k < dof.basis_specs.size() ? &dof.basis_specs[k] : nullptr;
+        dom->dof_bs[2 * f + k] = a ? int32_t(dom->bs_offsets[size_t(a->elem_id)] + a->elem_idx) : -1;
+      }
     }
-  }
+  });
+}
+
+int fem2d_domain_from_mesh_threads(const fem2d_mesh* m, int32_t cc, int32_t n_threads, fem2d_domain** out) {
+  FEM2D_REQUIRE(m && out, "null argument");
+  FEM2D_REQUIRE(cc >= 0 && cc <= 2, "unknown continuity condition");
+  FEM2D_REQUIRE(n_threads >= 0, "negative thread count");
+  FEM2D_GUARD({
+    const int threads = n_threads > 0 ? int(n_threads) : default_threads();
+    auto* dom = new fem2d_domain();
+    dom->d = Domain::from_mesh(m->m.clone(threads), ContinuityCondition(cc), threads);
+    flatten(dom, threads);
+    *out = dom;
+  })
 }
 
 int fem2d_domain_from_mesh(const fem2d_mesh* m, int32_t cc, fem2d_domain** out) {
+  return fem2d_domain_from_mesh_threads(m, cc, 0, out);
+}
+
+int fem2d_domain_from_mesh_serial(const fem2d_mesh* m, int32_t cc, fem2d_domain** out) {
   FEM2D_REQUIRE(m && out, "null argument");
   FEM2D_REQUIRE(cc >= 0 && cc <= 2, "unknown continuity condition");
   FEM2D_GUARD({
     auto* dom = new fem2d_domain();
-    dom->d = Domain::from_mesh(m->m, ContinuityCondition(cc));
-    flatten(dom);
+    dom->d = Domain::from_mesh_serial(m->m, ContinuityCondition(cc));
+    flatten(dom, 1);
     *out = dom;
   })
 }
@@ -443,7 +465,7 @@ int fem2d_domain_blank(int32_t cc, fem2d_domain** out) {
   FEM2D_GUARD({
     auto* dom = new fem2d_domain();
     dom->d = Domain::blank(ContinuityCondition(cc));
-    flatten(dom);
+    flatten(dom, 1);
     *out = dom;
   })
 }

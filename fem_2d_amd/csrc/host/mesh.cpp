@@ -1,5 +1,6 @@
 // Host-side mesh construction and hp-refinement. See mesh.hpp.
 #include "mesh.hpp"
+#include "parallel.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -822,15 +823,30 @@ std::vector<Elem> Mesh::upgrade_uninit_elems(std::vector<ElemUninit>& ne) {
   return out;
 }
 
-void Mesh::set_edge_activation() {
+void Mesh::set_edge_activation(int threads) {
+  if (threads <= 0) threads = default_threads();
   for (auto& e : edges) e.reset_activation();
   std::vector<int> base;
   for (const auto& e : edges)
     if (e.parent < 0 && !e.boundary) base.push_back(e.id);
-  for (int id : base)
-    if (!rec_set_edge_activation(id))
-      panic("Unable to find active Edge pair over Edge " + std::to_string(id) +
-            "; Something must be wrong with the mesh!");
+  // each base Edge's tree holds Edges of its own, so the trees are independent;
+  // the first failing base Edge in id order is the one reported
+  parallel_ranges(base.size(), threads, 256, [&](size_t b, size_t e) {
+    for (size_t k = b; k < e; ++k)
+      if (!rec_set_edge_activation(base[k]))
+        panic("Unable to find active Edge pair over Edge " + std::to_string(base[k]) +
+              "; Something must be wrong with the mesh!");
+  });
+}
+
+Mesh Mesh::clone(int threads) const {
+  if (threads <= 0) threads = default_threads();
+  Mesh m;
+  m.elements = elements;
+  parallel_copy(m.elems, elems, threads);
+  parallel_copy(m.nodes, nodes, threads);
+  parallel_copy(m.edges, edges, threads);
+  return m;
 }
 
 bool Mesh::rec_set_edge_activation(int edge_id) {

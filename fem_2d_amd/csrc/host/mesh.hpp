@@ -255,7 +255,11 @@ class Mesh {
   void set_expansion_on_elems(const std::vector<int>& ids, const Orders& o);
   void set_expansion_orders(const std::vector<std::pair<int, Orders>>& orders);
 
-  void set_edge_activation();  // mesh.rs:1180-1215
+  // mesh.rs:1180-1215; the Edge trees under distinct base Edges are set on up
+  // to `threads` threads (0: the machine's cores, at most 16)
+  void set_edge_activation(int threads = 1);
+  // a deep copy made on up to `threads` threads
+  Mesh clone(int threads) const;
 
  private:
   struct ElemUninit;

@@ -61,9 +61,19 @@ class Domain:
             pass
 
     @classmethod
-    def from_mesh(cls, mesh: Mesh, cc: ContinuityCondition = ContinuityCondition.HCurl) -> "Domain":
+    def from_mesh(cls, mesh: Mesh, cc: ContinuityCondition = ContinuityCondition.HCurl,
+                  n_threads: int = 0) -> "Domain":
+        """Domain::from_mesh (domain.rs:69-159), built on ``n_threads`` host threads (0: the
+        machine's cores, at most 16)."""
         h = C.c_void_p()
-        check(N.load().fem2d_domain_from_mesh(mesh._h, int(cc), C.byref(h)))
+        check(N.load().fem2d_domain_from_mesh_threads(mesh._h, int(cc), int(n_threads), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_mesh_serial(cls, mesh: Mesh, cc: ContinuityCondition = ContinuityCondition.HCurl) -> "Domain":
+        """Domain::from_mesh in the reference's sequential order (the same Domain as from_mesh)."""
+        h = C.c_void_p()
+        check(N.load().fem2d_domain_from_mesh_serial(mesh._h, int(cc), C.byref(h)))
         return cls(h)
 
     @classmethod

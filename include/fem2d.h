@@ -290,8 +290,14 @@ int fem2d_mesh_set_global_expansion_orders(fem2d_mesh* m, int32_t i, int32_t j);
 int fem2d_mesh_set_expansion_orders(fem2d_mesh* m, const int32_t* ids, const int32_t* is, const int32_t* js,
                                     int64_t n);
 
-/* Domain::from_mesh (domain.rs:69-159); the mesh is copied */
+/* Domain::from_mesh (domain.rs:69-159); the mesh is copied. Built data-parallel
+   on the host (up to 16 threads); the same Domain, bit for bit, as the
+   sequential fem2d_domain_from_mesh_serial. */
 int fem2d_domain_from_mesh(const fem2d_mesh* m, int32_t cc, fem2d_domain** out);
+/* fem2d_domain_from_mesh on n_threads host threads (0: the machine's cores, at most 16) */
+int fem2d_domain_from_mesh_threads(const fem2d_mesh* m, int32_t cc, int32_t n_threads, fem2d_domain** out);
+/* Domain::from_mesh in the reference's sequential order (domain.rs:69-159) */
+int fem2d_domain_from_mesh_serial(const fem2d_mesh* m, int32_t cc, fem2d_domain** out);
 int fem2d_domain_blank(int32_t cc, fem2d_domain** out);
 void fem2d_domain_free(fem2d_domain* d);
 int fem2d_domain_num_dofs(const fem2d_domain* d, int64_t* n);
