@@ -2,16 +2,18 @@
 
 The prelude mirrors the reference crate's (src/lib.rs:14-39) names for the
 assembly path: Mesh / HRef / PRef / Orders build the input, Domain numbers the
-DoFs, and galerkin_sample_gep_hcurl assembles A and B with HIP kernels on the
-GPU.
+DoFs, galerkin_sample_gep_hcurl assembles A and B with HIP kernels on the
+GPU, and nalgebra_solve_gep / solve_gep_dense solve the GEP there.
 """
 
 from .domain import BasisDir, BasisSpec, ContinuityCondition, Domain
-from .errors import (EmptyDOFSet, Fem2dError, GalerkinSamplingError, HRefError, InvalidGLQSettings, MeshAccessError,
-                     MeshPanic, NativeError, PRefError, WrongContinuityCondition)
+from .errors import (EmptyDOFSet, FailedToConverge, FailedToInvertB, Fem2dError, GalerkinSamplingError, HRefError,
+                     InvalidGLQSettings, MeshAccessError, MeshPanic, NalgebraGEPError, NativeError, PRefError,
+                     ProblemTooLarge, SpuriouslyConverged, WrongContinuityCondition)
 from .galerkin import (FP32, FP64, GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace,
                        HierCurlIntegral, HierPoly, KOLShapeFn, L2Inner, ShapeFn, default_context,
                        galerkin_sample_gep_hcurl, gauss_quadrature_points)
+from .linalg import EigenPair, nalgebra_solve_gep, solve_gep_dense
 from .mesh import (MAX_POLYNOMIAL_ORDER, MIN_EDGE_LENGTH, Edge, Elem, HLevels, HRef, Mesh, Node, Orders, PolyOrders,
                    PRef)
 
@@ -21,5 +23,7 @@ __all__ = [
     "WrongContinuityCondition", "FP32", "FP64", "GEP", "MIN_GLQ_ORDER", "Context", "CurlCurl", "DeviceDomain", "HierCurlBasisFnSpace",
     "HierCurlIntegral", "HierPoly", "KOLShapeFn", "L2Inner", "ShapeFn", "default_context",
     "galerkin_sample_gep_hcurl", "gauss_quadrature_points", "MAX_POLYNOMIAL_ORDER", "MIN_EDGE_LENGTH", "Edge", "Elem",
-    "HLevels", "HRef", "Mesh", "Node", "Orders", "PolyOrders", "PRef",
+    "HLevels", "HRef", "Mesh", "Node", "Orders", "PolyOrders", "PRef", "EigenPair", "nalgebra_solve_gep",
+    "solve_gep_dense", "NalgebraGEPError", "FailedToInvertB", "SpuriouslyConverged", "ProblemTooLarge",
+    "FailedToConverge",
 ]

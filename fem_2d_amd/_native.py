@@ -17,6 +17,10 @@ ERR_PREF = 11
 ERR_MESH_ACCESS = 12
 ERR_PANIC = 13
 ERR_IO = 14
+ERR_FAILED_TO_INVERT_B = 20
+ERR_SPURIOUSLY_CONVERGED = 21
+ERR_PROBLEM_TOO_LARGE = 22
+ERR_NOT_CONVERGED = 23
 ERR_INVALID_ARG = 100
 ERR_INVALID_VIEW = 101
 ERR_UNSUPPORTED = 102
@@ -64,6 +68,14 @@ class GepDevice(C.Structure):
         ("a_val32", C.POINTER(C.c_float)),
         ("b_val32", C.POINTER(C.c_float)),
     ]
+
+
+class EigenPair(C.Structure):
+    _fields_ = [("value", C.c_double), ("n", C.c_int64), ("vector", C.POINTER(C.c_double))]
+
+
+NALGEBRA_MAX_DENSE_SIZE = 1000  # FEM2D_NALGEBRA_MAX_DENSE_SIZE (nalgebra_solve.rs:6)
+DENSE_GEP_MAX_SIZE = 32768  # FEM2D_DENSE_GEP_MAX_SIZE
 
 
 class Timing(C.Structure):
@@ -131,6 +143,11 @@ SIGNATURES = {
     "fem2d_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
     "fem2d_gep_device_print_to_petsc_binary_files": (C.c_int, [vp, P(GepDevice), C.c_char_p, C.c_char_p]),
     "fem2d_gep_csr_print_to_petsc_binary_files": (C.c_int, [vp, P(GepCsr), C.c_char_p, C.c_char_p]),
+    "fem2d_eigenpair_free": (None, [P(EigenPair)]),
+    "fem2d_nalgebra_solve_gep": (C.c_int, [vp, P(GepCsr), C.c_double, P(EigenPair)]),
+    "fem2d_gep_device_nalgebra_solve": (C.c_int, [vp, P(GepDevice), C.c_double, P(EigenPair)]),
+    "fem2d_solve_gep_dense": (C.c_int, [vp, P(GepCsr), C.c_double, P(EigenPair)]),
+    "fem2d_gep_device_solve_dense": (C.c_int, [vp, P(GepDevice), C.c_double, P(EigenPair)]),
     "fem2d_ctx_last_timing": (C.c_int, [vp, P(Timing)]),
     "fem2d_last_error": (C.c_char_p, []),
     "fem2d_last_error_detail": (C.c_int, []),

@@ -25,7 +25,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 HOST_SOURCES = ["host/mesh.cpp", "host/domain.cpp", "host/capi_host.cpp"]
-HIP_SOURCES = ["assemble.hip", "petsc.hip"]
+HIP_SOURCES = ["assemble.hip", "petsc.hip", "eigen.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra"]
 HIPFLAGS = [
@@ -75,7 +75,9 @@ def build_library(verbose: bool = False) -> str:
         _run([hipcc, *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, f"-L{rocm_lib}",
+          f"-Wl,-rpath,{rocm_lib}", "-lrocsolver", "-lrocblas"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -1780,9 +1780,11 @@ struct fem2d_ctx {
   bool use_graphs = true;
   hipGraphExec_t graph_exec[1] = {nullptr};  // the last recorded assembly
   GraphKey graph_key;
+  void* solver = nullptr;  // eigensolver handle, made by the first solve
 };
 
 int fem2d::capi::ctx_device(fem2d_ctx* ctx) { return ctx->device; }
+void** fem2d::capi::ctx_solver_slot(fem2d_ctx* ctx) { return &ctx->solver; }
 
 namespace {
 
@@ -1956,6 +1958,7 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->h_status) (void)hipHostFree(ctx->h_status);
+  fem2d::capi::release_solver(ctx->solver);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

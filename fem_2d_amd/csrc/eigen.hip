@@ -1,0 +1,423 @@
+// Dense generalized eigensolves of an assembled GEP, on the context's stream.
+//
+// fem2d_nalgebra_solve_gep replaces nalgebra_solve_gep (nalgebra_solve.rs:14-50)
+// with its arithmetic: B = L L^T, C = B^-1 A, then the symmetric eigenproblem
+// of C's lower triangle (C is not symmetric; nalgebra's SymmetricEigen reads
+// only that triangle), the eigenvalue nearest the target and its unit vector.
+// fem2d_solve_gep_dense solves A x = lambda B x itself (symmetric-definite,
+// dsygvd), the problem slepc_solve_gep (slepc_solve.rs:14-68) hands to SLEPc.
+//
+// The upper-triangular CSR is expanded to full symmetric column-major matrices
+// in HBM by k_expand (one wavefront per row); the factorisations, the product
+// and the eigensolves are rocSOLVER / rocBLAS calls on the context's stream.
+// Only the n eigenvalues and the chosen vector come back to the host.
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fem2d.h"
+#include "host/capi_internal.hpp"
+
+namespace {
+
+using fem2d::capi::set_error;
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr double kSpurious = 1e-12;  // nalgebra_solve.rs:24
+
+__global__ void k_zero(double* __restrict__ p, long long count) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count; i += stride) p[i] = 0.0;
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+// Full symmetric A and B (column-major, leading dimension n) from the stored
+// upper triangle, one wavefront per row. An entry outside the upper triangle
+// is not written and sets *err.
+template <class V>
+__global__ void k_expand(long long n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                         const V* __restrict__ a, const V* __restrict__ b, double* __restrict__ A,
+                         double* __restrict__ B, int* __restrict__ err) {
+  const int lane = threadIdx.x % kWave;
+  const long long wave = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave;
+  const long long waves = (gridDim.x * (long long)blockDim.x) / kWave;
+  for (long long r = wave; r < n; r += waves) {
+    const int64_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+    for (int64_t k = k0 + lane; k < k1; k += kWave) {
+      const long long c = col[k];
+      if (c < r || c >= n) {
+        atomicOr(err, 1);
+        continue;
+      }
+      const double av = double(a[k]), bv = double(b[k]);
+      A[r + c * n] = av;
+      A[c + r * n] = av;
+      B[r + c * n] = bv;
+      B[c + r * n] = bv;
+    }
+  }
+}
+
+// M(j, i) = M(i, j) for i > j: the upper triangle from the lower one.
+__global__ void k_lower_to_upper(long long n, double* __restrict__ M) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n * n; idx += stride) {
+    const long long i = idx % n, j = idx / n;
+    if (i > j) M[j + i * n] = M[i + j * n];
+  }
+}
+
+unsigned grid_for(long long count) {
+  return unsigned(std::max<long long>(1, std::min<long long>((count + kBlock - 1) / kBlock, 1 << 16)));
+}
+
+struct DevMem {
+  std::vector<void*> ptrs;
+  ~DevMem() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 8);
+    if (e == hipSuccess) ptrs.push_back(q);
+    *p = static_cast<T*>(q);
+    return e;
+  }
+};
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e), 0);
+  return e == hipErrorOutOfMemory ? FEM2D_ERR_OOM : FEM2D_ERR_HIP;
+}
+
+int blas_fail(rocblas_status s, const char* what) {
+  set_error(std::string(what) + ": " + rocblas_status_to_string(s), 0);
+  return s == rocblas_status_memory_error ? FEM2D_ERR_OOM : FEM2D_ERR_HIP;
+}
+
+#define EIG_HIP(expr)                                 \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+#define EIG_BLAS(expr)                                               \
+  do {                                                               \
+    rocblas_status _s = (expr);                                      \
+    if (_s != rocblas_status_success) return blas_fail(_s, #expr);   \
+  } while (0)
+
+// The context's rocBLAS handle (made on first use, bound to the context's stream).
+int solver_handle(fem2d_ctx* ctx, hipStream_t st, rocblas_handle* out) {
+  void** slot = fem2d::capi::ctx_solver_slot(ctx);
+  if (!*slot) {
+    rocblas_handle h = nullptr;
+    EIG_BLAS(rocblas_create_handle(&h));
+    *slot = h;
+  }
+  *out = static_cast<rocblas_handle>(*slot);
+  EIG_BLAS(rocblas_set_stream(*out, st));
+  return FEM2D_OK;
+}
+
+int read_info(const rocblas_int* d_info, hipStream_t st, rocblas_int* h_info) {
+  EIG_HIP(hipMemcpyAsync(h_info, d_info, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  EIG_HIP(hipStreamSynchronize(st));
+  return FEM2D_OK;
+}
+
+// Dense A and B of a device GEP.
+int expand(const fem2d_gep_device* g, hipStream_t st, DevMem& m, double** A, double** B) {
+  const long long n = g->n;
+  const bool fp32 = g->precision == FEM2D_PRECISION_FP32;
+  if (g->precision != FEM2D_PRECISION_FP64 && !fp32) {
+    set_error("unknown precision", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  if (g->nnz < 0 || !g->row_ptr || (g->nnz > 0 && (!g->col || (fp32 ? !g->a_val32 || !g->b_val32
+                                                                      : !g->a_val || !g->b_val)))) {
+    set_error("GEP arrays missing", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  EIG_HIP(m.alloc(A, size_t(n * n)));
+  EIG_HIP(m.alloc(B, size_t(n * n)));
+  int* err = nullptr;
+  EIG_HIP(m.alloc(&err, 1));
+  k_zero<<<grid_for(n * n), kBlock, 0, st>>>(*A, n * n);
+  EIG_HIP(hipGetLastError());
+  k_zero<<<grid_for(n * n), kBlock, 0, st>>>(*B, n * n);
+  EIG_HIP(hipGetLastError());
+  k_set_int<<<1, kWave, 0, st>>>(err, 0);
+  EIG_HIP(hipGetLastError());
+  const unsigned row_blocks = unsigned(std::max<long long>(1, std::min<long long>((n + 3) / 4, 1 << 16)));
+  if (fp32)
+    k_expand<<<row_blocks, kBlock, 0, st>>>(n, g->row_ptr, g->col, g->a_val32, g->b_val32, *A, *B, err);
+  else
+    k_expand<<<row_blocks, kBlock, 0, st>>>(n, g->row_ptr, g->col, g->a_val, g->b_val, *A, *B, err);
+  EIG_HIP(hipGetLastError());
+  int h_err = 0;
+  EIG_HIP(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
+  EIG_HIP(hipStreamSynchronize(st));
+  if (h_err) {
+    set_error("GEP is not upper-triangular CSR with columns inside the matrix", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  return FEM2D_OK;
+}
+
+// The eigenpair whose value is nearest target (nalgebra_solve.rs:28-46): D
+// holds the n eigenvalues in ascending order, column k of V (on the device,
+// leading dimension n) the vector of D[k].
+int pick(const std::vector<double>& D, const double* V, long long n, double target, hipStream_t st,
+         fem2d_eigenpair* out) {
+  if (std::all_of(D.begin(), D.end(), [](double e) { return std::fabs(e) < kSpurious; })) {
+    set_error("Only spurious modes were found!", 0);
+    return FEM2D_ERR_SPURIOUSLY_CONVERGED;
+  }
+  size_t best = 0;
+  double delta_min = HUGE_VAL;
+  for (size_t k = 0; k < D.size(); ++k) {
+    const double delta = std::fabs(D[k] - target);
+    if (delta < delta_min) {
+      delta_min = delta;
+      best = k;
+    }
+  }
+  double* v = static_cast<double*>(std::malloc(size_t(n) * sizeof(double)));
+  if (!v) {
+    set_error("host allocation of the eigenvector failed", 0);
+    return FEM2D_ERR_OOM;
+  }
+  const hipError_t e1 = hipMemcpyAsync(v, V + best * size_t(n), size_t(n) * sizeof(double), hipMemcpyDeviceToHost, st);
+  const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(st) : e1;
+  if (e2 != hipSuccess) {
+    std::free(v);
+    return hip_fail(e2, "eigenvector download");
+  }
+  out->value = D[best];
+  out->n = n;
+  out->vector = v;
+  return FEM2D_OK;
+}
+
+int check_device_args(fem2d_ctx* ctx, const fem2d_gep_device* gep, fem2d_eigenpair* out) {
+  if (!ctx || !gep || !out) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  if (gep->n < 0) {
+    set_error("negative GEP size", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  out->value = 0.0;
+  out->n = 0;
+  out->vector = nullptr;
+  return FEM2D_OK;
+}
+
+int nalgebra_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem2d_eigenpair* out) {
+  if (int rc = check_device_args(ctx, gep, out)) return rc;
+  const long long n = gep->n;
+  if (n > FEM2D_NALGEBRA_MAX_DENSE_SIZE) {  // nalgebra_solve.rs:15-17
+    set_error("Matrices Exceeded Maximum Size (1000x1000); Cannot Solve!", 0);
+    return FEM2D_ERR_PROBLEM_TOO_LARGE;
+  }
+  if (n == 0) {  // no eigenvalues: all of none are spurious (nalgebra_solve.rs:24)
+    set_error("Only spurious modes were found!", 0);
+    return FEM2D_ERR_SPURIOUSLY_CONVERGED;
+  }
+  EIG_HIP(hipSetDevice(fem2d::capi::ctx_device(ctx)));
+  hipStream_t st = static_cast<hipStream_t>(fem2d_ctx_stream(ctx));
+  rocblas_handle h;
+  if (int rc = solver_handle(ctx, st, &h)) return rc;
+  DevMem m;
+  double *A, *B, *C, *D, *E;
+  if (int rc = expand(gep, st, m, &A, &B)) return rc;
+  EIG_HIP(m.alloc(&C, size_t(n * n)));
+  EIG_HIP(m.alloc(&D, size_t(n)));
+  EIG_HIP(m.alloc(&E, size_t(n)));
+  rocblas_int* d_info;
+  EIG_HIP(m.alloc(&d_info, 1));
+  rocblas_int info = 0;
+  const rocblas_int ni = rocblas_int(n);
+  // B = L L^T, then B^-1 from the factor (nalgebra_solve.rs:19-20)
+  EIG_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, ni, B, ni, d_info));
+  if (int rc = read_info(d_info, st, &info)) return rc;
+  if (info != 0) {
+    set_error("Failed to invert B-matrix (via cholesky); likely ill-conditioned!", int(info));
+    return FEM2D_ERR_FAILED_TO_INVERT_B;
+  }
+  EIG_BLAS(rocsolver_dpotri(h, rocblas_fill_lower, ni, B, ni, d_info));
+  if (int rc = read_info(d_info, st, &info)) return rc;
+  if (info != 0) {
+    set_error("Failed to invert B-matrix (via cholesky); likely ill-conditioned!", int(info));
+    return FEM2D_ERR_FAILED_TO_INVERT_B;
+  }
+  k_lower_to_upper<<<grid_for(n * n), kBlock, 0, st>>>(n, B);
+  EIG_HIP(hipGetLastError());
+  // C = B^-1 A (nalgebra_solve.rs:21)
+  const double one = 1.0, zero = 0.0;
+  EIG_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, ni, ni, &one, B, ni, A, ni, &zero, C,
+                         ni));
+  // SymmetricEigen::new(C) reads C's lower triangle (nalgebra_solve.rs:22)
+  EIG_BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, ni, C, ni, D, E, d_info));
+  if (int rc = read_info(d_info, st, &info)) return rc;
+  if (info != 0) {
+    set_error("the symmetric eigensolver did not converge", int(info));
+    return FEM2D_ERR_NOT_CONVERGED;
+  }
+  std::vector<double> hD(static_cast<size_t>(n));
+  EIG_HIP(hipMemcpyAsync(hD.data(), D, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, st));
+  EIG_HIP(hipStreamSynchronize(st));
+  return pick(hD, C, n, target, st, out);
+}
+
+int dense_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem2d_eigenpair* out) {
+  if (int rc = check_device_args(ctx, gep, out)) return rc;
+  const long long n = gep->n;
+  if (n > FEM2D_DENSE_GEP_MAX_SIZE) {
+    set_error("Matrices exceed the dense solver's maximum size (32768 DoFs)", 0);
+    return FEM2D_ERR_PROBLEM_TOO_LARGE;
+  }
+  if (n == 0) {
+    set_error("Only spurious modes were found!", 0);
+    return FEM2D_ERR_SPURIOUSLY_CONVERGED;
+  }
+  EIG_HIP(hipSetDevice(fem2d::capi::ctx_device(ctx)));
+  hipStream_t st = static_cast<hipStream_t>(fem2d_ctx_stream(ctx));
+  rocblas_handle h;
+  if (int rc = solver_handle(ctx, st, &h)) return rc;
+  DevMem m;
+  double *A, *B, *D, *E;
+  if (int rc = expand(gep, st, m, &A, &B)) return rc;
+  EIG_HIP(m.alloc(&D, size_t(n)));
+  EIG_HIP(m.alloc(&E, size_t(n)));
+  rocblas_int* d_info;
+  EIG_HIP(m.alloc(&d_info, 1));
+  rocblas_int info = 0;
+  const rocblas_int ni = rocblas_int(n);
+  EIG_BLAS(rocsolver_dsygvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, ni, A, ni, B, ni, D, E,
+                            d_info));
+  if (int rc = read_info(d_info, st, &info)) return rc;
+  if (info > ni) {
+    set_error("Failed to invert B-matrix (via cholesky); likely ill-conditioned!", int(info - ni));
+    return FEM2D_ERR_FAILED_TO_INVERT_B;
+  }
+  if (info != 0) {
+    set_error("the generalized eigensolver did not converge", int(info));
+    return FEM2D_ERR_NOT_CONVERGED;
+  }
+  std::vector<double> hD(static_cast<size_t>(n));
+  EIG_HIP(hipMemcpyAsync(hD.data(), D, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, st));
+  EIG_HIP(hipStreamSynchronize(st));
+  return pick(hD, A, n, target, st, out);
+}
+
+// A host CSR on the device (checked on the host before any device read).
+int upload(const fem2d_gep_csr* gep, DevMem& m, fem2d_gep_device* d) {
+  if (gep->n < 0 || gep->nnz < 0 || !gep->row_ptr || (gep->nnz > 0 && (!gep->col || !gep->a_val || !gep->b_val))) {
+    set_error("GEP sizes or arrays invalid", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  for (long long r = 0; r < gep->n; ++r)
+    if (gep->row_ptr[r + 1] < gep->row_ptr[r]) {
+      set_error("row_ptr decreases", 0);
+      return FEM2D_ERR_INVALID_ARG;
+    }
+  if (gep->row_ptr[0] != 0 || gep->row_ptr[gep->n] != gep->nnz) {
+    set_error("row_ptr does not span [0, nnz]", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  *d = fem2d_gep_device{};
+  d->n = gep->n;
+  d->nnz = gep->nnz;
+  d->precision = FEM2D_PRECISION_FP64;
+  EIG_HIP(m.alloc(&d->row_ptr, size_t(gep->n + 1)));
+  EIG_HIP(m.alloc(&d->col, size_t(gep->nnz)));
+  EIG_HIP(m.alloc(&d->a_val, size_t(gep->nnz)));
+  EIG_HIP(m.alloc(&d->b_val, size_t(gep->nnz)));
+  EIG_HIP(hipMemcpy(d->row_ptr, gep->row_ptr, size_t(gep->n + 1) * 8, hipMemcpyHostToDevice));
+  if (gep->nnz > 0) {
+    EIG_HIP(hipMemcpy(d->col, gep->col, size_t(gep->nnz) * 4, hipMemcpyHostToDevice));
+    EIG_HIP(hipMemcpy(d->a_val, gep->a_val, size_t(gep->nnz) * 8, hipMemcpyHostToDevice));
+    EIG_HIP(hipMemcpy(d->b_val, gep->b_val, size_t(gep->nnz) * 8, hipMemcpyHostToDevice));
+  }
+  return FEM2D_OK;
+}
+
+template <class Solve>
+int solve_host(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target, fem2d_eigenpair* out, Solve solve) {
+  if (!ctx || !gep || !out) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  EIG_HIP(hipSetDevice(fem2d::capi::ctx_device(ctx)));
+  DevMem m;
+  fem2d_gep_device d;
+  if (int rc = upload(gep, m, &d)) return rc;
+  return solve(ctx, &d, target, out);
+}
+
+}  // namespace
+
+void fem2d::capi::release_solver(void* handle) {
+  if (handle) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(handle));
+}
+
+extern "C" {
+
+void fem2d_eigenpair_free(fem2d_eigenpair* pair) {
+  if (!pair) return;
+  std::free(pair->vector);
+  pair->vector = nullptr;
+  pair->n = 0;
+}
+
+int fem2d_gep_device_nalgebra_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
+                                    fem2d_eigenpair* out) {
+  return nalgebra_solve(ctx, gep, target_eigenvalue, out);
+}
+
+int fem2d_nalgebra_solve_gep(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target_eigenvalue,
+                             fem2d_eigenpair* out) {
+  if (gep && gep->n > FEM2D_NALGEBRA_MAX_DENSE_SIZE && out) {  // before any upload, as the reference
+    out->value = 0.0;
+    out->n = 0;
+    out->vector = nullptr;
+    set_error("Matrices Exceeded Maximum Size (1000x1000); Cannot Solve!", 0);
+    return FEM2D_ERR_PROBLEM_TOO_LARGE;
+  }
+  return solve_host(ctx, gep, target_eigenvalue, out, nalgebra_solve);
+}
+
+int fem2d_gep_device_solve_dense(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
+                                 fem2d_eigenpair* out) {
+  return dense_solve(ctx, gep, target_eigenvalue, out);
+}
+
+int fem2d_solve_gep_dense(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target_eigenvalue, fem2d_eigenpair* out) {
+  if (gep && gep->n > FEM2D_DENSE_GEP_MAX_SIZE && out) {
+    out->value = 0.0;
+    out->n = 0;
+    out->vector = nullptr;
+    set_error("Matrices exceed the dense solver's maximum size (32768 DoFs)", 0);
+    return FEM2D_ERR_PROBLEM_TOO_LARGE;
+  }
+  return solve_host(ctx, gep, target_eigenvalue, out, dense_solve);
+}
+
+}  // extern "C"

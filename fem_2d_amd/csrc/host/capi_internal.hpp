@@ -16,6 +16,10 @@ void set_error(const std::string& msg, int detail);
 int translate_exception();
 // HIP device of a context
 int ctx_device(fem2d_ctx* ctx);
+// the context's eigensolver handle (null until the first solve)
+void** ctx_solver_slot(fem2d_ctx* ctx);
+// destroys a handle held in that slot
+void release_solver(void* handle);
 }  // namespace capi
 
 }  // namespace fem2d

@@ -55,6 +55,26 @@ class MeshPanic(Fem2dError):
     """A condition on which the reference panics (assert!, panic!, unwrap of an Err)."""
 
 
+class NalgebraGEPError(Fem2dError):
+    """NalgebraGEPError (nalgebra_solve.rs:52-58)."""
+
+
+class FailedToInvertB(NalgebraGEPError):
+    pass
+
+
+class SpuriouslyConverged(NalgebraGEPError):
+    pass
+
+
+class ProblemTooLarge(NalgebraGEPError):
+    pass
+
+
+class FailedToConverge(Fem2dError):
+    """The eigensolver's iteration did not converge (SlepcGEPError::FailedToConverge, slepc_solve.rs:70-81)."""
+
+
 class NativeError(Fem2dError):
     """Library-level failure: HIP runtime, allocation, invalid domain view, unsupported size."""
 
@@ -85,6 +105,14 @@ def check(rc: int) -> None:
         raise MeshPanic(msg)
     if rc == N.ERR_IO:
         raise FileNotFoundError(msg)
+    if rc == N.ERR_FAILED_TO_INVERT_B:
+        raise FailedToInvertB(msg)
+    if rc == N.ERR_SPURIOUSLY_CONVERGED:
+        raise SpuriouslyConverged(msg)
+    if rc == N.ERR_PROBLEM_TOO_LARGE:
+        raise ProblemTooLarge(msg)
+    if rc == N.ERR_NOT_CONVERGED:
+        raise FailedToConverge(msg)
     if rc == N.ERR_INVALID_ARG:
         raise ValueError(msg)
     raise NativeError(f"status {rc}: {msg}", rc)

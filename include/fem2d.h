@@ -41,6 +41,12 @@ enum fem2d_status {
   FEM2D_ERR_MESH_ACCESS = 12, /* MeshAccessError (mesh.rs:1799-1804) */
   FEM2D_ERR_PANIC = 13,       /* a condition on which the reference panics */
   FEM2D_ERR_IO = 14,          /* mesh file missing or unreadable */
+  /* eigensolver errors: NalgebraGEPError (nalgebra_solve.rs:52-58) and
+   * SlepcGEPError::FailedToConverge (slepc_solve.rs:70-81) */
+  FEM2D_ERR_FAILED_TO_INVERT_B = 20,   /* B is not positive definite (its Cholesky factorisation fails) */
+  FEM2D_ERR_SPURIOUSLY_CONVERGED = 21, /* every eigenvalue is below 1e-12 in magnitude */
+  FEM2D_ERR_PROBLEM_TOO_LARGE = 22,    /* more DoFs than the solver takes */
+  FEM2D_ERR_NOT_CONVERGED = 23,        /* the eigensolver's iteration did not converge */
   /* library errors */
   FEM2D_ERR_INVALID_ARG = 100,
   FEM2D_ERR_INVALID_VIEW = 101, /* domain view violates the Domain invariants */
@@ -188,6 +194,46 @@ int fem2d_gep_device_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep
                                                  const char* prefix);
 int fem2d_gep_csr_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep_csr* gep, const char* dir,
                                               const char* prefix);
+
+/* ---------------------------------------------------------------------------
+ * Dense generalized eigensolves of an assembled GEP (rocSOLVER on the
+ * context's stream; the matrices are expanded to dense column-major storage
+ * in HBM by the library's own kernel)
+ * ------------------------------------------------------------------------- */
+
+/* EigenPair (linalg.rs:84-97) */
+typedef struct fem2d_eigenpair {
+  double value;
+  int64_t n;
+  double* vector; /* n values in host memory owned by the library; fem2d_eigenpair_free */
+} fem2d_eigenpair;
+void fem2d_eigenpair_free(fem2d_eigenpair* pair);
+
+/* nalgebra_solve_gep (nalgebra_solve.rs:14-50), its arithmetic kept: at most
+ * FEM2D_NALGEBRA_MAX_DENSE_SIZE DoFs (FEM2D_ERR_PROBLEM_TOO_LARGE above);
+ * B = L L^T (FEM2D_ERR_FAILED_TO_INVERT_B if that fails); C = B^-1 A; the
+ * symmetric eigenproblem of C's lower triangle (C itself is not symmetric,
+ * and this is what the reference's SymmetricEigen reads); FEM2D_ERR_SPURIOUSLY_CONVERGED
+ * if every eigenvalue is below 1e-12 in magnitude; else the eigenpair whose
+ * value is nearest target_eigenvalue (the lowest such value on a tie), with a
+ * unit-2-norm vector. */
+#define FEM2D_NALGEBRA_MAX_DENSE_SIZE 1000
+int fem2d_nalgebra_solve_gep(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target_eigenvalue,
+                             fem2d_eigenpair* out);
+int fem2d_gep_device_nalgebra_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
+                                    fem2d_eigenpair* out);
+
+/* The symmetric-definite problem A x = lambda B x itself (rocSOLVER dsygvd),
+ * standing in for slepc_solve_gep (slepc_solve.rs:14-68) up to
+ * FEM2D_DENSE_GEP_MAX_SIZE DoFs: the eigenpair whose value is nearest
+ * target_eigenvalue (the lowest on a tie), the vector B-normalised
+ * (x^T B x = 1). Errors: FEM2D_ERR_PROBLEM_TOO_LARGE,
+ * FEM2D_ERR_FAILED_TO_INVERT_B, FEM2D_ERR_SPURIOUSLY_CONVERGED (as above),
+ * FEM2D_ERR_NOT_CONVERGED. */
+#define FEM2D_DENSE_GEP_MAX_SIZE 32768
+int fem2d_solve_gep_dense(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target_eigenvalue, fem2d_eigenpair* out);
+int fem2d_gep_device_solve_dense(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
+                                 fem2d_eigenpair* out);
 
 /* Per-stage device time of the last assembly on this context, measured with
  * HIP events recorded on the context's stream (milliseconds). An assembly that

@@ -1,0 +1,168 @@
+"""GPU eigensolves of assembled GEPs (fem2d_nalgebra_solve_gep, fem2d_solve_gep_dense and their
+device-resident variants) against the oracle's restatements (oracle/eigen.py) and the reference's
+known answers:
+
+* nalgebra_solve_gep on the 60-DoF problem of lib.rs:45-80 gives 2.6479657 (tolerance 1e-6), the
+  value of the lower triangle of B^-1 A; eigenvalue within EIG_REL of the oracle's, vector
+  parallel to the oracle's within VEC_TOL (unit 2-norm);
+* the symmetric-definite solve on the 624-DoF problem of lib.rs:82-119 gives SLEPc's
+  1.4745880937 (tolerance 1e-9) with the vector of test_evec.dat;
+* the 1096-DoF waveguide exceeds nalgebra's 1000-DoF cap (ProblemTooLarge); its dense solve gives
+  the oracle's TE10 eigenvalue (analytic pi^2 within the discretisation error);
+* the 4496-DoF precision-study matrices (beyond what the oracle's dense solve does in seconds):
+  TE10 within 1e-6 of pi^2, residual and B-normalisation;
+* FP32-stored matrices, error variants, and an invalid CSR."""
+
+import numpy as np
+import pytest
+
+import cases
+import fem_2d_amd as F
+import oracle
+from conftest import golden
+from fem_2d_amd import workloads as W
+from oracle import eigen as OE
+
+pytestmark = pytest.mark.gpu
+
+EIG_REL = 1e-9  # eigenvalue against the oracle (cond(B) ~ 1e5 amplifies rounding in B^-1 A)
+VEC_TOL = 1e-9  # 1 - |cos| between GPU and oracle vectors (eigenvalue gaps > 0.7 here)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return F.Context(0)
+
+
+def _dense(dom, glq):
+    rc, ref = oracle.assemble(dom.view(), basis=0, glq=glq, n_threads=16)
+    assert rc == 0
+    return ref.to_dense()
+
+
+def _cos(u, v):
+    return abs(u @ v) / (np.linalg.norm(u) * np.linalg.norm(v))
+
+
+def _golden_vector():
+    import struct
+    raw = open(golden("test_evec.dat"), "rb").read()
+    _, n = struct.unpack(">ii", raw[:8])
+    return np.array(struct.unpack(f">{n}d", raw[8:8 + 8 * n]))
+
+
+def test_nalgebra_solve_reference_value(ctx):
+    """lib.rs:45-80 on the host GEP and on the device-resident one."""
+    print("[eigen] nalgebra nalg_problem", flush=True)
+    dom, glq = cases.nalg_problem()
+    gep = F.galerkin_sample_gep_hcurl(dom, glq, ctx=ctx)
+    pair = F.nalgebra_solve_gep(gep, 2.64, ctx=ctx)
+    assert abs(pair.value - 2.6479657) < 1e-6
+    assert pair.vector.size == dom.num_dofs == 60
+    assert abs(np.linalg.norm(pair.vector) - 1.0) < 1e-12
+    A, B = _dense(dom, glq)
+    lam, v = OE.nalgebra_solve_gep(A, B, 2.64)
+    assert abs(pair.value - lam) <= EIG_REL * lam
+    assert 1 - _cos(pair.vector, v) < VEC_TOL
+    dd = F.DeviceDomain(dom, ctx)
+    dev = dd.assemble(glq)
+    pair_d = F.nalgebra_solve_gep(dev, 2.64, ctx=ctx)
+    assert abs(pair_d.value - pair.value) <= 1e-12 * lam
+    assert 1 - _cos(pair_d.vector, pair.vector) < 1e-12
+
+
+def test_nalgebra_solve_matches_oracle_on_624_dofs(ctx):
+    print("[eigen] nalgebra slepc_problem", flush=True)
+    dom, glq = cases.slepc_problem()
+    gep = F.galerkin_sample_gep_hcurl(dom, glq, ctx=ctx)
+    A, B = _dense(dom, glq)
+    for target in (1.475, 3.5, 20.0):
+        pair = F.nalgebra_solve_gep(gep, target, ctx=ctx)
+        lam, v = OE.nalgebra_solve_gep(A, B, target)
+        assert abs(pair.value - lam) <= EIG_REL * abs(lam), (target, pair.value, lam)
+        assert 1 - _cos(pair.vector, v) < 1e-8, target
+
+
+def test_dense_solve_slepc_reference_pair(ctx):
+    """lib.rs:82-119: 1.4745880937 (tolerance 1e-9) and the vector of test_evec.dat."""
+    print("[eigen] dense slepc_problem", flush=True)
+    dom, glq = cases.slepc_problem()
+    gep = F.galerkin_sample_gep_hcurl(dom, glq, ctx=ctx)
+    pair = F.solve_gep_dense(gep, 1.475, ctx=ctx)
+    assert abs(pair.value - 1.4745880937) < 1e-9
+    assert 1 - _cos(pair.vector, _golden_vector()) < 1e-9
+    A, B = gep.to_dense()
+    assert abs(pair.vector @ B @ pair.vector - 1.0) < 1e-10
+    dev = F.DeviceDomain(dom, ctx).assemble(glq)
+    pair_d = F.solve_gep_dense(dev, 1.475, ctx=ctx)
+    assert abs(pair_d.value - pair.value) < 1e-12
+    assert len(pair.normalized_eigenvector()) == 624
+
+
+def test_waveguide_cap_and_dense_te10(ctx):
+    print("[eigen] waveguide_small", flush=True)
+    w = W.waveguide_small()
+    gep = F.galerkin_sample_gep_hcurl(w.domain, w.glq, ctx=ctx)
+    assert gep.n == 1096
+    with pytest.raises(F.ProblemTooLarge):
+        F.nalgebra_solve_gep(gep, 10.0, ctx=ctx)
+    dd = F.DeviceDomain(w.domain, ctx)
+    with pytest.raises(F.ProblemTooLarge):
+        F.nalgebra_solve_gep(dd.assemble(w.glq), 10.0, ctx=ctx)
+    pair = F.solve_gep_dense(gep, 10.0, ctx=ctx)
+    A, B = _dense(w.domain, w.glq)
+    lam, v = OE.dense_solve_gep(A, B, 10.0)
+    assert abs(pair.value - lam) <= EIG_REL * lam
+    assert abs(pair.value - np.pi ** 2) / np.pi ** 2 < 1e-6
+    assert 1 - _cos(pair.vector, v) < VEC_TOL
+    # FP32-stored matrices: the same mode within the storage rounding
+    p32 = F.solve_gep_dense(dd.assemble(w.glq, precision=F.FP32), 10.0, ctx=ctx)
+    assert abs(p32.value - pair.value) / pair.value < 1e-6
+
+
+def test_dense_solve_4496_dofs(ctx):
+    """Precision-study matrices (orders 8, 12x12 Gauss): TE10 = pi^2, residual, B-normalisation."""
+    print("[eigen] dense precision_study", flush=True)
+    w = W.precision_study()
+    dd = F.DeviceDomain(w.domain, ctx)
+    dev = dd.assemble(w.glq)
+    pair = F.solve_gep_dense(dev, 10.0, ctx=ctx)
+    assert pair.vector.size == 4496
+    assert abs(pair.value - np.pi ** 2) / np.pi ** 2 < 1e-6
+    A, B = dd.download(dev).to_scipy()
+    x = pair.vector
+    resid = np.linalg.norm(A @ x - pair.value * (B @ x)) / np.linalg.norm(A @ x)
+    assert resid < 1e-9, resid
+    assert abs(x @ (B @ x) - 1.0) < 1e-9
+
+
+def _gep(n, rows, cols, a, b):
+    rows = np.asarray(rows)
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(row_ptr, rows + 1, 1)
+    return F.GEP(n, np.cumsum(row_ptr), np.asarray(cols, dtype=np.int32), np.asarray(a, dtype=np.float64),
+                 np.asarray(b, dtype=np.float64))
+
+
+def test_closed_forms_and_error_variants(ctx):
+    print("[eigen] closed forms and errors", flush=True)
+    a = [4.0, 1.0, 9.0, 2.5]
+    b = [2.0, 0.5, 3.0, 5.0]
+    diag = _gep(4, range(4), range(4), a, b)
+    p = F.solve_gep_dense(diag, 2.9, ctx=ctx)
+    assert p.value == pytest.approx(3.0, rel=1e-14) and abs(p.vector[2]) == pytest.approx(3.0 ** -0.5, rel=1e-14)
+    p = F.nalgebra_solve_gep(diag, 0.4, ctx=ctx)
+    assert p.value == pytest.approx(0.5, rel=1e-14) and abs(p.vector[3]) == pytest.approx(1.0, rel=1e-14)
+    neg_b = _gep(3, range(3), range(3), [1.0, 1.0, 1.0], [-1.0, -1.0, -1.0])
+    with pytest.raises(F.FailedToInvertB):
+        F.nalgebra_solve_gep(neg_b, 1.0, ctx=ctx)
+    with pytest.raises(F.FailedToInvertB):
+        F.solve_gep_dense(neg_b, 1.0, ctx=ctx)
+    zero_a = _gep(3, range(3), range(3), [0.0, 0.0, 0.0], [1.0, 1.0, 1.0])
+    with pytest.raises(F.SpuriouslyConverged):
+        F.nalgebra_solve_gep(zero_a, 1.0, ctx=ctx)
+    with pytest.raises(F.SpuriouslyConverged):
+        F.solve_gep_dense(zero_a, 1.0, ctx=ctx)
+    lower = _gep(2, [1], [0], [1.0], [1.0])  # an entry below the diagonal
+    with pytest.raises(ValueError):
+        F.solve_gep_dense(lower, 1.0, ctx=ctx)

@@ -18,6 +18,7 @@ import scipy.linalg as sl
 
 import oracle
 import cases
+from oracle import eigen as OE
 from conftest import golden
 from exact_ref import compare_to_exact
 from fem_2d_amd import workloads as W
@@ -48,15 +49,60 @@ def test_slepc_golden_eigenpair():
 
 def test_nalgebra_eigenvalue():
     """lib.rs:45-80: nalgebra_solve_gep (nalgebra_solve.rs:14-50) near 2.64 gives 2.6479657 (tol 1e-6).
-    B^-1 A is not symmetric; nalgebra's SymmetricEigen reads its lower triangle."""
+    B^-1 A is not symmetric; nalgebra's SymmetricEigen reads its lower triangle. The oracle's
+    restatement reproduces the reference's value; the symmetric-definite problem does not (its
+    eigenvalue nearest 2.64 is 3.618), so the lower-triangle reading is what the value pins."""
     dom, glq = cases.nalg_problem()
     rc, res = oracle.assemble(dom.view(), glq=glq)
     assert rc == 0
     A, B = res.to_dense()
-    C = np.linalg.inv(np.linalg.cholesky(B)).T @ np.linalg.inv(np.linalg.cholesky(B)) @ A  # B^-1 A
-    lower = np.tril(C) + np.tril(C, -1).T
-    w = np.linalg.eigvalsh(lower)
-    assert abs(w[np.argmin(np.abs(w - 2.64))] - 2.6479657) < 1e-6
+    lam, v = OE.nalgebra_solve_gep(A, B, 2.64)
+    assert abs(lam - 2.6479657) < 1e-6
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-12
+    lam_true, _ = OE.dense_solve_gep(A, B, 2.64)
+    assert abs(lam_true - 2.6479657) > 0.5
+
+
+def test_oracle_dense_solve_slepc_pair():
+    """lib.rs:82-119: the symmetric-definite eigenpair near 1.475 is SLEPc's (1.4745880937, tol 1e-9),
+    its vector parallel to test_evec.dat and B-normalised."""
+    dom, glq = cases.slepc_problem()
+    lam_ref, v_ref = _golden_eigenpair()
+    rc, res = oracle.assemble(dom.view(), glq=glq)
+    assert rc == 0
+    A, B = res.to_dense()
+    lam, v = OE.dense_solve_gep(A, B, 1.475)
+    assert abs(lam - 1.4745880937) < 1e-9 and abs(lam - lam_ref) < 1e-9
+    cos = abs(v @ v_ref) / (np.linalg.norm(v) * np.linalg.norm(v_ref))
+    assert cos > 1 - 1e-9, cos
+    assert abs(v @ B @ v - 1.0) < 1e-10
+    with pytest.raises(OE.ProblemTooLarge):  # 624 DoFs fit; the nalgebra cap is 1000
+        OE.nalgebra_solve_gep(np.eye(1001), np.eye(1001), 1.0)
+
+
+def test_oracle_eigensolves_on_closed_forms():
+    """Diagonal problems: the eigenvalues are a_ii / b_ii; with B = I the lower-triangle reading of a
+    symmetric A is A itself; the error variants of nalgebra_solve.rs:15-26."""
+    a = np.array([4.0, 1.0, 9.0, 2.5])
+    b = np.array([2.0, 0.5, 3.0, 5.0])
+    lam, v = OE.dense_solve_gep(np.diag(a), np.diag(b), 2.9)
+    assert lam == pytest.approx(3.0, rel=1e-14) and abs(v[2]) == pytest.approx(1 / np.sqrt(3.0), rel=1e-14)
+    lam, v = OE.nalgebra_solve_gep(np.diag(a), np.diag(b), 0.4)
+    assert lam == pytest.approx(0.5, rel=1e-14) and abs(v[3]) == pytest.approx(1.0, rel=1e-14)
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((6, 6))
+    S = M + M.T
+    lam, _ = OE.nalgebra_solve_gep(S, np.eye(6), 0.0)
+    w = np.linalg.eigvalsh(S)
+    assert lam == pytest.approx(w[np.argmin(np.abs(w))], rel=1e-13)
+    with pytest.raises(OE.FailedToInvertB):
+        OE.nalgebra_solve_gep(np.eye(3), -np.eye(3), 1.0)
+    with pytest.raises(OE.FailedToInvertB):
+        OE.dense_solve_gep(np.eye(3), -np.eye(3), 1.0)
+    with pytest.raises(OE.SpuriouslyConverged):
+        OE.nalgebra_solve_gep(np.zeros((3, 3)), np.eye(3), 1.0)
+    with pytest.raises(OE.SpuriouslyConverged):
+        OE.dense_solve_gep(np.zeros((3, 3)), np.eye(3), 1.0)
 
 
 @pytest.mark.parametrize("name", ["nalg_problem", "deep_unit", "mixed_refinements", "readme_example"])
