@@ -18,7 +18,9 @@ stream around the kernel in the same number of kernel-by-kernel steps;
 `traffic` is the HBM bytes per launch from the committed rocprofv3 counter
 summary of that kernel. `host_boundary` times the drop-in entry point with
 host arrays on both sides. `domain_build` times the host Domain construction
-from the workload's Mesh, data-parallel and sequential. `cpu_baseline` times the oracle (the C++
+from the workload's Mesh, data-parallel and sequential. `eigensolve` times the
+dense GPU eigensolves of assembled GEPs (with LAPACK on the host beside them).
+`cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
 workload on the host cores. `mixed_precision` repeats the measurement with the values
 stored as FP32 (FEM2D_PRECISION_FP32: FP64 arithmetic, 12 B per stored entry) and states its
@@ -219,6 +221,58 @@ def domain_build(w, calls: int = 3):
     return out
 
 
+def eigensolve(calls: int = 3):
+    """Dense eigensolves after the assembly, device-resident GEP in, eigenpair out (rocSOLVER after
+    the library's CSR-to-dense kernel): the symmetric-definite solve near the TE10 cutoff on the
+    1096-DoF waveguide and the 4496-DoF precision-study matrices, nalgebra_solve_gep on the reference's
+    624-DoF problem; the CPU baseline is LAPACK dsygvd (scipy, the oracle's dense solve) on the
+    1096-DoF matrices."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    import fem_2d_amd as F
+    from fem_2d_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+
+    ctx = F.Context(0)
+    out = {"solver": "rocSOLVER dsygvd / dpotrf+dpotri+dgemm+dsyevd, FP64"}
+
+    def timed(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            r = fn()
+        return r, (time.perf_counter() - t0) * 1e3 / calls
+
+    for name, w in (("waveguide_1096", W.waveguide_small()), ("precision_study_4496", W.precision_study())):
+        dd = F.DeviceDomain(w.domain, ctx)
+        dev = dd.assemble(w.glq)
+        pair, ms = timed(lambda: F.solve_gep_dense(dev, 10.0, ctx=ctx))
+        out[name] = {"n": int(dev.n), "ms": ms, "eigenvalue": pair.value,
+                     "rel_err_vs_te10": abs(pair.value - np.pi ** 2) / np.pi ** 2}
+        if name == "waveguide_1096":
+            from oracle import eigen as OE
+
+            A, B = dd.download(dev).to_dense()
+            with threadpool_limits(16):
+                t0 = time.perf_counter()
+                lam, _ = OE.dense_solve_gep(A, B, 10.0)
+                cpu_ms = (time.perf_counter() - t0) * 1e3
+            out["cpu_baseline"] = {"ms": cpu_ms, "n": int(dev.n), "threads": min(16, os.cpu_count() or 1),
+                                   "kind": "LAPACK dsygvd (scipy)", "eigenvalue": lam,
+                                   "speedup": cpu_ms / ms}
+    dom, glq = cases.slepc_problem()
+    dev = F.DeviceDomain(dom, ctx).assemble(glq)
+    pair, ms = timed(lambda: F.nalgebra_solve_gep(dev, 2.64, ctx=ctx))
+    out["nalgebra_624"] = {"n": int(dev.n), "ms": ms, "eigenvalue": pair.value}
+    pair, ms = timed(lambda: F.solve_gep_dense(dev, 1.475, ctx=ctx))
+    out["slepc_problem_624"] = {"n": int(dev.n), "ms": ms, "eigenvalue": pair.value,
+                                "abs_err_vs_reference": abs(pair.value - 1.4745880937)}
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +283,7 @@ def main():
     ap.add_argument("--no-host-boundary", action="store_true",
                     help="skip timing the host-array entry point (upload + assembly + download)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-eigen", action="store_true", help="skip timing the dense eigensolves")
     ap.add_argument("--no-domain-build", action="store_true",
                     help="skip timing the host Domain construction (parallel and sequential builders)")
     ap.add_argument("--no-mixed", action="store_true",
@@ -363,6 +418,8 @@ def main():
         line["host_boundary"] = host_boundary(w, nnz)
     if not args.no_domain_build:
         line["domain_build"] = domain_build(w)
+    if not args.no_eigen:
+        line["eigensolve"] = eigensolve()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, pairs, nnz, args.cpu_seconds)
     print(json.dumps(line), flush=True)
