@@ -8,8 +8,10 @@ GPU, and nalgebra_solve_gep / solve_gep_dense solve the GEP there.
 
 from .domain import BasisDir, BasisSpec, ContinuityCondition, Domain
 from .errors import (EmptyDOFSet, FailedToConverge, FailedToInvertB, Fem2dError, GalerkinSamplingError, HRefError,
-                     InvalidGLQSettings, MeshAccessError, MeshPanic, NalgebraGEPError, NativeError, PRefError,
-                     ProblemTooLarge, SpuriouslyConverged, WrongContinuityCondition)
+                     InvalidGLQSettings, MeshAccessError, MeshPanic, MismatchedSolutionSize, MissingQuantity,
+                     NalgebraGEPError, NativeError, PRefError, ProblemTooLarge, SpuriouslyConverged,
+                     UniformFieldError, WrongContinuityCondition)
+from .fields import UniformFieldSpace, xy_fields_array
 from .galerkin import (FP32, FP64, GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace,
                        HierCurlIntegral, HierPoly, KOLShapeFn, L2Inner, ShapeFn, default_context,
                        galerkin_sample_gep_hcurl, gauss_quadrature_points)
@@ -25,5 +27,6 @@ __all__ = [
     "galerkin_sample_gep_hcurl", "gauss_quadrature_points", "MAX_POLYNOMIAL_ORDER", "MIN_EDGE_LENGTH", "Edge", "Elem",
     "HLevels", "HRef", "Mesh", "Node", "Orders", "PolyOrders", "PRef", "EigenPair", "nalgebra_solve_gep",
     "solve_gep_dense", "NalgebraGEPError", "FailedToInvertB", "SpuriouslyConverged", "ProblemTooLarge",
-    "FailedToConverge",
+    "FailedToConverge", "UniformFieldSpace", "xy_fields_array", "UniformFieldError", "MismatchedSolutionSize",
+    "MissingQuantity",
 ]

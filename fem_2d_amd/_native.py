@@ -21,6 +21,7 @@ ERR_FAILED_TO_INVERT_B = 20
 ERR_SPURIOUSLY_CONVERGED = 21
 ERR_PROBLEM_TOO_LARGE = 22
 ERR_NOT_CONVERGED = 23
+ERR_MISMATCHED_SOLUTION_SIZE = 24
 ERR_INVALID_ARG = 100
 ERR_INVALID_VIEW = 101
 ERR_UNSUPPORTED = 102
@@ -143,6 +144,14 @@ SIGNATURES = {
     "fem2d_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
     "fem2d_gep_device_print_to_petsc_binary_files": (C.c_int, [vp, P(GepDevice), C.c_char_p, C.c_char_p]),
     "fem2d_gep_csr_print_to_petsc_binary_files": (C.c_int, [vp, P(GepCsr), C.c_char_p, C.c_char_p]),
+    "fem2d_device_domain_num_leaves": (C.c_int, [vp, P(C.c_int64)]),
+    "fem2d_device_malloc": (C.c_int, [vp, C.c_int64, P(vp)]),
+    "fem2d_device_free": (C.c_int, [vp, vp]),
+    "fem2d_memcpy_to_device": (C.c_int, [vp, vp, vp, C.c_int64]),
+    "fem2d_memcpy_to_host": (C.c_int, [vp, vp, vp, C.c_int64]),
+    "fem2d_device_xy_fields": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int64, vp, vp]),
+    "fem2d_xy_fields": (C.c_int, [vp, P(DomainView), C.c_int, C.c_int, C.c_int, P(C.c_double), C.c_int64,
+                                  P(C.c_double), P(C.c_double)]),
     "fem2d_eigenpair_free": (None, [P(EigenPair)]),
     "fem2d_nalgebra_solve_gep": (C.c_int, [vp, P(GepCsr), C.c_double, P(EigenPair)]),
     "fem2d_gep_device_nalgebra_solve": (C.c_int, [vp, P(GepDevice), C.c_double, P(EigenPair)]),

@@ -1705,6 +1705,122 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
   }
 }
 
+// ---------------------------------------------------------------------------
+// Field evaluation: UniformFieldSpace::xy_fields (fields.rs:63-127)
+// ---------------------------------------------------------------------------
+
+// Halves [lo, hi] per axis down a chain of HRefLocs (h_refinement.rs:247-279),
+// locs[from - 1] first (the step nearest the top), locs[to] last.
+__device__ __forceinline__ void fold_range(const unsigned char* locs, int from, int to, double& u0, double& u1,
+                                           double& v0, double& v1) {
+  u0 = -1.0; u1 = 1.0; v0 = -1.0; v1 = 1.0;
+  for (int t = from - 1; t >= to; --t) {
+    const double mu = (u0 + u1) / 2.0, mv = (v0 + v1) / 2.0;
+    const int l = locs[t];
+    const bool split_u = l <= FEM2D_LOC_E;
+    const bool split_v = l <= FEM2D_LOC_NE || l >= FEM2D_LOC_S;
+    const bool hi_u = l == FEM2D_LOC_SE || l == FEM2D_LOC_NE || l == FEM2D_LOC_E;
+    const bool hi_v = l == FEM2D_LOC_NW || l == FEM2D_LOC_NE || l == FEM2D_LOC_N;
+    if (split_u) {
+      if (hi_u) u0 = mu; else u1 = mu;
+    }
+    if (split_v) {
+      if (hi_v) v0 = mv; else v1 = mv;
+    }
+  }
+}
+
+// One thread per grid point (leaf k, u point m, v point n) of every leaf:
+// the sum over the leaf and its ancestors (leaf first, as
+// Mesh::ancestor_elems lists them), over each one's basis specs in list
+// order, of f_u or f_v (basis.rs:225-233) at the point mapped into that
+// Elem's parametric space (defined_over with the leaf as descendant,
+// basis.rs:365-423), times the spec's DoF value. Every product is formed in
+// the reference's order: (jac_inv row * shape) * shape, then * solution, with
+// jac_inv = [[b, -0], [-0, a]] / (a b) (space.rs:138-147).
+template <int kBasis>
+__global__ void __launch_bounds__(kBlock) k_xy_fields(DevView v, const int* __restrict__ leaves, long long n_leaves,
+                                                      int d, const double* __restrict__ pts,
+                                                      const double* __restrict__ sol, double* __restrict__ fx,
+                                                      double* __restrict__ fy, int* __restrict__ err) {
+  const long long per_leaf = (long long)d * d;
+  const long long total = n_leaves * per_leaf;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int leaf = leaves[idx / per_leaf];
+    const int m = int((idx / d) % d), n = int(idx % d);
+    unsigned char locs[kMaxDepth];
+    int depth = 0;
+    bool ok = true;
+    for (int x = leaf; v.parent[x] >= 0;) {
+      const int p = v.parent[x];
+      if (p >= v.n_elems || p == x || depth >= kMaxDepth || v.loc[x] > FEM2D_LOC_N) {
+        ok = false;
+        break;
+      }
+      locs[depth++] = v.loc[x];
+      x = p;
+    }
+    double sx = 0.0, sy = 0.0;
+    if (!ok) {
+      set_err(err, kErrView);
+      fx[idx] = sx;
+      fy[idx] = sy;
+      continue;
+    }
+    int A = leaf;
+    for (int t = 0; t <= depth; ++t) {
+      // the Elem t steps above the leaf: its parametric range and Jacobian
+      // (element.rs:33-50) ...
+      double u0, u1, v0, v1;
+      fold_range(locs, depth, t, u0, u1, v0, v1);
+      const double* bx = v.box + 4 * A;
+      const double a = (map_range(u1, -1.0, 1.0, bx[0], bx[2]) - map_range(u0, -1.0, 1.0, bx[0], bx[2])) / 2.0;
+      const double b = (map_range(v1, -1.0, 1.0, bx[1], bx[3]) - map_range(v0, -1.0, 1.0, bx[1], bx[3])) / 2.0;
+      const double det = a * b - 0.0 * 0.0;
+      const double ju0 = b / det, jz = (-1.0 * 0.0) / det, jv1 = a / det;
+      // ... and the leaf's grid point in it (glq.rs:238-249 over the leaf's
+      // range relative to this Elem; the leaf itself takes the points as given)
+      double x = pts[m], y = pts[n];
+      if (t > 0) {
+        double r0, r1, q0, q1;
+        fold_range(locs, t, 0, r0, r1, q0, q1);
+        x = x * ((r1 - r0) / 2.0) + (r1 + r0) / 2.0;
+        y = y * ((q1 - q0) / 2.0) + (q1 + q0) / 2.0;
+      }
+      for (long long k = v.bs_off[A]; k < v.bs_off[A + 1]; ++k) {
+        const int dof = v.bs_dof[k];
+        if (dof < 0 || dof >= v.n_dofs) {
+          set_err(err, kErrView);
+          continue;
+        }
+        const int dir = v.bs_dir[k], i = v.bs_i[k], j = v.bs_j[k];
+        const double s = sol[dof];
+        double Nu, Ndu, Tu, Tdu, Nv, Ndv, Tv, Tdv;
+        if (kBasis == FEM2D_BASIS_KOL) {
+          kol(i, x, Nu, Ndu, Tu, Tdu);
+          kol(j, y, Nv, Ndv, Tv, Tdv);
+        } else {
+          hier_poly(i, x, Nu, Ndu, Tu, Tdu);
+          hier_poly(j, y, Nv, Ndv, Tv, Tdv);
+        }
+        if (dir == 0) {  // f_u = jac_inv.u * N_i(x) * T_j(y)
+          sx += ((ju0 * Nu) * Tv) * s;
+          sy += ((jz * Nu) * Tv) * s;
+        } else if (dir == 1) {  // f_v = jac_inv.v * T_i(x) * N_j(y)
+          sx += ((jz * Tu) * Nv) * s;
+          sy += ((jv1 * Tu) * Nv) * s;
+        } else {  // any other direction contributes the zero vector times the DoF value
+          sx += 0.0 * s;
+          sy += 0.0 * s;
+        }
+      }
+      A = v.parent[A];
+    }
+    fx[idx] = sx;
+    fy[idx] = sy;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -1759,6 +1875,8 @@ struct fem2d_device_domain {
   unsigned char* bs_j = nullptr;
   int* bs_dof = nullptr;
   int* dof_bs = nullptr;
+  int* leaves = nullptr;  // Elems without children, ascending (the field grids)
+  long long n_leaves = 0;
 };
 
 struct fem2d_ctx {
@@ -1829,6 +1947,13 @@ int slot(fem2d_ctx* ctx, int k, size_t count, T** out) {
   if (int(ctx->bufs.size()) <= k) ctx->bufs.resize(k + 1);
   if (int rc = ensure(ctx->bufs[k], count * sizeof(T))) return rc;
   *out = static_cast<T*>(ctx->bufs[k].p);
+  return FEM2D_OK;
+}
+
+template <class T>
+int alloc_device(size_t count, T** dst) {
+  *dst = nullptr;
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), std::max<size_t>(1, count) * sizeof(T)));
   return FEM2D_OK;
 }
 
@@ -2077,6 +2202,19 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
     fem2d_device_domain_free(dd);
     return rc;
   }
+  {
+    std::vector<char> has_child(size_t(ne), 0);
+    for (int e = 0; e < ne; ++e)
+      if (v->parent[e] >= 0 && v->parent[e] < ne) has_child[size_t(v->parent[e])] = 1;
+    std::vector<int> leaves;
+    for (int e = 0; e < ne; ++e)
+      if (!has_child[size_t(e)]) leaves.push_back(e);
+    dd->n_leaves = (long long)leaves.size();
+    if ((rc = upload(leaves.data(), leaves.size(), &dd->leaves))) {
+      fem2d_device_domain_free(dd);
+      return rc;
+    }
+  }
   *out = dd;
   return FEM2D_OK;
 }
@@ -2085,7 +2223,8 @@ void fem2d_device_domain_free(fem2d_device_domain* dd) {
   if (!dd) return;
   (void)hipSetDevice(dd->device);
   for (void* p : {(void*)dd->parent, (void*)dd->loc, (void*)dd->box, (void*)dd->mat, (void*)dd->bs_off,
-                  (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof, (void*)dd->dof_bs})
+                  (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof, (void*)dd->dof_bs,
+                  (void*)dd->leaves})
     if (p) (void)hipFree(p);
   delete dd;
 }
@@ -2383,6 +2522,178 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   out->a_val32 = fp32 ? static_cast<float*>(ctx->out_a.p) : nullptr;
   out->b_val32 = fp32 ? static_cast<float*>(ctx->out_b.p) : nullptr;
   return FEM2D_OK;
+}
+
+int fem2d_device_malloc(fem2d_ctx* ctx, int64_t bytes, void** out) {
+  if (!ctx || !out || bytes < 0) {
+    set_error("invalid argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMalloc(out, size_t(std::max<int64_t>(bytes, 1))));
+  return FEM2D_OK;
+}
+
+int fem2d_device_free(fem2d_ctx* ctx, void* p) {
+  if (!ctx) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (p) HIP_TRY(hipFree(p));
+  return FEM2D_OK;
+}
+
+int fem2d_memcpy_to_device(fem2d_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) {
+    set_error("invalid argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, size_t(bytes), hipMemcpyHostToDevice));
+  return FEM2D_OK;
+}
+
+int fem2d_memcpy_to_host(fem2d_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) {
+    set_error("invalid argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, size_t(bytes), hipMemcpyDeviceToHost));
+  return FEM2D_OK;
+}
+
+int fem2d_device_domain_num_leaves(const fem2d_device_domain* dd, int64_t* out) {
+  if (!dd || !out) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  *out = dd->n_leaves;
+  return FEM2D_OK;
+}
+
+int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int density_u, int density_v,
+                           const double* solution, int64_t n_solution, double* out_x, double* out_y) {
+  if (!ctx || !dd) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  if (basis != FEM2D_BASIS_HIERPOLY && basis != FEM2D_BASIS_KOL) {
+    set_error("unknown basis", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  if (density_u < 0 || density_v < 0 || density_u > 4096 || density_v > 4096) {
+    set_error("densities outside [0, 4096]", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  // fields.rs:68-72
+  if (n_solution != dd->n_dofs) {
+    set_error("Domain size (" + std::to_string(dd->n_dofs) + ") does not match solution size (" +
+                  std::to_string(n_solution) + ")!",
+              0);
+    return FEM2D_ERR_MISMATCHED_SOLUTION_SIZE;
+  }
+  // fields.rs:80-81, 106-107: the value grids are [densities[1]][densities[0]]
+  // and are indexed [m][n] with m < densities[0], n < densities[1], which
+  // leaves the grid unless the densities are equal (once any spec is visited)
+  if (density_u != density_v && density_u > 0 && density_v > 0 && dd->n_dofs > 0) {
+    set_error("index out of bounds: the field grid is indexed [u][v] but sized [v][u]", 0);
+    return FEM2D_ERR_PANIC;
+  }
+  const int d = density_u;
+  const long long total = dd->n_leaves * (long long)d * (d > 0 && density_v > 0 ? d : 0);
+  if (total == 0) return FEM2D_OK;
+  if (!solution || !out_x || !out_y) {
+    set_error("null array", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  // uniform_range(-1.0, 1.0, d) (fields.rs:402-405)
+  std::vector<double> pts(static_cast<size_t>(d));
+  const double step = (1.0 - -1.0) / double(d - 1);
+  for (int i = 0; i < d; ++i) pts[size_t(i)] = double(i) * step + -1.0;
+  double* d_pts = nullptr;
+  int* d_err = nullptr;
+  int rc = upload(pts.data(), pts.size(), &d_pts);
+  const int zero = 0;
+  if (!rc) rc = upload(&zero, 1, &d_err);
+  if (!rc) {
+    DevView v{};
+    v.n_elems = dd->n_elems;
+    v.n_dofs = dd->n_dofs;
+    v.n_bs = dd->n_bs;
+    v.parent = dd->parent;
+    v.loc = dd->loc;
+    v.box = dd->box;
+    v.mat = dd->mat;
+    v.bs_off = dd->bs_off;
+    v.bs_dir = dd->bs_dir;
+    v.bs_i = dd->bs_i;
+    v.bs_j = dd->bs_j;
+    v.bs_dof = dd->bs_dof;
+    v.dof_bs = dd->dof_bs;
+    const unsigned grid =
+        unsigned(std::min<long long>((total + kBlock - 1) / kBlock, std::max(1, ctx->n_cu) * 16LL));
+    if (basis == FEM2D_BASIS_KOL)
+      k_xy_fields<FEM2D_BASIS_KOL><<<grid, kBlock, 0, ctx->stream>>>(v, dd->leaves, dd->n_leaves, d, d_pts, solution,
+                                                                    out_x, out_y, d_err);
+    else
+      k_xy_fields<FEM2D_BASIS_HIERPOLY><<<grid, kBlock, 0, ctx->stream>>>(v, dd->leaves, dd->n_leaves, d, d_pts,
+                                                                         solution, out_x, out_y, d_err);
+    hipError_t e = hipGetLastError();
+    int h_err = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "field evaluation");
+    else if (h_err) {
+      set_error("domain view violates the Domain invariants (parent chain, location or DoF id)", 0);
+      rc = FEM2D_ERR_INVALID_VIEW;
+    }
+  }
+  if (d_pts) (void)hipFree(d_pts);
+  if (d_err) (void)hipFree(d_err);
+  return rc;
+}
+
+int fem2d_xy_fields(fem2d_ctx* ctx, const fem2d_domain_view* view, int basis, int density_u, int density_v,
+                    const double* solution, int64_t n_solution, double* out_x, double* out_y) {
+  if (!ctx || !view) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  if (n_solution != view->n_dofs) {
+    set_error("Domain size (" + std::to_string(view->n_dofs) + ") does not match solution size (" +
+                  std::to_string(n_solution) + ")!",
+              0);
+    return FEM2D_ERR_MISMATCHED_SOLUTION_SIZE;
+  }
+  fem2d_device_domain* dd = nullptr;
+  if (int rc = fem2d_device_domain_upload(ctx, view, &dd)) return rc;
+  const long long total =
+      dd->n_leaves * (long long)std::max(0, density_u) * (long long)std::max(0, density_v);
+  double *d_sol = nullptr, *d_x = nullptr, *d_y = nullptr;
+  int rc = FEM2D_OK;
+  if (density_u == density_v && total > 0) {
+    if (!solution || !out_x || !out_y) {
+      set_error("null array", 0);
+      rc = FEM2D_ERR_INVALID_ARG;
+    }
+    if (!rc) rc = upload(solution, size_t(n_solution), &d_sol);
+    if (!rc) rc = alloc_device(size_t(total), &d_x);
+    if (!rc) rc = alloc_device(size_t(total), &d_y);
+  }
+  if (!rc) rc = fem2d_device_xy_fields(ctx, dd, basis, density_u, density_v, d_sol, n_solution, d_x, d_y);
+  if (!rc && d_x) {
+    hipError_t e = hipMemcpy(out_x, d_x, size_t(total) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out_y, d_y, size_t(total) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "field download");
+  }
+  for (void* p : {(void*)d_sol, (void*)d_x, (void*)d_y})
+    if (p) (void)hipFree(p);
+  fem2d_device_domain_free(dd);
+  return rc;
 }
 
 int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d_gep_csr* out) {

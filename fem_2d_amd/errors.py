@@ -75,6 +75,18 @@ class FailedToConverge(Fem2dError):
     """The eigensolver's iteration did not converge (SlepcGEPError::FailedToConverge, slepc_solve.rs:70-81)."""
 
 
+class UniformFieldError(Fem2dError):
+    """UniformFieldError (fields.rs:407-411)."""
+
+
+class MismatchedSolutionSize(UniformFieldError):
+    pass
+
+
+class MissingQuantity(UniformFieldError):
+    pass
+
+
 class NativeError(Fem2dError):
     """Library-level failure: HIP runtime, allocation, invalid domain view, unsupported size."""
 
@@ -113,6 +125,8 @@ def check(rc: int) -> None:
         raise ProblemTooLarge(msg)
     if rc == N.ERR_NOT_CONVERGED:
         raise FailedToConverge(msg)
+    if rc == N.ERR_MISMATCHED_SOLUTION_SIZE:
+        raise MismatchedSolutionSize(msg)
     if rc == N.ERR_INVALID_ARG:
         raise ValueError(msg)
     raise NativeError(f"status {rc}: {msg}", rc)

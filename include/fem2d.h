@@ -47,6 +47,8 @@ enum fem2d_status {
   FEM2D_ERR_SPURIOUSLY_CONVERGED = 21, /* every eigenvalue is below 1e-12 in magnitude */
   FEM2D_ERR_PROBLEM_TOO_LARGE = 22,    /* more DoFs than the solver takes */
   FEM2D_ERR_NOT_CONVERGED = 23,        /* the eigensolver's iteration did not converge */
+  /* UniformFieldError::MismatchedSolutionSize (fields.rs:407-411) */
+  FEM2D_ERR_MISMATCHED_SOLUTION_SIZE = 24,
   /* library errors */
   FEM2D_ERR_INVALID_ARG = 100,
   FEM2D_ERR_INVALID_VIEW = 101, /* domain view violates the Domain invariants */
@@ -234,6 +236,36 @@ int fem2d_gep_device_nalgebra_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep,
 int fem2d_solve_gep_dense(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target_eigenvalue, fem2d_eigenpair* out);
 int fem2d_gep_device_solve_dense(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
                                  fem2d_eigenpair* out);
+
+/* ---------------------------------------------------------------------------
+ * Field evaluation: UniformFieldSpace::xy_fields (fields.rs:63-127)
+ * ------------------------------------------------------------------------- */
+
+/* The x and y fields of a solution vector (one value per DoF) on a
+ * density x density grid of points uniform_range(-1, 1, density)
+ * (fields.rs:402-405) over each leaf Elem (an Elem without children), leaves
+ * in ascending id: out_x / out_y[(k * density + m) * density + n] for leaf k,
+ * u point m and v point n. The value at a point is the sum, over the leaf and
+ * its ancestors, of every basis spec's f_u or f_v (basis.rs:225-233) times its
+ * DoF's value. Errors, in the reference's order:
+ * FEM2D_ERR_MISMATCHED_SOLUTION_SIZE if n_solution != the Domain's DoFs;
+ * FEM2D_ERR_PANIC if density_u != density_v (the reference sizes its grids
+ * [density_v][density_u] and indexes them [m][n], which fails unless they are
+ * equal). Densities above 4096 are rejected (FEM2D_ERR_INVALID_ARG).
+ * The device variant takes device arrays (solution and outputs of
+ * num_leaves * density^2 values); the host variant uploads the view and the
+ * solution, and downloads the fields. */
+int fem2d_device_domain_num_leaves(const fem2d_device_domain* dd, int64_t* out);
+/* Device memory on the context's device for callers without an allocator of
+ * their own (hipMalloc / hipFree / synchronous copies). */
+int fem2d_device_malloc(fem2d_ctx* ctx, int64_t bytes, void** out);
+int fem2d_device_free(fem2d_ctx* ctx, void* p);
+int fem2d_memcpy_to_device(fem2d_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int fem2d_memcpy_to_host(fem2d_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int density_u, int density_v,
+                           const double* solution, int64_t n_solution, double* out_x, double* out_y);
+int fem2d_xy_fields(fem2d_ctx* ctx, const fem2d_domain_view* domain, int basis, int density_u, int density_v,
+                    const double* solution, int64_t n_solution, double* out_x, double* out_y);
 
 /* Per-stage device time of the last assembly on this context, measured with
  * HIP events recorded on the context's stream (milliseconds). An assembly that

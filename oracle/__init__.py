@@ -42,6 +42,8 @@ def load():
         lib.oracle_shape_tables.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)] + \
             [C.POINTER(C.c_double)] * 4
         lib.oracle_default_ngq.argtypes = [C.c_int]
+        lib.oracle_xy_fields.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int64,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.oracle_default_ngq.restype = C.c_int
         lib.oracle_assemble.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int64, C.POINTER(C.c_uint8), C.POINTER(OracleGep)]
@@ -130,3 +132,24 @@ def assemble(view, basis: int = 0, a_integral: int = 0, b_integral: int = 1, glq
     finally:
         load().oracle_gep_free(C.byref(g))
     return 0, res
+
+
+def xy_fields(view, basis: int, density, solution):
+    """UniformFieldSpace::xy_fields (fields.rs:63-127) on the CPU: (rc, x, y) with x, y of shape
+    [leaves, density_u, density_v]; rc 1 = MismatchedSolutionSize, 2 = the reference's index panic."""
+    import numpy as np
+
+    parent = np.ctypeslib.as_array(view.parent, shape=(view.n_elems,)) if view.n_elems else np.zeros(0, np.int32)
+    has_child = np.zeros(view.n_elems, dtype=bool)
+    has_child[parent[parent >= 0]] = True
+    n_leaves = int(np.count_nonzero(~has_child))
+    du, dv = int(density[0]), int(density[1])
+    total = n_leaves * max(du, 0) * max(dv, 0)
+    x = np.zeros(max(total, 1))
+    y = np.zeros(max(total, 1))
+    sol = np.ascontiguousarray(solution, dtype=np.float64)
+    P = C.POINTER(C.c_double)
+    rc = load().oracle_xy_fields(C.addressof(view), int(basis), du, dv, sol.ctypes.data_as(P),
+                                 sol.size, x.ctypes.data_as(P), y.ctypes.data_as(P))
+    return rc, x[:total].reshape(n_leaves, max(du, 0), max(dv, 0)), y[:total].reshape(n_leaves, max(du, 0),
+                                                                                      max(dv, 0))

@@ -596,6 +596,61 @@ int oracle_shape_tables(int basis, int n_max, int np, const double* x, double* n
 
 int oracle_default_ngq(int max_order) { return oracle::default_ngq(max_order); }
 
+// UniformFieldSpace::xy_fields (fields.rs:63-127): per leaf Elem (ascending
+// id) the grids x_values / y_values, sized [densities[1]][densities[0]] and
+// indexed [m][n] as the reference does, written to out_x / out_y as
+// [leaf][m][n]. Returns 1 for MismatchedSolutionSize (fields.rs:68-72) and 2
+// where the reference's grid indexing panics (out of range), else 0.
+int oracle_xy_fields(const fem2d_domain_view* v, int basis, int du, int dv, const double* sol, int64_t n_sol,
+                     double* out_x, double* out_y) {
+  if (n_sol != v->n_dofs) return 1;
+  oracle::Setup s(v);
+  s.basis = basis;
+  s.i_max = 0;
+  s.j_max = 0;
+  for (int e = 0; e < v->n_elems; ++e) {  // Mesh::max_expansion_orders (mesh.rs:626-630)
+    s.i_max = std::max<int>(s.i_max, v->orders[2 * e]);
+    s.j_max = std::max<int>(s.j_max, v->orders[2 * e + 1]);
+  }
+  auto uniform_range = [](double lo, double hi, int n) {  // fields.rs:402-405
+    const double step = (hi - lo) / double(n - 1);
+    std::vector<double> r;
+    for (int i = 0; i < n; ++i) r.push_back(double(i) * step + lo);
+    return r;
+  };
+  s.up = uniform_range(-1.0, 1.0, du);
+  s.vp = uniform_range(-1.0, 1.0, dv);
+  size_t out = 0;
+  for (int e = 0; e < v->n_elems; ++e) {
+    if (!s.tree.children[e].empty()) continue;
+    std::vector<std::vector<double>> xv(size_t(dv), std::vector<double>(size_t(du), 0.0)), yv = xv;
+    std::vector<int> anc = s.tree.path(e);  // root .. e
+    std::reverse(anc.begin(), anc.end());   // Mesh::ancestor_elems(e, true): e first
+    for (int a : anc) {
+      const oracle::BasisFn bf = oracle::sample_basis_fn(s, a, e);
+      for (int64_t t = v->bs_offsets[a]; t < v->bs_offsets[a + 1]; ++t)
+        for (int m = 0; m < du; ++m)
+          for (int n = 0; n < dv; ++n) {
+            const int i = v->bs_i[t], j = v->bs_j[t];
+            oracle::V2 value = v->bs_dir[t] == 0 ? bf.f_u(i, j, m, n)
+                               : v->bs_dir[t] == 1 ? bf.f_v(i, j, m, n)
+                                                   : oracle::V2{0.0, 0.0};
+            value = oracle::vmul(value, sol[v->bs_dof[t]]);
+            if (size_t(m) >= xv.size() || size_t(n) >= xv[size_t(m)].size()) return 2;
+            xv[size_t(m)][size_t(n)] += value.x;
+            yv[size_t(m)][size_t(n)] += value.y;
+          }
+    }
+    for (int m = 0; m < du; ++m)
+      for (int n = 0; n < dv; ++n) {
+        out_x[out] = xv[size_t(m)][size_t(n)];
+        out_y[out] = yv[size_t(m)][size_t(n)];
+        ++out;
+      }
+  }
+  return 0;
+}
+
 // Galerkin GEP assembly on the CPU. Returns the galerkin.rs error codes
 // (1 wrong cc, 2 empty DoF set, 3 invalid GLQ settings) before any work.
 // n_threads > 1 runs elem tasks in parallel; results are merged in elem order.

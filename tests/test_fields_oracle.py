@@ -1,0 +1,144 @@
+"""Pins of the oracle's UniformFieldSpace::xy_fields (fields.rs:63-127) restatement:
+
+* the reference's doc-test (fields.rs:46-62): a 10x10 grid on the unit Domain's one leaf;
+* closed forms on a single Elem: with one DoF set to 1, the x field of U(i, j) is
+  N_i(u) T_j(v) / a and the y field of V(i, j) is T_i(u) N_j(v) / b on the grid
+  uniform_range(-1, 1, d) (u along the first grid index), the other component exactly 0;
+* H(curl) conformity for ANY solution vector: the tangential component agrees on both sides of
+  every shared edge, including hanging edges where the coarse side's basis functions come from
+  ancestors evaluated on the fine leaf's sub-range (a wrong sub-range, Jacobian or index order
+  breaks it);
+* the TE10 mode of the 1 x 0.5 waveguide: E_x = 0 and E_y proportional to sin(pi x);
+* the error variants (mismatched solution size; the reference's index panic for unequal densities)."""
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import cases
+import fem_2d_amd as F
+import oracle
+from fem_2d_amd import workloads as W
+
+
+def _leaf_boxes(dom):
+    a = dom.arrays()
+    parent = np.asarray(a["parent"])
+    has_child = np.zeros(parent.size, dtype=bool)
+    has_child[parent[parent >= 0]] = True
+    leaves = np.flatnonzero(~has_child)
+    boxes = []
+    for e in leaves:
+        pts = dom.mesh.elem_points(int(e))
+        boxes.append((pts[0][0], pts[0][1], pts[3][0], pts[3][1]))
+    return leaves, boxes
+
+
+def test_unit_domain_doc_example():
+    """fields.rs:46-62: the unit Domain with a unit solution on a 10x10 grid."""
+    dom = F.Domain.unit(F.ContinuityCondition.HCurl)
+    rc, x, y = oracle.xy_fields(dom.view(), 0, (10, 10), np.ones(dom.num_dofs))
+    assert rc == 0 and x.shape == y.shape == (1, 10, 10)
+
+
+def test_single_elem_closed_forms():
+    m = F.Mesh.unit()
+    m.set_global_expansion_orders(F.Orders(3, 2))
+    dom = F.Domain.from_mesh(m)
+    (x0, y0, x1, y1), = _leaf_boxes(dom)[1]
+    a_, b_ = (x1 - x0) / 2, (y1 - y0) / 2
+    d = 7
+    u = np.array([i * (2.0 / (d - 1)) - 1.0 for i in range(d)])
+    U, V = np.meshgrid(u, u, indexing="ij")  # [m][n]: u along m
+
+    def T(n, t):
+        return 1 - t if n == 0 else 1 + t if n == 1 else t ** n - (1 if n % 2 == 0 else t)
+
+    specs = dom.local_basis_specs(0)
+    assert len(specs) == dom.num_dofs > 0
+    for s in specs:
+        sol = np.zeros(dom.num_dofs)
+        sol[s.dof_id] = 1.0
+        rc, x, y = oracle.xy_fields(dom.view(), 0, (d, d), sol)
+        assert rc == 0
+        if s.dir == F.BasisDir.U:
+            np.testing.assert_allclose(x[0], U ** s.i * T(s.j, V) / a_, rtol=1e-14, atol=1e-14)
+            assert np.all(y[0] == 0.0)
+        else:
+            np.testing.assert_allclose(y[0], T(s.i, U) * V ** s.j / b_, rtol=1e-14, atol=1e-14)
+            assert np.all(x[0] == 0.0)
+
+
+def tangential_jump(dom, x, y, d):
+    """Largest tangential jump over shared edges, and the number of point pairs compared."""
+    leaves, boxes = _leaf_boxes(dom)
+    pts = {}
+    for k, (x0, y0, x1, y1) in enumerate(boxes):
+        t = np.array([i * (2.0 / (d - 1)) - 1.0 for i in range(d)])
+        px = x0 + (t + 1) / 2 * (x1 - x0)
+        py = y0 + (t + 1) / 2 * (y1 - y0)
+        for m in range(d):
+            for side, n in (("bottom", 0), ("top", d - 1)):
+                pts.setdefault((round(px[m], 8), round(py[n], 8)), []).append((k, side, x[k, m, n]))
+        for n in range(d):
+            for side, m in (("left", 0), ("right", d - 1)):
+                pts.setdefault((round(px[m], 8), round(py[n], 8)), []).append((k, side, y[k, m, n]))
+    worst, pairs = 0.0, 0
+    for entries in pts.values():
+        for ka, sa, va in entries:
+            for kb, sb, vb in entries:
+                a, b = boxes[ka], boxes[kb]
+                if (sa, sb) == ("top", "bottom"):
+                    overlap = min(a[2], b[2]) - max(a[0], b[0])
+                elif (sa, sb) == ("right", "left"):
+                    overlap = min(a[3], b[3]) - max(a[1], b[1])
+                else:
+                    continue
+                if ka != kb and overlap > 1e-12:
+                    worst = max(worst, abs(va - vb))
+                    pairs += 1
+    return worst, pairs
+
+
+@pytest.mark.parametrize("name", ["mixed_refinements", "deep_unit", "nalg_problem", "waveguide_small"])
+@pytest.mark.parametrize("basis", [0, 1])
+def test_tangential_continuity(name, basis):
+    dom, _ = cases.ALL[name]()
+    d = 5
+    sol = np.random.default_rng(7).standard_normal(dom.num_dofs)
+    rc, x, y = oracle.xy_fields(dom.view(), basis, (d, d), sol)
+    assert rc == 0
+    worst, pairs = tangential_jump(dom, x, y, d)
+    scale = max(np.abs(x).max(), np.abs(y).max())
+    assert pairs >= 10
+    assert worst <= 1e-10 * scale, (worst, scale)
+
+
+def test_te10_mode_field():
+    w = W.waveguide_small()
+    rc, ref = oracle.assemble(w.domain.view(), glq=w.glq, n_threads=8)
+    assert rc == 0
+    A, B = ref.to_dense()
+    lam, vec = sla.eigh(A, B, subset_by_value=(1.0, 20.0), driver="gvx")
+    assert abs(lam[0] - np.pi ** 2) / np.pi ** 2 < 1e-6
+    d = 5
+    rc, x, y = oracle.xy_fields(w.domain.view(), 0, (d, d), vec[:, 0])
+    assert rc == 0
+    leaves, boxes = _leaf_boxes(w.domain)
+    t = np.array([i * (2.0 / (d - 1)) - 1.0 for i in range(d)])
+    shape = np.stack([np.outer(np.sin(np.pi * (b[0] + (t + 1) / 2 * (b[2] - b[0]))), np.ones(d)) for b in boxes])
+    scale = y.ravel() @ shape.ravel() / (shape.ravel() @ shape.ravel())
+    assert np.abs(x).max() < 1e-4 * abs(scale)
+    assert np.abs(y - scale * shape).max() < 1e-4 * abs(scale)
+
+
+def test_error_variants():
+    dom, _ = cases.nalg_problem()
+    sol = np.ones(dom.num_dofs)
+    assert oracle.xy_fields(dom.view(), 0, (4, 4), sol[:-1])[0] == 1
+    assert oracle.xy_fields(dom.view(), 0, (4, 5), sol)[0] == 2
+    assert oracle.xy_fields(dom.view(), 0, (5, 4), sol)[0] == 2
+    rc, x, _ = oracle.xy_fields(dom.view(), 0, (0, 0), sol)
+    assert rc == 0 and x.size == 0
+    rc, x, y = oracle.xy_fields(dom.view(), 0, (1, 1), sol)  # uniform_range(-1, 1, 1) is [NaN]
+    assert rc == 0 and np.all(np.isnan(x)) and np.all(np.isnan(y))
