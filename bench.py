@@ -19,7 +19,9 @@ stream around the kernel in the same number of kernel-by-kernel steps;
 summary of that kernel. `host_boundary` times the drop-in entry point with
 host arrays on both sides. `domain_build` times the host Domain construction
 from the workload's Mesh, data-parallel and sequential. `eigensolve` times the
-dense GPU eigensolves of assembled GEPs (with LAPACK on the host beside them).
+dense GPU eigensolves of assembled GEPs (with LAPACK on the host beside them);
+`fields` the field evaluation over every leaf of the Domain (with the oracle
+beside it).
 `cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
 workload on the host cores. `mixed_precision` repeats the measurement with the values
@@ -273,6 +275,53 @@ def eigensolve(calls: int = 3):
     return out
 
 
+def fields(w, calls: int = 5, density: int = 8):
+    """UniformFieldSpace::xy_fields on the workload's Domain (device-resident Domain, solution and
+    field buffers; density x density points per leaf Elem), with the oracle's single-threaded
+    restatement timed on the host beside it."""
+    import ctypes as C
+
+    import numpy as np
+
+    import fem_2d_amd as F
+    from fem_2d_amd import _native as N
+
+    lib = N.load()
+    ctx = F.Context(0)
+    dd = F.DeviceDomain(w.domain, ctx)
+    n_leaves = C.c_int64()
+    F.errors.check(lib.fem2d_device_domain_num_leaves(dd._h, C.byref(n_leaves)))
+    total = n_leaves.value * density * density
+    sol = np.random.default_rng(5).standard_normal(w.domain.num_dofs)
+    bufs = [C.c_void_p() for _ in range(3)]
+    for b, nbytes in zip(bufs, (8 * sol.size, 8 * total, 8 * total)):
+        F.errors.check(lib.fem2d_device_malloc(ctx._h, nbytes, C.byref(b)))
+    F.errors.check(lib.fem2d_memcpy_to_device(ctx._h, bufs[0], sol.ctypes.data, 8 * sol.size))
+
+    def call():
+        F.errors.check(lib.fem2d_device_xy_fields(ctx._h, dd._h, w.basis.code, density, density, bufs[0], sol.size,
+                                                  bufs[1], bufs[2]))
+
+    call()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        call()
+    ms = (time.perf_counter() - t0) * 1e3 / calls
+    for b in bufs:
+        lib.fem2d_device_free(ctx._h, b)
+    ctx.close()
+    out = {"leaves": n_leaves.value, "density": density, "points": total, "ms_per_call": ms,
+           "points_per_s": total / (ms * 1e-3), "output_GBs": 16 * total / (ms * 1e-3) / 1e9}
+    import oracle
+
+    t0 = time.perf_counter()
+    rc, _, _ = oracle.xy_fields(w.domain.view(), w.basis.code, (density, density), sol)
+    cpu_s = time.perf_counter() - t0
+    out["cpu_baseline"] = {"points_per_s": total / cpu_s, "s": cpu_s, "cores": 1, "kind": "port",
+                           "sample": "oracle xy_fields over every leaf of the same Domain"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +333,7 @@ def main():
                     help="skip timing the host-array entry point (upload + assembly + download)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-eigen", action="store_true", help="skip timing the dense eigensolves")
+    ap.add_argument("--no-fields", action="store_true", help="skip timing the field evaluation")
     ap.add_argument("--no-domain-build", action="store_true",
                     help="skip timing the host Domain construction (parallel and sequential builders)")
     ap.add_argument("--no-mixed", action="store_true",
@@ -420,6 +470,8 @@ def main():
         line["domain_build"] = domain_build(w)
     if not args.no_eigen:
         line["eigensolve"] = eigensolve()
+    if not args.no_fields:
+        line["fields"] = fields(w)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, pairs, nnz, args.cpu_seconds)
     print(json.dumps(line), flush=True)

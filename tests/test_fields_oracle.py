@@ -142,3 +142,46 @@ def test_error_variants():
     assert rc == 0 and x.size == 0
     rc, x, y = oracle.xy_fields(dom.view(), 0, (1, 1), sol)  # uniform_range(-1, 1, 1) is [NaN]
     assert rc == 0 and np.all(np.isnan(x)) and np.all(np.isnan(y))
+
+
+def test_vtk_writer_layout(tmp_path):
+    """print_all_to_vtk (fields.rs:135-245) on quantities filled from the oracle: header, one point per
+    leaf grid node (x outer, y inner, from the leaf's diagonal corners), (nx-1)(ny-1) quads per leaf
+    with the reference's corner order, cell type 9, and the quantities' values in leaf order with a
+    trailing space each and no line break between quantities."""
+    dom, _ = cases.deep_unit()
+    d = 4
+    sol = np.random.default_rng(1).standard_normal(dom.num_dofs)
+    rc, x, y = oracle.xy_fields(dom.view(), 0, (d, d), sol)
+    assert rc == 0
+    ufs = F.UniformFieldSpace(dom, [d, d])
+    leaves, boxes = _leaf_boxes(dom)
+    ufs.quantities["E_x"] = {int(e): x[k] for k, e in enumerate(leaves)}
+    ufs.quantities["E_y"] = {int(e): y[k] for k, e in enumerate(leaves)}
+    ufs.expression_2arg(["E_x", "E_y"], "E_mag", lambda a, b: (a * a + b * b) ** 0.5)
+    ufs.map_to_quantity("E_x", "E_x2", lambda a: a * a)
+    np.testing.assert_allclose(ufs.quantities["E_x2"][int(leaves[0])], x[0] ** 2, rtol=1e-15)
+    path = tmp_path / "fields.vtk"
+    ufs.print_quantities_to_vkt(str(path), ["E_x", "E_mag"])
+    text = path.read_text()
+    lines = text.split("\n")
+    assert lines[0] == "# vtk DataFile Version 3.0" and lines[3] == "ASCII"
+    assert lines[4] == "DATASET UNSTRUCTURED_GRID"
+    n_pts = d * d * leaves.size
+    i = lines.index(f"POINTS {n_pts} double")
+    x0, y0, x1, y1 = boxes[0]
+    assert lines[i + 1] == f"{x0:.10f} {y0:.10f} 0.0" and lines[i + 2] == f"{x0:.10f} {y0 + (y1 - y0) / 3:.10f} 0.0"
+    n_cells = (d - 1) ** 2 * leaves.size
+    j = lines.index(f"CELLS {n_cells} {5 * n_cells}")
+    assert lines[j + 1] == f"4\t0\t1\t{d + 1}\t{d}"
+    k = lines.index(f"CELL_TYPES {n_cells}")
+    assert lines[k + 1] == " 9" * n_cells
+    assert lines[k + 2] == f"POINT_DATA {n_pts}"
+    assert lines[k + 3] == "SCALARS E_x double 1 " and lines[k + 4] == "LOOKUP_TABLE default"
+    vals, rest = lines[k + 5].split("SCALARS E_mag double 1 ")
+    np.testing.assert_allclose(np.array(vals.split(), dtype=float), x.ravel(), atol=1e-15)
+    assert rest == "" and lines[k + 6] == "LOOKUP_TABLE default"
+    mag = np.array(lines[k + 7].split(), dtype=float)
+    np.testing.assert_allclose(mag, np.hypot(x, y).ravel(), atol=2e-15)
+    with pytest.raises(F.MissingQuantity):
+        ufs.print_quantities_to_vkt(str(path), ["H_x"])
