@@ -1885,6 +1885,7 @@ struct fem2d_ctx {
   hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<DevBuf> bufs;  // grow-only work buffers, by slot
   DevBuf out_col, out_a, out_b;
+  DevBuf field_scratch;  // grid points and status word of the field evaluation
   long long out_capacity = 0;
   long long* h_status = nullptr;  // mapped pinned status block: nnz, err, n_large, n_groups
   long long* d_status = nullptr;  // its device address
@@ -2076,7 +2077,7 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& b : ctx->bufs)
     if (b.p) (void)hipFree(b.p);
-  for (DevBuf* b : {&ctx->out_col, &ctx->out_a, &ctx->out_b})
+  for (DevBuf* b : {&ctx->out_col, &ctx->out_a, &ctx->out_b, &ctx->field_scratch})
     if (b->p) (void)hipFree(b->p);
   for (auto& g : ctx->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -2610,15 +2611,19 @@ int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int ba
     return FEM2D_ERR_INVALID_ARG;
   }
   HIP_TRY(hipSetDevice(ctx->device));
-  // uniform_range(-1.0, 1.0, d) (fields.rs:402-405)
-  std::vector<double> pts(static_cast<size_t>(d));
+  // uniform_range(-1.0, 1.0, d) (fields.rs:402-405), followed by a zeroed
+  // status word, staged in one copy to the context's scratch buffer
+  std::vector<double> pts(static_cast<size_t>(d) + 1, 0.0);
   const double step = (1.0 - -1.0) / double(d - 1);
   for (int i = 0; i < d; ++i) pts[size_t(i)] = double(i) * step + -1.0;
-  double* d_pts = nullptr;
-  int* d_err = nullptr;
-  int rc = upload(pts.data(), pts.size(), &d_pts);
-  const int zero = 0;
-  if (!rc) rc = upload(&zero, 1, &d_err);
+  int rc = ensure(ctx->field_scratch, pts.size() * sizeof(double));
+  double* d_pts = static_cast<double*>(ctx->field_scratch.p);
+  int* d_err = reinterpret_cast<int*>(d_pts + d);
+  if (!rc) {
+    const hipError_t e = hipMemcpyAsync(d_pts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice,
+                                        ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "field grid upload");
+  }
   if (!rc) {
     DevView v{};
     v.n_elems = dd->n_elems;
@@ -2652,8 +2657,6 @@ int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int ba
       rc = FEM2D_ERR_INVALID_VIEW;
     }
   }
-  if (d_pts) (void)hipFree(d_pts);
-  if (d_err) (void)hipFree(d_err);
   return rc;
 }
 
