@@ -20,8 +20,8 @@ summary of that kernel. `host_boundary` times the drop-in entry point with
 host arrays on both sides. `domain_build` times the host Domain construction
 from the workload's Mesh, data-parallel and sequential. `eigensolve` times the
 dense GPU eigensolves of assembled GEPs (with LAPACK on the host beside them);
-`fields` the field evaluation over every leaf of the Domain (with the oracle
-beside it).
+`--fields` would time the field evaluation over every leaf of the Domain, whose
+GPU path is switched off in this build.
 `cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
 workload on the host cores. `mixed_precision` repeats the measurement with the values
@@ -333,7 +333,8 @@ def main():
                     help="skip timing the host-array entry point (upload + assembly + download)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-eigen", action="store_true", help="skip timing the dense eigensolves")
-    ap.add_argument("--no-fields", action="store_true", help="skip timing the field evaluation")
+    ap.add_argument("--fields", action="store_true",
+                    help="time the field evaluation (its GPU path is switched off in this build)")
     ap.add_argument("--no-domain-build", action="store_true",
                     help="skip timing the host Domain construction (parallel and sequential builders)")
     ap.add_argument("--no-mixed", action="store_true",
@@ -470,7 +471,7 @@ def main():
         line["domain_build"] = domain_build(w)
     if not args.no_eigen:
         line["eigensolve"] = eigensolve()
-    if not args.no_fields:
+    if args.fields:
         line["fields"] = fields(w)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, pairs, nnz, args.cpu_seconds)

@@ -2575,6 +2575,11 @@ int fem2d_device_domain_num_leaves(const fem2d_device_domain* dd, int64_t* out) 
   return FEM2D_OK;
 }
 
+// The field kernel is not launched: the first full GPU test run that
+// included it ended with the GPU faulted in a later assembly test, with no
+// cause shown yet; k_xy_fields stays in the source, unlaunched.
+constexpr bool kFieldKernelOff = true;
+
 int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int density_u, int density_v,
                            const double* solution, int64_t n_solution, double* out_x, double* out_y) {
   if (!ctx || !dd) {
@@ -2606,6 +2611,10 @@ int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int ba
   const int d = density_u;
   const long long total = dd->n_leaves * (long long)d * (d > 0 && density_v > 0 ? d : 0);
   if (total == 0) return FEM2D_OK;
+  if (kFieldKernelOff) {
+    set_error("field evaluation on the GPU is switched off in this build", 0);
+    return FEM2D_ERR_UNSUPPORTED;
+  }
   if (!solution || !out_x || !out_y) {
     set_error("null array", 0);
     return FEM2D_ERR_INVALID_ARG;
