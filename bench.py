@@ -238,7 +238,8 @@ def eigensolve(calls: int = 3):
     import cases
 
     ctx = F.Context(0)
-    out = {"solver": "rocSOLVER dsygvd / dpotrf+dpotri+dgemm+dsyevd, FP64"}
+    out = {"solver": "dense symmetric-definite: shift-invert inverse iteration on rocSOLVER dgetrf/dgetrs with one "
+                     "refinement step (dsygvd when inconclusive); nalgebra: dpotrf+dpotri+dgemm+dsyevd; FP64"}
 
     def timed(fn):
         fn()

@@ -4,8 +4,11 @@
 // with its arithmetic: B = L L^T, C = B^-1 A, then the symmetric eigenproblem
 // of C's lower triangle (C is not symmetric; nalgebra's SymmetricEigen reads
 // only that triangle), the eigenvalue nearest the target and its unit vector.
-// fem2d_solve_gep_dense solves A x = lambda B x itself (symmetric-definite,
-// dsygvd), the problem slepc_solve_gep (slepc_solve.rs:14-68) hands to SLEPc.
+// fem2d_solve_gep_dense solves A x = lambda B x itself (symmetric-definite),
+// the problem slepc_solve_gep (slepc_solve.rs:14-68) hands to SLEPc with a
+// target: shift-invert inverse iteration on the LU factors of A - target B,
+// falling back to the whole spectrum (dsygvd) when the iteration is
+// inconclusive.
 //
 // The upper-triangular CSR is expanded to full symmetric column-major matrices
 // in HBM by k_expand (one wavefront per row); the factorisations, the product
@@ -286,6 +289,99 @@ int nalgebra_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, f
   return pick(hD, C, n, target, st, out);
 }
 
+// K = A - shift B, elementwise over the dense matrices
+__global__ void k_shift(long long count, const double* __restrict__ A, const double* __restrict__ B, double shift,
+                        double* __restrict__ K) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count; i += stride) K[i] = A[i] - shift * B[i];
+}
+
+// The eigenpair of A x = lambda B x nearest `target` by shift-invert inverse
+// iteration: K = A - target B is LU-factored once; x <- K^-1 B x, B-normalised,
+// converges to the eigenvector whose eigenvalue is nearest the target (the
+// dominant one of K^-1 B), lambda = x^T A x. Returns 1 when the iteration is
+// inconclusive (K singular, no convergence, or an eigenvalue so small that the
+// spurious-mode test needs the whole spectrum), leaving A and B unchanged.
+int inverse_iteration(rocblas_handle h, hipStream_t st, long long n, const double* A, const double* B, double* K,
+                      double target, DevMem& m, double* value, std::vector<double>* vec) {
+  const rocblas_int ni = rocblas_int(n);
+  rocblas_int *ipiv, *d_info;
+  double *x, *y, *bx, *ax, *r, *K0;
+  EIG_HIP(m.alloc(&K0, size_t(n * n)));
+  EIG_HIP(m.alloc(&r, size_t(n)));
+  EIG_HIP(m.alloc(&ipiv, size_t(n)));
+  EIG_HIP(m.alloc(&d_info, 1));
+  EIG_HIP(m.alloc(&x, size_t(n)));
+  EIG_HIP(m.alloc(&y, size_t(n)));
+  EIG_HIP(m.alloc(&bx, size_t(n)));
+  EIG_HIP(m.alloc(&ax, size_t(n)));
+  k_shift<<<grid_for(n * n), kBlock, 0, st>>>(n * n, A, B, target, K0);
+  EIG_HIP(hipGetLastError());
+  EIG_HIP(hipMemcpyAsync(K, K0, size_t(n * n) * sizeof(double), hipMemcpyDeviceToDevice, st));
+  EIG_BLAS(rocsolver_dgetrf(h, ni, ni, K, ni, ipiv, d_info));
+  rocblas_int info = 0;
+  if (int rc = read_info(d_info, st, &info)) return rc;
+  if (info != 0) return 1;  // the target is an eigenvalue to working precision
+  // a fixed start vector with components in [0.5, 1.5) (SplitMix64)
+  std::vector<double> x0(static_cast<size_t>(n));
+  unsigned long long state = 0x9E3779B97F4A7C15ull;
+  for (auto& v : x0) {
+    unsigned long long z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    v = 0.5 + double(z >> 11) * 0x1.0p-53;
+  }
+  EIG_HIP(hipMemcpyAsync(x, x0.data(), size_t(n) * sizeof(double), hipMemcpyHostToDevice, st));
+  const double one = 1.0, zero = 0.0;
+  EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, B, ni, x, 1, &zero, bx, 1));
+  // converged: a relative residual |A x - lambda B x| / |A x| of kTol, or of
+  // kTolStall once the Rayleigh quotient no longer moves (the residual's
+  // rounding floor grows with the conditioning of A and B)
+  constexpr int kMaxIter = 100;
+  constexpr double kTol = 1e-12, kTolStall = 1e-10;
+  double lam_prev = HUGE_VAL;
+  for (int it = 0; it < kMaxIter; ++it) {
+    EIG_BLAS(rocblas_dcopy(h, ni, bx, 1, y, 1));
+    EIG_BLAS(rocsolver_dgetrs(h, rocblas_operation_none, ni, 1, K, ni, ipiv, y, ni));  // y = K^-1 B x
+    // one step of iterative refinement against the unfactored K: the
+    // triangular solves alone leave errors of the size of K's conditioning
+    const double m_one = -1.0;
+    EIG_BLAS(rocblas_dcopy(h, ni, bx, 1, r, 1));
+    EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &m_one, K0, ni, y, 1, &one, r, 1));  // B x - K y
+    EIG_BLAS(rocsolver_dgetrs(h, rocblas_operation_none, ni, 1, K, ni, ipiv, r, ni));
+    EIG_BLAS(rocblas_daxpy(h, ni, &one, r, 1, y, 1));
+    EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, B, ni, y, 1, &zero, bx, 1));
+    double s = 0.0;
+    EIG_BLAS(rocblas_ddot(h, ni, y, 1, bx, 1, &s));
+    if (!(s > 0.0) || !std::isfinite(s)) return 1;
+    const double inv = 1.0 / std::sqrt(s);
+    EIG_BLAS(rocblas_dcopy(h, ni, y, 1, x, 1));
+    EIG_BLAS(rocblas_dscal(h, ni, &inv, x, 1));   // x^T B x = 1
+    EIG_BLAS(rocblas_dscal(h, ni, &inv, bx, 1));  // B x
+    EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, A, ni, x, 1, &zero, ax, 1));
+    double lam = 0.0, na = 0.0, nr = 0.0;
+    EIG_BLAS(rocblas_ddot(h, ni, x, 1, ax, 1, &lam));
+    EIG_BLAS(rocblas_dnrm2(h, ni, ax, 1, &na));
+    const double neg = -lam;
+    EIG_BLAS(rocblas_dcopy(h, ni, ax, 1, y, 1));
+    EIG_BLAS(rocblas_daxpy(h, ni, &neg, bx, 1, y, 1));  // A x - lambda B x
+    EIG_BLAS(rocblas_dnrm2(h, ni, y, 1, &nr));
+    if (!std::isfinite(lam) || !std::isfinite(nr)) return 1;
+    const bool stalled = std::fabs(lam - lam_prev) <= 1e-14 * std::fabs(lam);
+    lam_prev = lam;
+    if (nr <= kTol * na || (stalled && nr <= kTolStall * na)) {
+      if (std::fabs(lam) < kSpurious) return 1;
+      vec->resize(static_cast<size_t>(n));
+      EIG_HIP(hipMemcpyAsync(vec->data(), x, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, st));
+      EIG_HIP(hipStreamSynchronize(st));
+      *value = lam;
+      return 0;
+    }
+  }
+  return 1;
+}
+
 int dense_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem2d_eigenpair* out) {
   if (int rc = check_device_args(ctx, gep, out)) return rc;
   const long long n = gep->n;
@@ -302,14 +398,43 @@ int dense_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem2
   rocblas_handle h;
   if (int rc = solver_handle(ctx, st, &h)) return rc;
   DevMem m;
-  double *A, *B, *D, *E;
+  double *A, *B, *C, *D, *E;
   if (int rc = expand(gep, st, m, &A, &B)) return rc;
+  EIG_HIP(m.alloc(&C, size_t(n * n)));
   EIG_HIP(m.alloc(&D, size_t(n)));
   EIG_HIP(m.alloc(&E, size_t(n)));
   rocblas_int* d_info;
   EIG_HIP(m.alloc(&d_info, 1));
   rocblas_int info = 0;
   const rocblas_int ni = rocblas_int(n);
+  // B positive definite: its Cholesky factorisation (of a copy) succeeds, the
+  // condition the generalized solver checks first
+  EIG_HIP(hipMemcpyAsync(C, B, size_t(n * n) * sizeof(double), hipMemcpyDeviceToDevice, st));
+  EIG_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, ni, C, ni, d_info));
+  if (int rc = read_info(d_info, st, &info)) return rc;
+  if (info != 0) {
+    set_error("Failed to invert B-matrix (via cholesky); likely ill-conditioned!", int(info));
+    return FEM2D_ERR_FAILED_TO_INVERT_B;
+  }
+  {
+    double lam = 0.0;
+    std::vector<double> vec;
+    const int it = inverse_iteration(h, st, n, A, B, C, target, m, &lam, &vec);
+    if (it < 0 || it > 1) return it;  // a HIP or rocBLAS failure (status codes are >= 100)
+    if (it == 0) {
+      double* v = static_cast<double*>(std::malloc(size_t(n) * sizeof(double)));
+      if (!v) {
+        set_error("host allocation of the eigenvector failed", 0);
+        return FEM2D_ERR_OOM;
+      }
+      std::memcpy(v, vec.data(), size_t(n) * sizeof(double));
+      out->value = lam;
+      out->n = n;
+      out->vector = v;
+      return FEM2D_OK;
+    }
+  }
+  // the whole spectrum (dsygvd): inverse iteration was inconclusive
   EIG_BLAS(rocsolver_dsygvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, ni, A, ni, B, ni, D, E,
                             d_info));
   if (int rc = read_info(d_info, st, &info)) return rc;

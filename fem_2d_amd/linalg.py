@@ -62,8 +62,8 @@ def nalgebra_solve_gep(gep: Union[GEP, N.GepDevice], target_eigenvalue: float,
 
 def solve_gep_dense(gep: Union[GEP, N.GepDevice], target_eigenvalue: float,
                     ctx: Optional[Context] = None) -> EigenPair:
-    """The symmetric-definite problem A x = lambda B x (rocSOLVER dsygvd), the one slepc_solve_gep
-    (slepc_solve.rs:14-68) hands to SLEPc, for up to DENSE_GEP_MAX_SIZE DoFs: the eigenpair nearest
-    ``target_eigenvalue``, its vector B-normalised. Raises ProblemTooLarge, FailedToInvertB,
-    SpuriouslyConverged or FailedToConverge."""
+    """The symmetric-definite problem A x = lambda B x, the one slepc_solve_gep (slepc_solve.rs:14-68)
+    hands to SLEPc, for up to DENSE_GEP_MAX_SIZE DoFs: the eigenpair nearest ``target_eigenvalue``
+    (shift-invert inverse iteration; the whole spectrum when that is inconclusive), its vector
+    B-normalised. Raises ProblemTooLarge, FailedToInvertB, SpuriouslyConverged or FailedToConverge."""
     return _solve("fem2d_solve_gep_dense", "fem2d_gep_device_solve_dense", gep, target_eigenvalue, ctx)

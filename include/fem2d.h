@@ -225,11 +225,12 @@ int fem2d_nalgebra_solve_gep(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double ta
 int fem2d_gep_device_nalgebra_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
                                     fem2d_eigenpair* out);
 
-/* The symmetric-definite problem A x = lambda B x itself (rocSOLVER dsygvd),
- * standing in for slepc_solve_gep (slepc_solve.rs:14-68) up to
- * FEM2D_DENSE_GEP_MAX_SIZE DoFs: the eigenpair whose value is nearest
- * target_eigenvalue (the lowest on a tie), the vector B-normalised
- * (x^T B x = 1). Errors: FEM2D_ERR_PROBLEM_TOO_LARGE,
+/* The symmetric-definite problem A x = lambda B x itself, standing in for
+ * slepc_solve_gep (slepc_solve.rs:14-68) up to FEM2D_DENSE_GEP_MAX_SIZE DoFs:
+ * the eigenpair whose value is nearest target_eigenvalue (the lowest on a tie),
+ * the vector B-normalised (x^T B x = 1). Shift-invert inverse iteration on the
+ * LU factors of A - target B; the whole spectrum (dsygvd) when that is
+ * inconclusive. Errors: FEM2D_ERR_PROBLEM_TOO_LARGE,
  * FEM2D_ERR_FAILED_TO_INVERT_B, FEM2D_ERR_SPURIOUSLY_CONVERGED (as above),
  * FEM2D_ERR_NOT_CONVERGED. */
 #define FEM2D_DENSE_GEP_MAX_SIZE 32768

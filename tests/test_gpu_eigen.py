@@ -5,8 +5,9 @@ known answers:
 * nalgebra_solve_gep on the 60-DoF problem of lib.rs:45-80 gives 2.6479657 (tolerance 1e-6), the
   value of the lower triangle of B^-1 A; eigenvalue within EIG_REL of the oracle's, vector
   parallel to the oracle's within VEC_TOL (unit 2-norm);
-* the symmetric-definite solve on the 624-DoF problem of lib.rs:82-119 gives SLEPc's
-  1.4745880937 (tolerance 1e-9) with the vector of test_evec.dat;
+* the symmetric-definite solve (shift-invert inverse iteration, whole spectrum when that is
+  inconclusive) on the 624-DoF problem of lib.rs:82-119 gives SLEPc's 1.4745880937 (tolerance
+  1e-9) with the vector of test_evec.dat;
 * the 1096-DoF waveguide exceeds nalgebra's 1000-DoF cap (ProblemTooLarge); its dense solve gives
   the oracle's TE10 eigenvalue (analytic pi^2 within the discretisation error);
 * the 4496-DoF precision-study matrices (beyond what the oracle's dense solve does in seconds):
@@ -151,6 +152,11 @@ def test_closed_forms_and_error_variants(ctx):
     diag = _gep(4, range(4), range(4), a, b)
     p = F.solve_gep_dense(diag, 2.9, ctx=ctx)
     assert p.value == pytest.approx(3.0, rel=1e-14) and abs(p.vector[2]) == pytest.approx(3.0 ** -0.5, rel=1e-14)
+    # the target an eigenvalue (A - target B singular) and a target equidistant from 2 (double) and 3:
+    # the shift-invert iteration is inconclusive and the whole spectrum decides (the lowest on a tie)
+    assert F.solve_gep_dense(diag, 3.0, ctx=ctx).value == pytest.approx(3.0, rel=1e-14)
+    assert F.solve_gep_dense(diag, 2.5, ctx=ctx).value == pytest.approx(2.0, rel=1e-14)
+    assert OE.dense_solve_gep(np.diag(a), np.diag(b), 2.5)[0] == pytest.approx(2.0, rel=1e-14)
     p = F.nalgebra_solve_gep(diag, 0.4, ctx=ctx)
     assert p.value == pytest.approx(0.5, rel=1e-14) and abs(p.vector[3]) == pytest.approx(1.0, rel=1e-14)
     neg_b = _gep(3, range(3), range(3), [1.0, 1.0, 1.0], [-1.0, -1.0, -1.0])
