@@ -20,7 +20,8 @@ summary of that kernel. `host_boundary` times the drop-in entry point with
 host arrays on both sides. `domain_build` times the host Domain construction
 from the workload's Mesh, data-parallel and sequential. `eigensolve` times the
 dense GPU eigensolves of assembled GEPs (with LAPACK on the host beside them);
-`--fields` would time the field evaluation over every leaf of the Domain, whose
+`petsc_writer` the PETSc AIJ files of the assembled GEP (with the oracle's writer
+beside it); `--fields` would time the field evaluation over every leaf of the Domain, whose
 GPU path is switched off in this build.
 `cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
@@ -223,6 +224,54 @@ def domain_build(w, calls: int = 3):
     return out
 
 
+def petsc_writer(w, calls: int = 2):
+    """GEP::print_to_petsc_binary_files (linalg.rs:44-52) from the device-resident GEP of the
+    workload: both full symmetric AIJ files (transpose, big-endian words on the device, then the
+    file writes) into a temporary directory; the oracle's numpy writer is timed on the host on
+    the leading rows' submatrix (about 2 M stored entries)."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    import fem_2d_amd as F
+
+    ctx = F.Context(0)
+    dd = F.DeviceDomain(w.domain, ctx)
+    dev = dd.assemble(w.glq)
+    tmp = tempfile.mkdtemp(prefix="fem2d_petsc_")
+    try:
+        os.makedirs(os.path.join(tmp, "tmp"))
+        dd.print_to_petsc_binary_files(dev, tmp, "bench")
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            dd.print_to_petsc_binary_files(dev, tmp, "bench")
+        ms = (time.perf_counter() - t0) * 1e3 / calls
+        nbytes = sum(os.path.getsize(os.path.join(tmp, "tmp", f"bench_{m}.dat")) for m in ("a", "b"))
+        g = dd.download(dev)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    ctx.close()
+    full = 2 * g.nnz - int(np.count_nonzero(g.rows() == g.col))
+    out = {"ms_per_call": ms, "file_bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9,
+           "full_entries": full, "entries_per_s": 2 * full / (ms * 1e-3)}
+    from oracle import petsc as OP
+
+    rows = g.rows()
+    k = int(np.searchsorted(g.row_ptr, 2_000_000))
+    sel = (rows < k) & (g.col < k)
+    t0 = time.perf_counter()
+    OP.aij_binary_bytes(k, rows[sel], g.col[sel], g.a_val[sel])
+    OP.aij_binary_bytes(k, rows[sel], g.col[sel], g.b_val[sel])
+    cpu_s = time.perf_counter() - t0
+    n_sel = int(np.count_nonzero(sel))
+    cpu_full = 2 * n_sel - int(np.count_nonzero(rows[sel] == g.col[sel]))
+    out["cpu_baseline"] = {"entries_per_s": 2 * cpu_full / cpu_s, "s": cpu_s, "cores": 1, "kind": "port",
+                           "sample": f"oracle numpy writer, both matrices of the leading {k} rows' submatrix "
+                                     f"({n_sel} stored entries), bytes in memory (no file writes)"}
+    return out
+
+
 def eigensolve(calls: int = 3):
     """Dense eigensolves after the assembly, device-resident GEP in, eigenpair out (rocSOLVER after
     the library's CSR-to-dense kernel): the symmetric-definite solve near the TE10 cutoff on the
@@ -334,6 +383,7 @@ def main():
                     help="skip timing the host-array entry point (upload + assembly + download)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-eigen", action="store_true", help="skip timing the dense eigensolves")
+    ap.add_argument("--no-petsc", action="store_true", help="skip timing the PETSc binary writer")
     ap.add_argument("--fields", action="store_true",
                     help="time the field evaluation (its GPU path is switched off in this build)")
     ap.add_argument("--no-domain-build", action="store_true",
@@ -472,6 +522,8 @@ def main():
         line["domain_build"] = domain_build(w)
     if not args.no_eigen:
         line["eigensolve"] = eigensolve()
+    if not args.no_petsc:
+        line["petsc_writer"] = petsc_writer(w)
     if args.fields:
         line["fields"] = fields(w)
     if not args.no_cpu_baseline:
