@@ -2629,8 +2629,9 @@ int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int ba
   double* d_pts = static_cast<double*>(ctx->field_scratch.p);
   int* d_err = reinterpret_cast<int*>(d_pts + d);
   if (!rc) {
-    const hipError_t e = hipMemcpyAsync(d_pts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice,
-                                        ctx->stream);
+    hipError_t e = hipMemcpyAsync(d_pts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // pts is read before any return below
     if (e != hipSuccess) rc = hip_fail(e, "field grid upload");
   }
   if (!rc) {

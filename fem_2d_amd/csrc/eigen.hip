@@ -333,6 +333,7 @@ int inverse_iteration(rocblas_handle h, hipStream_t st, long long n, const doubl
     v = 0.5 + double(z >> 11) * 0x1.0p-53;
   }
   EIG_HIP(hipMemcpyAsync(x, x0.data(), size_t(n) * sizeof(double), hipMemcpyHostToDevice, st));
+  EIG_HIP(hipStreamSynchronize(st));  // x0 is read before any return below
   const double one = 1.0, zero = 0.0;
   EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, B, ni, x, 1, &zero, bx, 1));
   // converged: a relative residual |A x - lambda B x| / |A x| of kTol, or of

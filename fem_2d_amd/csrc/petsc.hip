@@ -150,7 +150,9 @@ struct PinnedStream {
   static constexpr size_t kChunk = size_t(32) << 20;
   void* buf[2] = {nullptr, nullptr};
   hipEvent_t done[2] = {nullptr, nullptr};
+  hipStream_t stream = nullptr;  // the copies' stream: drained before the buffers go
   ~PinnedStream() {
+    if (stream) (void)hipStreamSynchronize(stream);
     for (int k = 0; k < 2; ++k) {
       if (done[k]) (void)hipEventDestroy(done[k]);
       if (buf[k]) (void)hipHostFree(buf[k]);
@@ -167,6 +169,7 @@ struct PinnedStream {
   }
   // appends bytes [src, src + bytes) of device memory to f
   int copy_out(FILE* f, const void* src, size_t bytes, hipStream_t st, const std::string& path) {
+    stream = st;
     const unsigned char* s = static_cast<const unsigned char*>(src);
     const size_t n_chunks = (bytes + kChunk - 1) / kChunk;
     auto issue = [&](size_t c) -> hipError_t {
