@@ -49,6 +49,8 @@
 
 #include "../../include/fem2d.h"
 #include "host/capi_internal.hpp"
+#include "host/parallel.hpp"
+#include "pinned_staging.hpp"
 
 namespace {
 
@@ -2727,36 +2729,46 @@ int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d
     return FEM2D_ERR_OOM;
   }
   HIP_TRY(hipSetDevice(ctx->device));
-  HIP_TRY(hipMemcpyAsync(out->row_ptr, src->row_ptr, size_t(src->n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
-                         ctx->stream));
-  if (src->nnz > 0) {
-    HIP_TRY(hipMemcpyAsync(out->col, src->col, size_t(src->nnz) * sizeof(int32_t), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    if (src->precision == FEM2D_PRECISION_FP32) {
-      // FP32 values land in the front half of the double arrays and are widened in place
-      HIP_TRY(hipMemcpyAsync(out->a_val, src->a_val32, size_t(src->nnz) * sizeof(float), hipMemcpyDeviceToHost,
-                             ctx->stream));
-      HIP_TRY(hipMemcpyAsync(out->b_val, src->b_val32, size_t(src->nnz) * sizeof(float), hipMemcpyDeviceToHost,
-                             ctx->stream));
-    } else {
-      HIP_TRY(hipMemcpyAsync(out->a_val, src->a_val, size_t(src->nnz) * sizeof(double), hipMemcpyDeviceToHost,
-                             ctx->stream));
-      HIP_TRY(hipMemcpyAsync(out->b_val, src->b_val, size_t(src->nnz) * sizeof(double), hipMemcpyDeviceToHost,
-                             ctx->stream));
-    }
-  }
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  if (src->precision == FEM2D_PRECISION_FP32) {
-    // from the last entry down, each double covers floats it has already read
-    for (double* v : {out->a_val, out->b_val}) {
-      unsigned char* bytes = reinterpret_cast<unsigned char*>(v);
-      for (long long i = src->nnz - 1; i >= 0; --i) {
-        float x;
-        std::memcpy(&x, bytes + 4 * i, sizeof(float));
-        const double y = double(x);
-        std::memcpy(bytes + 8 * i, &y, sizeof(double));
-      }
-    }
+  // through pinned staging buffers; each chunk is spread over the destination
+  // (pageable, first touched here) by several host threads while the next
+  // chunk is in flight
+  fem2d::PinnedStaging ps;
+  HIP_TRY(ps.init(ctx->stream));
+  const int threads = std::min(8, fem2d::default_threads());
+  auto copy_to = [&](void* dst) {
+    return [dst, threads](const unsigned char* p, size_t off, size_t len) -> int {
+      unsigned char* d = static_cast<unsigned char*>(dst) + off;
+      fem2d::parallel_ranges(len, threads, size_t(1) << 22,
+                             [&](size_t b, size_t e) { std::memcpy(d + b, p + b, e - b); });
+      return 0;
+    };
+  };
+  // FP32 values widened to double on the way
+  auto widen_to = [&](double* dst) {
+    return [dst, threads](const unsigned char* p, size_t off, size_t len) -> int {
+      double* d = dst + off / sizeof(float);
+      fem2d::parallel_ranges(len / sizeof(float), threads, size_t(1) << 20, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) {
+          float x;
+          std::memcpy(&x, p + i * sizeof(float), sizeof(float));
+          d[i] = double(x);
+        }
+      });
+      return 0;
+    };
+  };
+  hipError_t e = hipSuccess;
+  const size_t nnz = size_t(std::max<int64_t>(0, src->nnz));
+  const bool fp32 = src->precision == FEM2D_PRECISION_FP32;
+  int rc = ps.run(src->row_ptr, size_t(src->n + 1) * sizeof(int64_t), copy_to(out->row_ptr), &e);
+  if (!rc) rc = ps.run(src->col, nnz * sizeof(int32_t), copy_to(out->col), &e);
+  if (!rc && fp32) rc = ps.run(src->a_val32, nnz * sizeof(float), widen_to(out->a_val), &e);
+  if (!rc && fp32) rc = ps.run(src->b_val32, nnz * sizeof(float), widen_to(out->b_val), &e);
+  if (!rc && !fp32) rc = ps.run(src->a_val, nnz * sizeof(double), copy_to(out->a_val), &e);
+  if (!rc && !fp32) rc = ps.run(src->b_val, nnz * sizeof(double), copy_to(out->b_val), &e);
+  if (rc) {
+    fem2d_gep_csr_free(out);
+    return rc == -1 ? hip_fail(e, "GEP download") : rc;
   }
   return FEM2D_OK;
 }

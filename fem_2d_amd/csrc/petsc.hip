@@ -26,11 +26,13 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
 #include "../../include/fem2d.h"
 #include "host/capi_internal.hpp"
+#include "pinned_staging.hpp"
 
 namespace {
 
@@ -144,73 +146,31 @@ int hip_fail(hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail(_e, #expr);  \
   } while (0)
 
-// Two pinned host buffers through which device byte ranges stream to a file:
-// the copy of the next chunk runs while the previous one is written.
-struct PinnedStream {
-  static constexpr size_t kChunk = size_t(32) << 20;
-  void* buf[2] = {nullptr, nullptr};
-  hipEvent_t done[2] = {nullptr, nullptr};
-  hipStream_t stream = nullptr;  // the copies' stream: drained before the buffers go
-  ~PinnedStream() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (int k = 0; k < 2; ++k) {
-      if (done[k]) (void)hipEventDestroy(done[k]);
-      if (buf[k]) (void)hipHostFree(buf[k]);
-    }
-  }
-  hipError_t init() {
-    for (int k = 0; k < 2; ++k) {
-      hipError_t e = hipHostMalloc(&buf[k], kChunk, hipHostMallocDefault);
-      if (e != hipSuccess) return e;
-      e = hipEventCreateWithFlags(&done[k], hipEventDisableTiming);
-      if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
-  }
-  // appends bytes [src, src + bytes) of device memory to f
-  int copy_out(FILE* f, const void* src, size_t bytes, hipStream_t st, const std::string& path) {
-    stream = st;
-    const unsigned char* s = static_cast<const unsigned char*>(src);
-    const size_t n_chunks = (bytes + kChunk - 1) / kChunk;
-    auto issue = [&](size_t c) -> hipError_t {
-      const size_t off = c * kChunk, len = std::min(kChunk, bytes - off);
-      hipError_t e = hipMemcpyAsync(buf[c % 2], s + off, len, hipMemcpyDeviceToHost, st);
-      return e == hipSuccess ? hipEventRecord(done[c % 2], st) : e;
-    };
-    if (n_chunks) {
-      const hipError_t e0 = issue(0);
-      if (e0 != hipSuccess) return hip_fail(e0, "PETSc file staging copy");
-    }
-    for (size_t c = 0; c < n_chunks; ++c) {
-      hipError_t e = hipEventSynchronize(done[c % 2]);
-      if (e == hipSuccess && c + 1 < n_chunks) e = issue(c + 1);
-      if (e != hipSuccess) return hip_fail(e, "PETSc file staging copy");
-      const size_t len = std::min(kChunk, bytes - c * kChunk);
-      if (std::fwrite(buf[c % 2], 1, len, f) != len) {
-        set_error("short write to " + path, 0);
-        return FEM2D_ERR_IO;
-      }
-    }
-    return FEM2D_OK;
-  }
-};
-
 // One AIJ file: the header, then the row lengths, column indices and values,
 // each streamed from the device arrays the kernels produced.
 int write_file(const std::string& path, uint32_t n, uint32_t nnz, const uint32_t* counts_be, const uint32_t* col_be,
-               const uint64_t* val_be, hipStream_t st, PinnedStream& ps) {
+               const uint64_t* val_be, fem2d::PinnedStaging& ps) {
   FILE* f = std::fopen(path.c_str(), "wb");
   if (!f) {
     set_error("cannot create " + path, 0);
     return FEM2D_ERR_IO;
   }
+  auto to_file = [&](const unsigned char* p, size_t, size_t len) -> int {
+    if (std::fwrite(p, 1, len, f) == len) return FEM2D_OK;
+    set_error("short write to " + path, 0);
+    return FEM2D_ERR_IO;
+  };
   const uint32_t header[4] = {__builtin_bswap32(kMatFileClassId), __builtin_bswap32(n), __builtin_bswap32(n),
                               __builtin_bswap32(nnz)};
-  int rc = std::fwrite(header, sizeof(header), 1, f) == 1 ? FEM2D_OK : FEM2D_ERR_IO;
-  if (rc) set_error("short write to " + path, 0);
-  if (!rc) rc = ps.copy_out(f, counts_be, size_t(n) * 4, st, path);
-  if (!rc) rc = ps.copy_out(f, col_be, size_t(nnz) * 4, st, path);
-  if (!rc) rc = ps.copy_out(f, val_be, size_t(nnz) * 8, st, path);
+  int rc = to_file(reinterpret_cast<const unsigned char*>(header), 0, sizeof(header));
+  const std::pair<const void*, size_t> sections[3] = {
+      {counts_be, size_t(n) * 4}, {col_be, size_t(nnz) * 4}, {val_be, size_t(nnz) * 8}};
+  for (const auto& sec : sections) {
+    if (rc) break;
+    hipError_t e = hipSuccess;
+    rc = ps.run(sec.first, sec.second, to_file, &e);
+    if (rc == -1) rc = hip_fail(e, "PETSc file staging copy");
+  }
   if (std::fclose(f) != 0 && !rc) {
     set_error("short write to " + path, 0);
     rc = FEM2D_ERR_IO;
@@ -290,12 +250,12 @@ int fem2d_gep_device_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep
     PETSC_TRY(hipGetLastError());
   }
   PETSC_TRY(hipStreamSynchronize(st));
-  PinnedStream ps;
-  PETSC_TRY(ps.init());
+  fem2d::PinnedStaging ps;
+  PETSC_TRY(ps.init(st));
   // linalg.rs:49-50: "{dir}/tmp/{prefix}_a.dat" and "{dir}/tmp/{prefix}_b.dat"
   const std::string base = std::string(dir) + "/tmp/" + prefix;
-  if (int rc = write_file(base + "_a.dat", uint32_t(n), uint32_t(full), counts_be, col_be, a_be, st, ps)) return rc;
-  return write_file(base + "_b.dat", uint32_t(n), uint32_t(full), counts_be, col_be, b_be, st, ps);
+  if (int rc = write_file(base + "_a.dat", uint32_t(n), uint32_t(full), counts_be, col_be, a_be, ps)) return rc;
+  return write_file(base + "_b.dat", uint32_t(n), uint32_t(full), counts_be, col_be, b_be, ps);
 }
 
 int fem2d_gep_csr_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep_csr* gep, const char* dir,
