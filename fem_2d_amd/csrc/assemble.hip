@@ -73,12 +73,25 @@ constexpr int kMaxSrc = 128;
 constexpr int kMaxRaw = 1024;
 constexpr int kMaxDist = 1024;
 constexpr int kMaxRows = 1024;
-// streaming fill: 8 wavefronts per block share one LDS copy of the Gram
-// library (up to kLibLdsMax values); three blocks per CU (48 KB of LDS each)
-// keep 6 wavefronts per SIMD
-constexpr int kFillBlock = 512;
+// streaming fill: the wavefronts of a block share one LDS copy of the Gram
+// library (up to kLibLds values) beside their staged source records. FP64
+// output: 16 wavefronts and up to 96 KB of library (orders to 8 over three
+// levels of refinement), one block per CU, measured faster than three
+// 8-wavefront blocks with a 32 KB library; FP32 output (12 B per entry, more
+// latency-bound) keeps the three smaller blocks per CU.
+template <class V>
+struct FillShape;
+template <>
+struct FillShape<double> {
+  static constexpr int kBlock = 1024;
+  static constexpr int kLibLds = 12288;
+};
+template <>
+struct FillShape<float> {
+  static constexpr int kBlock = 512;
+  static constexpr int kLibLds = 4096;
+};
 constexpr int kFillWavesPerSimd = 6;
-constexpr int kLibLdsMax = 4096;
 // latency-bound kernels: at most 64 VGPRs, so that 8 wavefronts per SIMD are resident
 constexpr int kLatencyWavesPerSimd = 8;
 constexpr int kMaxDepth = 24;      // h-refinement depth below a base Elem
@@ -1377,12 +1390,12 @@ __device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, con
 // (else L2Inner); V: the stored precision (double, or float for
 // FEM2D_PRECISION_FP32; the arithmetic is FP64 either way).
 template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl, class V>
-__global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
+__global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
   if (*w.err) return;
   // the source records of the row group being written, per wavefront
-  __shared__ double4 src_lds[kFillBlock / kWave][kWS * kCoef / 4];
+  __shared__ double4 src_lds[FillShape<V>::kBlock / kWave][kWS * kCoef / 4];
   // the block's copy of the Gram library (when it fits)
-  __shared__ double lib_lds[kLibInLds ? kLibLdsMax : 1];
+  __shared__ double lib_lds[kLibInLds ? FillShape<V>::kLibLds : 1];
   const long long nnz = w.row_ptr[w.n_dofs];
   if (nnz > o.capacity) return;  // the host grows the buffers and runs the fill again
   if (kLibInLds) {
@@ -1466,10 +1479,11 @@ __global__ void __launch_bounds__(kFillBlock, kFillWavesPerSimd) k_fill_flat(Wor
 template <bool kLibInLds, class V>
 void launch_fill_lib(bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
                      const FillOut& o) {
-  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
-  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
-  else if (a_cc) k_fill_flat<kLibInLds, true, true, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
-  else k_fill_flat<kLibInLds, false, false, V><<<grid, kFillBlock, 0, st>>>(w, L, o);
+  constexpr int B = FillShape<V>::kBlock;
+  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false, V><<<grid, B, 0, st>>>(w, L, o);
+  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true, V><<<grid, B, 0, st>>>(w, L, o);
+  else if (a_cc) k_fill_flat<kLibInLds, true, true, V><<<grid, B, 0, st>>>(w, L, o);
+  else k_fill_flat<kLibInLds, false, false, V><<<grid, B, 0, st>>>(w, L, o);
 }
 
 void launch_fill(bool lds, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
@@ -1896,8 +1910,8 @@ struct fem2d_ctx {
   fem2d_timing timing{};
   int n_cu = 0;
   int count_grid = 0;  // resident blocks of the wave kernels on this device
-  int fill_grid = 0;      // resident blocks of the streaming fill (library in global memory)
-  int fill_grid_lds = 0;  // the same with the library in LDS
+  int fill_grid[2] = {0, 0};      // resident blocks of the streaming fill (library in global memory), FP64 / FP32
+  int fill_grid_lds[2] = {0, 0};  // the same with the library in LDS
   bool use_graphs = true;
   hipGraphExec_t graph_exec[1] = {nullptr};  // the last recorded assembly
   GraphKey graph_key;
@@ -2060,11 +2074,19 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_describe_wave, kBlock, 0));
   ctx->count_grid = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, double>, kFillBlock, 0));
-  ctx->fill_grid = std::max(1, per_cu) * ctx->n_cu;
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, double>,
+                                                   FillShape<double>::kBlock, 0));
+  ctx->fill_grid[0] = std::max(1, per_cu) * ctx->n_cu;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, float>,
+                                                       FillShape<float>::kBlock, 0));
+  ctx->fill_grid[1] = std::max(1, per_cu) * ctx->n_cu;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, float>,
+                                                       FillShape<float>::kBlock, 0));
+  ctx->fill_grid_lds[1] = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, double>, kFillBlock, 0));
-  ctx->fill_grid_lds = std::max(1, per_cu) * ctx->n_cu;
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, double>,
+                                                   FillShape<double>::kBlock, 0));
+  ctx->fill_grid_lds[0] = std::max(1, per_cu) * ctx->n_cu;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fill_large),
@@ -2391,8 +2413,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   // numeric: columns and values; with too small output buffers the fill
   // writes nothing and the host grows them and runs the assembly again
   auto part_b = [&](const FillOut& o) -> int {
-    const bool lds = lib_n <= kLibLdsMax;
-    const unsigned grid = unsigned(lds ? ctx->fill_grid_lds : ctx->fill_grid);
+    const int pi = o.fp32 ? 1 : 0;
+    const bool lds = lib_n <= (o.fp32 ? FillShape<float>::kLibLds : FillShape<double>::kLibLds);
+    const unsigned grid = unsigned(lds ? ctx->fill_grid_lds[pi] : ctx->fill_grid[pi]);
     launch_fill(lds, o.a_is_cc != 0, o.b_is_cc != 0, grid, st, w, L, o);
     HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
     return FEM2D_OK;
