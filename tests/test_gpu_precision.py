@@ -117,3 +117,18 @@ def test_fp32_rayleigh_quotients_of_lowest_modes(ctx, name):
     print(f"[fp32] case={name} n={g32.n} 10 lowest modes {np.round(lam, 4).tolist()} "
           f"max Rayleigh-quotient error {worst:.2e}", flush=True)
     assert worst <= RQ_TOL, worst
+
+
+def test_fp32_full_size_download_is_rounded_fp64(ctx):
+    """The 25.7 M-entry benchmark GEP: the download streams it in 32 MiB chunks and widens FP32 on
+    the way; every value equals the FP64 value rounded to float, across all chunks."""
+    print("[fp32] full-size download", flush=True)
+    w = W.waveguide_64x32()
+    dd = F.DeviceDomain(w.domain, ctx)
+    g64 = dd.download(dd.assemble(w.glq))
+    g32 = dd.download(dd.assemble(w.glq, precision=F.FP32))
+    assert g32.nnz == g64.nnz == 25704606
+    np.testing.assert_array_equal(g32.row_ptr, g64.row_ptr)
+    np.testing.assert_array_equal(g32.col, g64.col)
+    for got, exp in ((g32.a_val, g64.a_val), (g32.b_val, g64.b_val)):
+        np.testing.assert_array_equal(got, exp.astype(np.float32).astype(np.float64))
