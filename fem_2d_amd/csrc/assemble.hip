@@ -1309,13 +1309,14 @@ __device__ __forceinline__ void flat_contrib(const double* __restrict__ lib, uns
   const int4 tail = reinterpret_cast<const int4*>(coef_g)[ci / 2u + 3u];
   const unsigned rpk = (tail.z & 1) ? rp.y : rp.x;
   const bool anc = (tail.z >> 1) == kAnc;
-  const unsigned rd = spec_dir(rpk), cd = spec_dir(cbs);
-  // the p side of the reference's pair is the ancestor's spec
-  const unsigned pd = anc ? cd : rd, qd = anc ? rd : cd;
+  // the p side of the reference's pair is the ancestor's spec (selected as
+  // whole spec words, so that the field extraction below is branch-free)
+  const unsigned ps = anc ? cbs : rpk, qs = anc ? rpk : cbs;
+  const unsigned pd = spec_dir(ps), qd = spec_dir(qs);
   const unsigned pair = 2u * pd + qd;
   // table row a = p's order, entry b = q's order
-  const unsigned ix = unsigned(tail.x) + (anc ? spec_ip1(cbs) + spec_i(rpk) : spec_ip1(rpk) + spec_i(cbs));
-  const unsigned iy = unsigned(tail.y) + (anc ? spec_jp1(cbs) + spec_j(rpk) : spec_jp1(rpk) + spec_j(cbs));
+  const unsigned ix = unsigned(tail.x) + spec_ip1(ps) + spec_i(qs);
+  const unsigned iy = unsigned(tail.y) + spec_jp1(ps) + spec_j(qs);
   const double fx = ld32(lib, ix + __umul24(pair, tab));
   const double fy = ld32(lib, iy + __umul24(3u - pair, tab));
   cc = coef_g[ci + pair] * fx * fy;
@@ -1329,34 +1330,43 @@ __device__ __forceinline__ void flat_contrib(const double* __restrict__ lib, uns
 // Entry p of a row group's segment: row q = max{q : off(q) <= p} with
 // off(q) = q T - q (q - 1) / 2, candidate column t = q + p - off(q). The float
 // root (all operands below 2^24, hardware square root) lands within one row of
-// q; one branch-free step each way fixes it, and the clamps keep (q, t) inside
-// the group's records whatever the rounding.
-__device__ __forceinline__ void segment_entry(int p, int T, int nrows, float bq, int& q, int& t) {
+// q; one branch-free step each way fixes it, and the clamp keeps q inside the
+// group's row records whatever the rounding. Returns q and oq = off(q).
+__device__ __forceinline__ void segment_row(int p, int T, int nrows, float bq, int& q, int& oq) {
   q = int((bq - __builtin_amdgcn_sqrtf(fmaxf(bq * bq - 8.0f * float(p), 0.0f))) * 0.5f);
   q = min(max(q, 0), nrows - 1);
   // off(q) = q (2T - q + 1) / 2; both factors stay below 2^24
-  int oq = int(__umul24(unsigned(q), unsigned(2 * T - q + 1)) >> 1);
+  oq = int(__umul24(unsigned(q), unsigned(2 * T - q + 1)) >> 1);
   const bool down = oq > p && q > 0;
   q -= down ? 1 : 0;
   oq -= down ? T - q : 0;
   const bool up = q + 1 < nrows && oq + (T - q) <= p;
   oq += up ? T - q : 0;
   q += up ? 1 : 0;
-  t = min(max(q + (p - oq), q), T - 1);
 }
 
-// Two entries of a row group's segment from the group's records (bases are
-// wave-uniform; offsets are 32-bit). The two are computed side by side with
+// Moves (q, oq) forward to the row of a later entry p (row q has T - q
+// entries; q stays below nrows, so the loop ends).
+__device__ __forceinline__ void segment_advance(int p, int T, int nrows, int& q, int& oq) {
+  while (q + 1 < nrows && oq + (T - q) <= p) {
+    oq += T - q;
+    ++q;
+  }
+}
+
+// candidate column of entry p in row q (row start oq), clamped to the group's
+// column records
+__device__ __forceinline__ int segment_col(int p, int T, int q, int oq) { return min(max(q + (p - oq), q), T - 1); }
+
+// Two entries (rows qa, qb; candidate columns ta, tb) of a row group's segment
+// from the group's records (bases are wave-uniform; offsets are 32-bit). The two are computed side by side with
 // no branch between them, so that their loads are in flight together; the
 // second contribution (both specs of the row and of the column on the same
 // two Elems, a few entries in a hundred) follows.
 __device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, const uint2* __restrict__ row_g,
                                             const double* __restrict__ coef_g, const double* __restrict__ lib,
-                                            unsigned tab, int pa, int pb, int T, int nrows, float bq, int& ca,
-                                            double& cca, double& l2a, int& cb, double& ccb, double& l2b) {
-  int qa, ta, qb, tb;
-  segment_entry(pa, T, nrows, bq, qa, ta);
-  segment_entry(pb, T, nrows, bq, qb, tb);
+                                            unsigned tab, int qa, int ta, int qb, int tb, int& ca, double& cca,
+                                            double& l2a, int& cb, double& ccb, double& l2b) {
   const uint4 ra = ld32(col_g, unsigned(ta));
   const uint4 rb = ld32(col_g, unsigned(tb));
   const uint2 rpa = ld32(row_g, unsigned(qa));
@@ -1451,13 +1461,23 @@ __global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fil
         wave_sync();
         const double* coef_g = reinterpret_cast<const double*>(stage);
         // two entries per lane and iteration; past the end of the group's
-        // part of the chunk the second one repeats the first and is not stored
+        // part of the chunk the second one repeats the first and is not stored.
+        // Each lane finds the row of its first entry from the root and then
+        // follows its entries (64 apart) row by row.
+        int qa, oqa;
+        segment_row(j0 + lane + off, T, nrows, bq, qa, oqa);
         for (int j = j0 + lane; j < j1; j += 2 * kWave) {
           int c0, c1;
           double cc0, l20, cc1, l21;
           const bool second = j + kWave < j1;
-          flat_entry2(col_g, row_g, coef_g, lib, tab, j + off, (second ? j + kWave : j) + off, T, nrows, bq, c0, cc0,
-                      l20, c1, cc1, l21);
+          const int pa = j + off, pb = (second ? j + kWave : j) + off;
+          segment_advance(pa, T, nrows, qa, oqa);
+          int qb = qa, oqb = oqa;
+          segment_advance(pb, T, nrows, qb, oqb);
+          flat_entry2(col_g, row_g, coef_g, lib, tab, qa, segment_col(pa, T, qa, oqa), qb, segment_col(pb, T, qb, oqb),
+                      c0, cc0, l20, c1, cc1, l21);
+          qa = qb;
+          oqa = oqb;
           st32(col_out, unsigned(j), c0);
           st32(a_out, unsigned(j), V(kACurlCurl ? cc0 : l20));
           st32(b_out, unsigned(j), V(kBCurlCurl ? cc0 : l20));
