@@ -67,7 +67,6 @@ constexpr int kWD = 128;           // distinct row groups in one candidate set
 constexpr int kWC = 256;           // candidate columns from the group's first row on
 constexpr int kWRows = 128;        // rows per row group
 constexpr int kCoef = 8;           // doubles per source record: CurlCurl prefactors (4), L2Inner (2), padding
-constexpr int kChunk = 256;        // CSR entries per work item of the streaming fill
 // block path (one block per row group)
 constexpr int kMaxSrc = 128;
 constexpr int kMaxRaw = 1024;
@@ -81,15 +80,18 @@ constexpr int kMaxRows = 1024;
 // latency-bound) keeps the three smaller blocks per CU.
 template <class V>
 struct FillShape;
+// kChunk: CSR entries per work item (measured: 256 for FP64, 512 for FP32).
 template <>
 struct FillShape<double> {
   static constexpr int kBlock = 1024;
   static constexpr int kLibLds = 12288;
+  static constexpr int kChunk = 256;
 };
 template <>
 struct FillShape<float> {
   static constexpr int kBlock = 512;
   static constexpr int kLibLds = 4096;
+  static constexpr int kChunk = 512;
 };
 constexpr int kFillWavesPerSimd = 6;
 // latency-bound kernels: at most 64 VGPRs, so that 8 wavefronts per SIMD are resident
@@ -186,6 +188,7 @@ struct Work {
   int64_t* seg_start;  // [G+1] first CSR entry of each row group
   int* chunk_first;    // [chunk_cap] row group holding the first entry of each chunk
   long long chunk_cap;
+  int chunk;           // CSR entries per chunk (the stored precision's FillShape::kChunk)
   long long n_dofs;
 };
 
@@ -834,7 +837,7 @@ struct NestedScan {
 // of a streaming-fill group holds T - q entries, so the segment is
 // nrows T - nrows (nrows - 1) / 2 long; a block-path group sums the row
 // lengths its count kernel wrote. The exclusive sum is the segment's first
-// entry. The store writes the group id for the chunks of kChunk entries that
+// entry. The store writes the group id for the chunks of w.chunk entries that
 // start inside the segment and, for a block-path group, row_ptr of its rows
 // (the streaming fill writes those of the other groups). The item past the
 // last group carries nnz and writes the call's status block (nnz, error bits,
@@ -875,8 +878,8 @@ struct GroupScan {
         at += w.row_len[r0 + q];
       }
     }
-    for (long long c = (excl + kChunk - 1) / kChunk; c * kChunk < excl + len && c < w.chunk_cap; ++c)
-      w.chunk_first[c] = int(g);
+    const long long C = w.chunk;
+    for (long long c = (excl + C - 1) / C; c * C < excl + len && c < w.chunk_cap; ++c) w.chunk_first[c] = int(g);
   }
 };
 
@@ -1389,7 +1392,8 @@ __device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, con
   }
 }
 
-// Streaming fill: the CSR is cut into chunks of kChunk consecutive entries;
+// Streaming fill: the CSR is cut into chunks of kChunk consecutive entries
+// (w.chunk, which the layout scan used, equals FillShape<V>::kChunk);
 // wavefront v takes chunks v, v + waves, ..., so that the wavefronts write
 // neighbouring chunks at any time. For each row group a chunk meets, the
 // wavefront stages the group's source records in LDS and writes the entries
@@ -1428,6 +1432,7 @@ __global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fil
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
   const int waves = int((gridDim.x * blockDim.x) / kWave);
+  constexpr int kChunk = FillShape<V>::kChunk;
   const long long n_chunks = min((nnz + kChunk - 1) / kChunk, w.chunk_cap);
   const double* __restrict__ lib = kLibInLds ? lib_lds : w.lib;
   const unsigned tab = unsigned(L.tab);
@@ -2481,7 +2486,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     o.capacity = ctx->out_capacity;
     o.a_is_cc = ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
     o.b_is_cc = bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
-    w.chunk_cap = o.capacity / kChunk + 1;
+    w.chunk = o.fp32 ? FillShape<float>::kChunk : FillShape<double>::kChunk;
+    w.chunk_cap = o.capacity / w.chunk + 1;
     if ((rc = slot(ctx, chunk_slot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
     const bool fill = o.capacity > 0;
     const bool tail = fill && run_large;  // part (c) has work
