@@ -80,7 +80,9 @@ constexpr int kMaxRows = 1024;
 // latency-bound) keeps the three smaller blocks per CU.
 template <class V>
 struct FillShape;
-// kChunk: CSR entries per work item (measured: 256 for FP64, 512 for FP32).
+// kChunk: CSR entries per work item. 256 for both: 512 made the FP32 fill
+// faster on Domains of large row groups and slower on Domains of small ones,
+// and the FP64 fill slower on all that were measured.
 template <>
 struct FillShape<double> {
   static constexpr int kBlock = 1024;
@@ -91,7 +93,7 @@ template <>
 struct FillShape<float> {
   static constexpr int kBlock = 512;
   static constexpr int kLibLds = 4096;
-  static constexpr int kChunk = 512;
+  static constexpr int kChunk = 256;
 };
 constexpr int kFillWavesPerSimd = 6;
 // latency-bound kernels: at most 64 VGPRs, so that 8 wavefronts per SIMD are resident
