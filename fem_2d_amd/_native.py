@@ -68,6 +68,7 @@ class GepDevice(C.Structure):
         ("precision", C.c_int32),
         ("a_val32", C.POINTER(C.c_float)),
         ("b_val32", C.POINTER(C.c_float)),
+        ("generation", C.c_uint64),
     ]
 
 

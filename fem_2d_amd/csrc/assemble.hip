@@ -1686,8 +1686,19 @@ __global__ void __launch_bounds__(kBlock) k_count_large(DevView v, Work w) {
   }
 }
 
+// Entries of the block-path row groups. Everything this kernel reads about
+// the CSR layout was written by the layout scan of the same assembly: the
+// group's segment [seg_start[g], seg_start[g + 1]) and the row pointers of its
+// own rows (the layout scan writes those of block-path groups; the row
+// pointers of the other groups are written by the streaming fill, which skips
+// that work on an attempt whose output buffers are too small). Every candidate
+// index is checked against the group's candidate set and the source Elem's
+// spec list before it is used; a violation sets kErrView and the entry is
+// skipped.
 __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibShape L, FillOut o) {
   if (*w.err) return;  // an invalid view was detected: nothing below may trust it
+  // the host grows the buffers and runs the assembly again (as k_fill_flat)
+  if (w.row_ptr[w.n_dofs] > o.capacity) return;
   extern __shared__ unsigned char smem[];
   GroupShared& S = *reinterpret_cast<GroupShared*>(smem);
   const int64_t* __restrict__ row_ptr = w.row_ptr;
@@ -1698,10 +1709,18 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
     const int nrows = r1 - r0;
     if (nrows > kMaxRows) continue;
     if (!build_group<true>(v, w, g, S)) continue;
-    const long long seg0 = row_ptr[r0];
+    const long long seg0 = w.seg_start[g];
+    const long long seg_len = w.seg_start[g + 1] - seg0;
+    const int T = S.total - S.u_self;  // candidate columns of the group's first row
+    // the segment holds rows of T, T - 1, ... entries (block-uniform check)
+    if (seg_len != (long long)nrows * T - (long long)nrows * (nrows - 1) / 2 || nrows > T) {
+      if (threadIdx.x == 0) set_err(w.err, kErrView);
+      __syncthreads();
+      continue;
+    }
     const int e_k[2] = {w.g_e1[g], w.g_e2[g]};
     for (int q = threadIdx.x; q <= nrows; q += blockDim.x) {
-      S.row_rel[q] = int(row_ptr[r0 + q] - seg0);
+      S.row_rel[q] = q < nrows ? int(row_ptr[r0 + q] - seg0) : int(seg_len);
       if (q < nrows) {
         const int r = r0 + q;
         S.row_bs[0][q] = w.bs_pack[w.dof_t1[r]];
@@ -1709,7 +1728,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
       }
     }
     __syncthreads();
-    const int seg_n = S.row_rel[nrows];
+    const int seg_n = int(seg_len);
     const int n_dist = S.n_dist;
     for (int p = threadIdx.x; p < seg_n; p += blockDim.x) {
       int lo = 0, hi = nrows;  // row_rel[lo] <= p < row_rel[hi]
@@ -1718,6 +1737,10 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
         if (S.row_rel[mid] <= p) lo = mid; else hi = mid;
       }
       const int u = S.u_self + lo + (p - S.row_rel[lo]);
+      if (u < S.u_self || u >= S.total) {  // outside the candidate set: the layout disagrees with it
+        set_err(w.err, kErrView);
+        continue;
+      }
       int a = 0, b = n_dist;
       while (b - a > 1) {
         const int mid = (a + b) >> 1;
@@ -1726,6 +1749,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
       const int within = u - S.cum[a];
       const int c = S.dstart[a] + within;
       double cc_sum = 0.0, l2_sum = 0.0;
+      bool ok = true;
       for (int side = 0; side < 2; ++side) {
         const int sidx = side == 0 ? S.ds0[a] : S.ds1[a];
         if (sidx < 0) break;
@@ -1733,7 +1757,13 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
         const int k = info & 1, rel = (info >> 1) & 3;
         const int ru = (info >> 3) & 0x3fff, rv = (info >> 17) & 0x3fff;
         const int off = side == 0 ? S.doff0[a] : S.doff1[a];
-        const unsigned cbs = w.bs_pack[v.bs_off[S.src_x[sidx]] + off + within];
+        const int X = S.src_x[sidx];
+        const long long t = v.bs_off[X] + off + within;
+        if (t >= v.bs_off[X + 1]) {  // past the source Elem's spec list
+          ok = false;
+          break;
+        }
+        const unsigned cbs = w.bs_pack[t];
         const unsigned rbs = S.row_bs[k][lo];
         double cc, l2;
         const unsigned bx = unsigned(ru * L.range_stride), by = unsigned((L.nr + rv) * L.range_stride);
@@ -1741,6 +1771,10 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
         else pair_values(w.lib, unsigned(L.tab), S.cA[sidx], S.cB[sidx], bx, by, rbs, cbs, cc, l2);
         cc_sum += cc;
         l2_sum += l2;
+      }
+      if (!ok) {
+        set_err(w.err, kErrView);
+        continue;
       }
       write_entry(o, seg0 + p, c, cc_sum, l2_sum);
     }
@@ -1943,9 +1977,27 @@ struct fem2d_ctx {
   hipGraphExec_t graph_exec[1] = {nullptr};  // the last recorded assembly
   GraphKey graph_key;
   void* solver = nullptr;  // eigensolver handle, made by the first solve
+  // generation of the latest assembly whose outputs the buffers hold (0: none)
+  unsigned long long out_generation = 0;
 };
 
+namespace {
+// generations of device GEP handles, unique in the process
+std::atomic<unsigned long long> g_gep_generation{0};
+}  // namespace
+
 int fem2d::capi::ctx_device(fem2d_ctx* ctx) { return ctx->device; }
+
+int fem2d::capi::check_gep_handle(fem2d_ctx* ctx, const fem2d_gep_device* gep) {
+  if (gep->generation != 0 && gep->generation != ctx->out_generation) {
+    fem2d::capi::set_error(
+        "stale device GEP handle: its buffers belong to an earlier assembly (or another context) and may have "
+        "been overwritten or freed",
+        0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  return FEM2D_OK;
+}
 void** fem2d::capi::ctx_solver_slot(fem2d_ctx* ctx) { return &ctx->solver; }
 
 namespace {
@@ -2293,6 +2345,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     return FEM2D_ERR_INVALID_VIEW;
   }
   HIP_TRY(hipSetDevice(ctx->device));
+  // whatever happens below, the output buffers no longer hold the previous
+  // assembly's GEP: its handles become stale
+  ctx->out_generation = 0;
   const int ne = dd->n_elems;
   const long long nd = dd->n_dofs;
   const long long nbs = dd->n_bs;
@@ -2575,6 +2630,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   out->b_val = fp32 ? nullptr : static_cast<double*>(ctx->out_b.p);
   out->a_val32 = fp32 ? static_cast<float*>(ctx->out_a.p) : nullptr;
   out->b_val32 = fp32 ? static_cast<float*>(ctx->out_b.p) : nullptr;
+  ctx->out_generation = ++g_gep_generation;
+  out->generation = ctx->out_generation;
   return FEM2D_OK;
 }
 
@@ -2628,11 +2685,6 @@ int fem2d_device_domain_num_leaves(const fem2d_device_domain* dd, int64_t* out) 
   return FEM2D_OK;
 }
 
-// The field kernel is not launched: the first full GPU test run that
-// included it ended with the GPU faulted in a later assembly test, with no
-// cause shown yet; k_xy_fields stays in the source, unlaunched.
-constexpr bool kFieldKernelOff = true;
-
 int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int density_u, int density_v,
                            const double* solution, int64_t n_solution, double* out_x, double* out_y) {
   if (!ctx || !dd) {
@@ -2664,10 +2716,6 @@ int fem2d_device_xy_fields(fem2d_ctx* ctx, const fem2d_device_domain* dd, int ba
   const int d = density_u;
   const long long total = dd->n_leaves * (long long)d * (d > 0 && density_v > 0 ? d : 0);
   if (total == 0) return FEM2D_OK;
-  if (kFieldKernelOff) {
-    set_error("field evaluation on the GPU is switched off in this build", 0);
-    return FEM2D_ERR_UNSUPPORTED;
-  }
   if (!solution || !out_x || !out_y) {
     set_error("null array", 0);
     return FEM2D_ERR_INVALID_ARG;
@@ -2768,6 +2816,7 @@ int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d
     return FEM2D_ERR_INVALID_ARG;
   }
   std::memset(out, 0, sizeof(*out));
+  if (int rc = fem2d::capi::check_gep_handle(ctx, src)) return rc;
   out->n = src->n;
   out->nnz = src->nnz;
   out->row_ptr = static_cast<int64_t*>(std::malloc(size_t(src->n + 1) * sizeof(int64_t)));

@@ -222,6 +222,10 @@ int check_device_args(fem2d_ctx* ctx, const fem2d_gep_device* gep, fem2d_eigenpa
     set_error("null argument", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
+  out->value = 0.0;
+  out->n = 0;
+  out->vector = nullptr;
+  if (int rc = fem2d::capi::check_gep_handle(ctx, gep)) return rc;
   if (gep->n < 0) {
     set_error("negative GEP size", 0);
     return FEM2D_ERR_INVALID_ARG;

@@ -4,6 +4,7 @@
 #include <string>
 
 struct fem2d_ctx;
+struct fem2d_gep_device;
 
 namespace fem2d {
 
@@ -20,6 +21,10 @@ int ctx_device(fem2d_ctx* ctx);
 void** ctx_solver_slot(fem2d_ctx* ctx);
 // destroys a handle held in that slot
 void release_solver(void* handle);
+// FEM2D_OK for a device GEP handle that may be read on this context: one the
+// caller built over arrays of its own (generation 0), or the latest assembly
+// of this context; FEM2D_ERR_INVALID_ARG (message set) for a stale handle
+int check_gep_handle(fem2d_ctx* ctx, const fem2d_gep_device* gep);
 }  // namespace capi
 
 }  // namespace fem2d

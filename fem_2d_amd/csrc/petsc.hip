@@ -188,6 +188,7 @@ int fem2d_gep_device_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep
     set_error("null argument", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
+  if (int rc = fem2d::capi::check_gep_handle(ctx, gep)) return rc;
   const long long n = gep->n, nnz = gep->nnz;
   if (n < 0 || nnz < 0 || n >= 0xffffffffLL || 2 * nnz >= 0xffffffffLL) {
     set_error("matrix too large for the PETSc binary format's 32-bit sizes", 0);

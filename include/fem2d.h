@@ -164,7 +164,12 @@ void fem2d_gep_csr_free(fem2d_gep_csr* gep);
 typedef struct fem2d_device_domain fem2d_device_domain;
 
 /* Device pointers owned by the context; valid until the next assembly on that
- * context or its destruction. */
+ * context or its destruction. fem2d_assemble_device stamps the handle with the
+ * assembly's generation (unique in the process); every function that takes a
+ * handle rejects one whose generation is not the context's latest assembly
+ * with FEM2D_ERR_INVALID_ARG (a stale handle, or one from another context).
+ * Generation 0 marks a handle the caller built over device arrays of its own;
+ * those are not checked. */
 typedef struct fem2d_gep_device {
   int64_t n;
   int64_t nnz;
@@ -175,6 +180,7 @@ typedef struct fem2d_gep_device {
   int32_t precision; /* fem2d_precision of the values */
   float* a_val32;    /* FEM2D_PRECISION_FP32 values (NULL for FP64) */
   float* b_val32;
+  uint64_t generation; /* assembly that produced the handle; 0 for caller-owned arrays */
 } fem2d_gep_device;
 
 int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* domain, fem2d_device_domain** out);

@@ -14,11 +14,7 @@ import oracle
 from fem_2d_amd import workloads as W
 from test_fields_oracle import _leaf_boxes, tangential_jump
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the GPU field path is switched off: the first full GPU run that included these "
-                            "tests ended with the GPU faulted in a later assembly test, cause not shown"),
-]
+pytestmark = pytest.mark.gpu
 
 FIELD_TOL = 1e-13
 

@@ -65,6 +65,64 @@ def test_deterministic_and_device_resident(ctx):
         assert np.array_equal(g1.b_val.view(np.uint64), g.b_val.view(np.uint64))
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_block_path_after_smaller_domain(graphs):
+    """Regression: a Domain with block-path row groups assembled on a context whose output
+    buffers were sized for a smaller Domain. The first attempt finds them too small; its
+    block-path fill must not read row pointers that only the streaming fill writes (and that
+    fill skips that attempt). The result equals the oracle's, also when the buffers grow twice."""
+    print(f"[regress] block path after a smaller Domain, graphs={graphs}", flush=True)
+    c = F.Context(0)
+    c.set_graphs(graphs)
+    small, sglq = cases.nalg_problem()
+    F.galerkin_sample_gep_hcurl(small, sglq, ctx=c)
+    mid, mglq = cases.slepc_problem()
+    F.galerkin_sample_gep_hcurl(mid, mglq, F.KOLShapeFn, ctx=c)
+    dom, glq = cases.small_hp_irregular()
+    gep = F.galerkin_sample_gep_hcurl(dom, glq, F.KOLShapeFn, ctx=c)
+    assert c.last_timing()["n_large"] > 0, "the case must exercise the block path"
+    rc, ref = oracle.assemble(dom.view(), basis=1, glq=glq, n_threads=8)
+    assert rc == 0
+    assert_gep_matches(gep, ref)
+    # and back to the small Domain on the grown buffers
+    g_small = F.galerkin_sample_gep_hcurl(small, sglq, ctx=c)
+    rc, ref_small = oracle.assemble(small.view(), basis=0, glq=sglq)
+    assert rc == 0
+    assert_gep_matches(g_small, ref_small)
+    c.close()
+
+
+def test_stale_device_handle_rejected(ctx):
+    """A device GEP handle is valid until the next assembly on its context: afterwards the
+    download, the PETSc writer and the eigensolves reject it (the buffers may have been
+    reused or freed), while the latest handle works."""
+    dom, glq = cases.slepc_problem()
+    other, oglq = cases.small_hp_irregular()
+    dd = F.DeviceDomain(dom, ctx)
+    od = F.DeviceDomain(other, ctx)
+    h1 = dd.assemble(glq)
+    assert h1.generation != 0
+    h2 = od.assemble(oglq, F.KOLShapeFn)  # grows the buffers: h1's memory is gone
+    assert h2.generation != h1.generation
+    with pytest.raises(ValueError, match="stale"):
+        dd.download(h1)
+    with pytest.raises(ValueError, match="stale"):
+        F.solve_gep_dense(h1, 1.47, ctx=ctx)
+    with pytest.raises(ValueError, match="stale"):
+        F.nalgebra_solve_gep(h1, 1.47, ctx=ctx)
+    with pytest.raises(ValueError, match="stale"):
+        dd.print_to_petsc_binary_files(h1, "/nonexistent-dir", "x")
+    # a handle from another context is stale on this one
+    c2 = F.Context(0)
+    h3 = F.DeviceDomain(dom, c2).assemble(glq)
+    with pytest.raises(ValueError, match="stale"):
+        dd.download(h3)
+    c2.close()
+    # the latest handle of this context is good
+    g = od.download(h2)
+    assert g.nnz == h2.nnz
+
+
 def _same_bits(g, ref):
     np.testing.assert_array_equal(g.row_ptr, ref.row_ptr)
     np.testing.assert_array_equal(g.col, ref.col)
