@@ -12,7 +12,7 @@ from .errors import (EmptyDOFSet, FailedToConverge, FailedToInvertB, Fem2dError,
                      NalgebraGEPError, NativeError, PRefError, ProblemTooLarge, SpuriouslyConverged,
                      UniformFieldError, WrongContinuityCondition)
 from .fields import UniformFieldSpace, xy_fields_array
-from .galerkin import (FP32, FP64, GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace,
+from .galerkin import (FP32, FP32R, FP64, GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace,
                        HierCurlIntegral, HierPoly, KOLShapeFn, L2Inner, ShapeFn, default_context,
                        galerkin_sample_gep_hcurl, gauss_quadrature_points)
 from .linalg import EigenPair, nalgebra_solve_gep, solve_gep_dense
@@ -22,7 +22,7 @@ from .mesh import (MAX_POLYNOMIAL_ORDER, MIN_EDGE_LENGTH, Edge, Elem, HLevels, H
 __all__ = [
     "BasisDir", "BasisSpec", "ContinuityCondition", "Domain", "EmptyDOFSet", "Fem2dError", "GalerkinSamplingError",
     "HRefError", "InvalidGLQSettings", "MeshAccessError", "MeshPanic", "NativeError", "PRefError",
-    "WrongContinuityCondition", "FP32", "FP64", "GEP", "MIN_GLQ_ORDER", "Context", "CurlCurl", "DeviceDomain", "HierCurlBasisFnSpace",
+    "WrongContinuityCondition", "FP32", "FP32R", "FP64", "GEP", "MIN_GLQ_ORDER", "Context", "CurlCurl", "DeviceDomain", "HierCurlBasisFnSpace",
     "HierCurlIntegral", "HierPoly", "KOLShapeFn", "L2Inner", "ShapeFn", "default_context",
     "galerkin_sample_gep_hcurl", "gauss_quadrature_points", "MAX_POLYNOMIAL_ORDER", "MIN_EDGE_LENGTH", "Edge", "Elem",
     "HLevels", "HRef", "Mesh", "Node", "Orders", "PolyOrders", "PRef", "EigenPair", "nalgebra_solve_gep",

@@ -61,6 +61,7 @@ class GepCsr(C.Structure):
 
 PRECISION_FP64 = 0
 PRECISION_FP32 = 1
+PRECISION_FP32R = 2
 
 
 class GepDevice(C.Structure):
@@ -69,6 +70,7 @@ class GepDevice(C.Structure):
         ("a_val32", C.POINTER(C.c_float)),
         ("b_val32", C.POINTER(C.c_float)),
         ("generation", C.c_uint64),
+        ("res16", C.POINTER(C.c_uint32)),
     ]
 
 

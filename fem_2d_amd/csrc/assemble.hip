@@ -50,6 +50,7 @@
 #include "../../include/fem2d.h"
 #include "host/capi_internal.hpp"
 #include "host/parallel.hpp"
+#include "fp32r.hpp"
 #include "pinned_staging.hpp"
 
 namespace {
@@ -72,27 +73,44 @@ constexpr int kMaxSrc = 128;
 constexpr int kMaxRaw = 1024;
 constexpr int kMaxDist = 1024;
 constexpr int kMaxRows = 1024;
-// streaming fill: the wavefronts of a block share one LDS copy of the Gram
-// library (up to kLibLds values) beside their staged source records. FP64
-// output: 16 wavefronts and up to 96 KB of library (orders to 8 over three
-// levels of refinement), one block per CU, measured faster than three
-// 8-wavefront blocks with a 32 KB library; FP32 output (12 B per entry, more
-// latency-bound) keeps the three smaller blocks per CU.
-template <class V>
-struct FillShape;
-// kChunk: CSR entries per work item. 256 for both: 512 made the FP32 fill
-// faster on Domains of large row groups and slower on Domains of small ones,
-// and the FP64 fill slower on all that were measured.
+// Stored precision of the fill's output (fem2d_precision): the value types of
+// A and B and the streaming fill's launch shape. The wavefronts of a block
+// share one LDS copy of the Gram library (up to kLibLds values) beside their
+// staged source records. FP64 output: 16 wavefronts and up to 96 KB of
+// library (orders to 8 over three levels of refinement), one block per CU,
+// measured faster than three 8-wavefront blocks with a 32 KB library; FP32
+// output (12 B per entry, more latency-bound) keeps the three smaller blocks
+// per CU. kChunk: CSR entries per work item (256 for all; 512 made the FP32
+// fill faster on Domains of large row groups and slower on Domains of small
+// ones, and the FP64 fill slower on all that were measured).
+constexpr int kPrecisions = 3;
+template <int kPrecision>
+struct Store;
 template <>
-struct FillShape<double> {
+struct Store<FEM2D_PRECISION_FP64> {
+  using A = double;
+  using B = double;
+  static constexpr bool kRes = false;  // 16-bit residuals stored beside the values (FP32R)
   static constexpr int kBlock = 1024;
   static constexpr int kLibLds = 12288;
   static constexpr int kChunk = 256;
 };
 template <>
-struct FillShape<float> {
+struct Store<FEM2D_PRECISION_FP32> {
+  using A = float;
+  using B = float;
+  static constexpr bool kRes = false;
   static constexpr int kBlock = 512;
   static constexpr int kLibLds = 4096;
+  static constexpr int kChunk = 256;
+};
+template <>
+struct Store<FEM2D_PRECISION_FP32R> {  // 16 B per entry: the FP64 shape
+  using A = float;
+  using B = float;
+  static constexpr bool kRes = true;
+  static constexpr int kBlock = 1024;
+  static constexpr int kLibLds = 12288;
   static constexpr int kChunk = 256;
 };
 constexpr int kFillWavesPerSimd = 6;
@@ -190,7 +208,7 @@ struct Work {
   int64_t* seg_start;  // [G+1] first CSR entry of each row group
   int* chunk_first;    // [chunk_cap] row group holding the first entry of each chunk
   long long chunk_cap;
-  int chunk;           // CSR entries per chunk (the stored precision's FillShape::kChunk)
+  int chunk;           // CSR entries per chunk (the stored precision's Store::kChunk)
   long long n_dofs;
 };
 
@@ -210,28 +228,13 @@ struct LibShape {
 
 struct FillOut {
   int* col;
-  double* aval;  // FP64 values (these and the FP32 ones share the output buffers)
-  double* bval;
-  float* aval32;  // FP32 values
-  float* bval32;
+  void* aval;  // Store<precision>::A values
+  void* bval;  // Store<precision>::B values
+  unsigned* res;  // FP32R: the two 16-bit residuals of each entry (fp32r_pack)
   long long capacity;  // entries the output buffers hold
   int a_is_cc, b_is_cc;
-  int fp32;
+  int precision;  // fem2d_precision
 };
-
-// the value arrays of one precision
-template <class V>
-__device__ __forceinline__ V* out_a(const FillOut& o);
-template <>
-__device__ __forceinline__ double* out_a<double>(const FillOut& o) { return o.aval; }
-template <>
-__device__ __forceinline__ float* out_a<float>(const FillOut& o) { return o.aval32; }
-template <class V>
-__device__ __forceinline__ V* out_b(const FillOut& o);
-template <>
-__device__ __forceinline__ double* out_b<double>(const FillOut& o) { return o.bval; }
-template <>
-__device__ __forceinline__ float* out_b<float>(const FillOut& o) { return o.bval32; }
 
 __device__ __forceinline__ void set_err(int* err, int bits) { atomicOr(err, bits); }
 
@@ -1293,12 +1296,24 @@ __device__ __forceinline__ void write_entry(const FillOut& o, long long pos, int
   if (pos >= 0 && pos < o.capacity) {
     o.col[pos] = c;
     const double a = o.a_is_cc ? cc : l2, b = o.b_is_cc ? cc : l2;
-    if (o.fp32) {
-      o.aval32[pos] = float(a);
-      o.bval32[pos] = float(b);
-    } else {
-      o.aval[pos] = a;
-      o.bval[pos] = b;
+    switch (o.precision) {
+      case FEM2D_PRECISION_FP64:
+        static_cast<double*>(o.aval)[pos] = a;
+        static_cast<double*>(o.bval)[pos] = b;
+        break;
+      case FEM2D_PRECISION_FP32:
+        static_cast<float*>(o.aval)[pos] = float(a);
+        static_cast<float*>(o.bval)[pos] = float(b);
+        break;
+      default: {  // FEM2D_PRECISION_FP32R
+        float ha, hb;
+        int qa, qb;
+        fem2d::fp32r_split(a, ha, qa);
+        fem2d::fp32r_split(b, hb, qb);
+        static_cast<float*>(o.aval)[pos] = ha;
+        static_cast<float*>(o.bval)[pos] = hb;
+        o.res[pos] = fem2d::fp32r_pack(qa, qb);
+      }
     }
   }
 }
@@ -1394,8 +1409,30 @@ __device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, con
   }
 }
 
+// Entry j of a chunk in the stored precision P, as three (four for FP32R)
+// coalesced, non-temporal stores; FP32R splits each value into its FP32 value
+// and 16-bit residual (fp32r.hpp)
+template <int P>
+__device__ __forceinline__ void store_entry(int* __restrict__ col_out, typename Store<P>::A* __restrict__ a_out,
+                                            typename Store<P>::B* __restrict__ b_out, unsigned* __restrict__ r_out,
+                                            unsigned j, int c, double a, double b) {
+  st32(col_out, j, c);
+  if constexpr (Store<P>::kRes) {
+    float ha, hb;
+    int qa, qb;
+    fem2d::fp32r_split(a, ha, qa);
+    fem2d::fp32r_split(b, hb, qb);
+    st32(a_out, j, ha);
+    st32(b_out, j, hb);
+    st32(r_out, j, fem2d::fp32r_pack(qa, qb));
+  } else {
+    st32(a_out, j, typename Store<P>::A(a));
+    st32(b_out, j, typename Store<P>::B(b));
+  }
+}
+
 // Streaming fill: the CSR is cut into chunks of kChunk consecutive entries
-// (w.chunk, which the layout scan used, equals FillShape<V>::kChunk);
+// (w.chunk, which the layout scan used, equals Store<P>::kChunk);
 // wavefront v takes chunks v, v + waves, ..., so that the wavefronts write
 // neighbouring chunks at any time. For each row group a chunk meets, the
 // wavefront stages the group's source records in LDS and writes the entries
@@ -1403,15 +1440,17 @@ __device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, con
 // stores), reading the group's column and row records and the Gram library (a
 // per-block LDS copy when it fits). Entries of block-path groups are left to
 // k_fill_large. kACurlCurl / kBCurlCurl: A and B take the CurlCurl values
-// (else L2Inner); V: the stored precision (double, or float for
-// FEM2D_PRECISION_FP32; the arithmetic is FP64 either way).
-template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl, class V>
-__global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
+// (else L2Inner); P: the stored precision (Store<P>; the arithmetic is FP64
+// in every precision, values are rounded once at the store).
+template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl, int P>
+__global__ void __launch_bounds__(Store<P>::kBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
+  using VA = typename Store<P>::A;
+  using VB = typename Store<P>::B;
   if (*w.err) return;
   // the source records of the row group being written, per wavefront
-  __shared__ double4 src_lds[FillShape<V>::kBlock / kWave][kWS * kCoef / 4];
+  __shared__ double4 src_lds[Store<P>::kBlock / kWave][kWS * kCoef / 4];
   // the block's copy of the Gram library (when it fits)
-  __shared__ double lib_lds[kLibInLds ? FillShape<V>::kLibLds : 1];
+  __shared__ double lib_lds[kLibInLds ? Store<P>::kLibLds : 1];
   const long long nnz = w.row_ptr[w.n_dofs];
   if (nnz > o.capacity) return;  // the host grows the buffers and runs the fill again
   if (kLibInLds) {
@@ -1434,7 +1473,7 @@ __global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fil
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
   const int waves = int((gridDim.x * blockDim.x) / kWave);
-  constexpr int kChunk = FillShape<V>::kChunk;
+  constexpr int kChunk = Store<P>::kChunk;
   const long long n_chunks = min((nnz + kChunk - 1) / kChunk, w.chunk_cap);
   const double* __restrict__ lib = kLibInLds ? lib_lds : w.lib;
   const unsigned tab = unsigned(L.tab);
@@ -1443,8 +1482,9 @@ __global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fil
     const int len = int(min((long long)kChunk, nnz - pbase));
     const long long pb = uniform(pbase);
     int* __restrict__ col_out = o.col + pb;
-    V* __restrict__ a_out = out_a<V>(o) + pb;
-    V* __restrict__ b_out = out_b<V>(o) + pb;
+    VA* __restrict__ a_out = static_cast<VA*>(o.aval) + pb;
+    VB* __restrict__ b_out = static_cast<VB*>(o.bval) + pb;
+    unsigned* __restrict__ r_out = Store<P>::kRes ? o.res + pb : nullptr;
     // per-group values are wave-uniform: keep them (and the record bases) in
     // scalar registers
     int g = uniform(w.chunk_first[ch]);
@@ -1485,14 +1525,11 @@ __global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fil
                       c0, cc0, l20, c1, cc1, l21);
           qa = qb;
           oqa = oqb;
-          st32(col_out, unsigned(j), c0);
-          st32(a_out, unsigned(j), V(kACurlCurl ? cc0 : l20));
-          st32(b_out, unsigned(j), V(kBCurlCurl ? cc0 : l20));
-          if (second) {
-            st32(col_out, unsigned(j + kWave), c1);
-            st32(a_out, unsigned(j + kWave), V(kACurlCurl ? cc1 : l21));
-            st32(b_out, unsigned(j + kWave), V(kBCurlCurl ? cc1 : l21));
-          }
+          store_entry<P>(col_out, a_out, b_out, r_out, unsigned(j), c0, kACurlCurl ? cc0 : l20,
+                         kBCurlCurl ? cc0 : l20);
+          if (second)
+            store_entry<P>(col_out, a_out, b_out, r_out, unsigned(j + kWave), c1, kACurlCurl ? cc1 : l21,
+                           kBCurlCurl ? cc1 : l21);
         }
         wave_sync();  // every lane is done with the staged records
       }
@@ -1503,24 +1540,40 @@ __global__ void __launch_bounds__(FillShape<V>::kBlock, kFillWavesPerSimd) k_fil
 }
 
 // one instance per library placement, integral pair and stored precision
-template <bool kLibInLds, class V>
+template <bool kLibInLds, int P>
 void launch_fill_lib(bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
                      const FillOut& o) {
-  constexpr int B = FillShape<V>::kBlock;
-  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false, V><<<grid, B, 0, st>>>(w, L, o);
-  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true, V><<<grid, B, 0, st>>>(w, L, o);
-  else if (a_cc) k_fill_flat<kLibInLds, true, true, V><<<grid, B, 0, st>>>(w, L, o);
-  else k_fill_flat<kLibInLds, false, false, V><<<grid, B, 0, st>>>(w, L, o);
+  constexpr int B = Store<P>::kBlock;
+  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false, P><<<grid, B, 0, st>>>(w, L, o);
+  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true, P><<<grid, B, 0, st>>>(w, L, o);
+  else if (a_cc) k_fill_flat<kLibInLds, true, true, P><<<grid, B, 0, st>>>(w, L, o);
+  else k_fill_flat<kLibInLds, false, false, P><<<grid, B, 0, st>>>(w, L, o);
+}
+
+template <int P>
+void launch_fill_prec(bool lds, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w,
+                      const LibShape& L, const FillOut& o) {
+  if (lds) launch_fill_lib<true, P>(a_cc, b_cc, grid, st, w, L, o);
+  else launch_fill_lib<false, P>(a_cc, b_cc, grid, st, w, L, o);
 }
 
 void launch_fill(bool lds, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
                  const FillOut& o) {
-  if (lds) {
-    if (o.fp32) launch_fill_lib<true, float>(a_cc, b_cc, grid, st, w, L, o);
-    else launch_fill_lib<true, double>(a_cc, b_cc, grid, st, w, L, o);
-  } else {
-    if (o.fp32) launch_fill_lib<false, float>(a_cc, b_cc, grid, st, w, L, o);
-    else launch_fill_lib<false, double>(a_cc, b_cc, grid, st, w, L, o);
+  switch (o.precision) {
+    case FEM2D_PRECISION_FP32: launch_fill_prec<FEM2D_PRECISION_FP32>(lds, a_cc, b_cc, grid, st, w, L, o); break;
+    case FEM2D_PRECISION_FP32R:
+      launch_fill_prec<FEM2D_PRECISION_FP32R>(lds, a_cc, b_cc, grid, st, w, L, o);
+      break;
+    default: launch_fill_prec<FEM2D_PRECISION_FP64>(lds, a_cc, b_cc, grid, st, w, L, o);
+  }
+}
+
+// the library fits the fill's LDS copy
+bool lib_in_lds(int precision, long long lib_n) {
+  switch (precision) {
+    case FEM2D_PRECISION_FP32: return lib_n <= Store<FEM2D_PRECISION_FP32>::kLibLds;
+    case FEM2D_PRECISION_FP32R: return lib_n <= Store<FEM2D_PRECISION_FP32R>::kLibLds;
+    default: return lib_n <= Store<FEM2D_PRECISION_FP64>::kLibLds;
   }
 }
 
@@ -1962,6 +2015,7 @@ struct fem2d_ctx {
   hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<DevBuf> bufs;  // grow-only work buffers, by slot
   DevBuf out_col, out_a, out_b;
+  DevBuf out_r;  // FP32R residuals (grown on the first FP32R assembly that needs them)
   DevBuf field_scratch;  // grid points and status word of the field evaluation
   long long out_capacity = 0;
   long long* h_status = nullptr;  // mapped pinned status block: nnz, err, n_large, n_groups
@@ -1971,8 +2025,10 @@ struct fem2d_ctx {
   fem2d_timing timing{};
   int n_cu = 0;
   int count_grid = 0;  // resident blocks of the wave kernels on this device
-  int fill_grid[2] = {0, 0};      // resident blocks of the streaming fill (library in global memory), FP64 / FP32
-  int fill_grid_lds[2] = {0, 0};  // the same with the library in LDS
+  // resident blocks of the streaming fill per stored precision, with the library in
+  // global memory / in LDS
+  int fill_grid[kPrecisions] = {};
+  int fill_grid_lds[kPrecisions] = {};
   bool use_graphs = true;
   hipGraphExec_t graph_exec[1] = {nullptr};  // the last recorded assembly
   GraphKey graph_key;
@@ -1984,6 +2040,7 @@ struct fem2d_ctx {
 namespace {
 // generations of device GEP handles, unique in the process
 std::atomic<unsigned long long> g_gep_generation{0};
+
 }  // namespace
 
 int fem2d::capi::ctx_device(fem2d_ctx* ctx) { return ctx->device; }
@@ -2020,6 +2077,11 @@ int hip_fail(hipError_t e, const char* what) {
     hipError_t _e = (expr);                                                       \
     if (_e != hipSuccess) return hip_fail(_e, (std::string(#expr) + " " + where).c_str()); \
   } while (0)
+
+// context slots of the assembly's row pointers and fill chunk table (after
+// the work arrays that bind_work numbers)
+constexpr int kRowPtrSlot = 48;
+constexpr int kChunkSlot = 49;
 
 // bumped whenever a device buffer is (re)allocated: recorded graphs hold
 // buffer addresses and are recorded again after any change
@@ -2097,7 +2159,8 @@ int check_common(int cc, long long n_dofs, int glq_u, int glq_v, int basis, int 
     set_error("unknown integral", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
-  if (precision != FEM2D_PRECISION_FP64 && precision != FEM2D_PRECISION_FP32) {
+  if (precision != FEM2D_PRECISION_FP64 && precision != FEM2D_PRECISION_FP32 &&
+      precision != FEM2D_PRECISION_FP32R) {
     set_error("unknown precision", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
@@ -2119,6 +2182,236 @@ int err_to_status(long long bits) {
   }
   set_error("a row group exceeds this build's limits (128 nested Elems, 1024 memberships or rows)", 0);
   return FEM2D_ERR_UNSUPPORTED;
+}
+
+// ---------------------------------------------------------------------------
+// One assembly: its device arrays, launch shapes and kernel sequence; the
+// arrays are bound by a binder (fem2d_assemble_device: the context's
+// grow-only slots)
+// ---------------------------------------------------------------------------
+
+struct Assembly {
+  DevView v{};
+  Work w{};
+  LibShape L{};
+  long long lib_n = 0;
+  long long n_grp_items = 0;  // row groups the records are sized for, plus one
+  long long tiles_nested = 0, tiles_dof = 0, tiles_mem = 0, tiles_grp = 0;
+  long long *ts_nested = nullptr, *ts_dof = nullptr, *ts_mem = nullptr, *ts_grp = nullptr;
+  unsigned slot_blocks = 0;
+  bool run_large = true;  // launch the block path (unless the Domain is known to need none)
+};
+
+// Option and size checks after check_common; *nu, *nv receive the Gauss point counts.
+int check_assembly(const fem2d_device_domain* dd, int glq_u, int glq_v, int* nu, int* nv) {
+  if (dd->n_bs == 0 || dd->n_elems == 0) {
+    set_error("domain view has DoFs but no basis specs", 0);
+    return FEM2D_ERR_INVALID_VIEW;
+  }
+  *nu = glq_u != 0 ? glq_u : default_ngq(dd->max_i);
+  *nv = glq_v != 0 ? glq_v : default_ngq(dd->max_j);
+  if (*nu > kMaxGlq || *nv > kMaxGlq || *nu < 1 || *nv < 1) {
+    set_error("more Gauss points per axis than this build supports (64)", 0);
+    return FEM2D_ERR_UNSUPPORTED;
+  }
+  if (dd->max_i > kMaxOrder || dd->max_j > kMaxOrder) {
+    set_error("expansion order above MAX_POLYNOMIAL_ORDER", 0);
+    return FEM2D_ERR_INVALID_VIEW;
+  }
+  if (dd->max_du > kMaxRelDepth || dd->max_dv > kMaxRelDepth) return err_to_status(kErrDepth);
+  return FEM2D_OK;
+}
+
+// Binds every work array of the assembly except row_ptr (an output) and
+// chunk_first (sized by the output capacity) through bind(k, count, &ptr),
+// k numbering the arrays; fills in the view, the library shape and the
+// prefix-sum tile layout.
+template <class Bind>
+int bind_work(const fem2d_device_domain* dd, int nu, int nv, int basis, Bind&& bind, Assembly& a) {
+  const int ne = dd->n_elems;
+  const long long nd = dd->n_dofs;
+  const long long nbs = dd->n_bs;
+  Work& w = a.w;
+  w = Work{};
+  w.n_dofs = nd;
+  int rc = FEM2D_OK;
+  int k = 0;
+  const size_t E1 = size_t(std::max(ne, 1)), N1 = size_t(nd + 1), B1 = size_t(nbs + 1);
+  if ((rc = bind(k++, E1, &w.geo)) || (rc = bind(k++, E1, &w.depth)) || (rc = bind(k++, E1, &w.n_desc)) ||
+      (rc = bind(k++, E1, &w.desc_cursor)) || (rc = bind(k++, size_t(ne + 1), &w.nested_ptr)) ||
+      (rc = bind(k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) || (rc = bind(k++, E1, &w.dyad)) ||
+      (rc = bind(k++, B1, &w.spec_run)) || (rc = bind(k++, B1, &w.spec_pos)) || (rc = bind(k++, B1, &w.bs_elem)) ||
+      (rc = bind(k++, B1, &w.bs_pack)) || (rc = bind(k++, N1, &w.dof_e1)) || (rc = bind(k++, N1, &w.dof_e2)) ||
+      (rc = bind(k++, N1, &w.dof_t1)) || (rc = bind(k++, N1, &w.dof_t2)) || (rc = bind(k++, N1, &w.dof_flag)) ||
+      (rc = bind(k++, N1, &w.dof_grp)) || (rc = bind(k++, N1, &w.g_start)) || (rc = bind(k++, N1, &w.g_e1)) ||
+      (rc = bind(k++, N1, &w.g_e2)) || (rc = bind(k++, B1, &w.mem_idx)) || (rc = bind(k++, B1, &w.mem_group)) ||
+      (rc = bind(k++, B1, &w.mem_bsoff)) || (rc = bind(k++, N1, &w.row_len)) || (rc = bind(k++, size_t(8), &w.err)) ||
+      (rc = bind(k++, N1, &w.large_list)) || (rc = bind(k++, N1, &w.is_large)))
+    return rc;
+  w.n_large = w.err + 1;
+  const size_t G1 = size_t(std::max(1LL, dd->n_groups));
+  w.meta_cap = int(G1);
+  if ((rc = bind(k++, G1, &w.meta_hdr)) || (rc = bind(k++, G1 * kWS * kCoef, &w.meta_coef)) ||
+      (rc = bind(k++, G1 * kWC, &w.meta_col)) || (rc = bind(k++, G1 * kWRows, &w.meta_row)) ||
+      (rc = bind(k++, G1 + 1, &w.seg_start)))
+    return rc;
+
+  LibShape& L = a.L;
+  L.p1 = std::max(dd->max_i, dd->max_j) + 1;
+  L.tab = L.p1 * L.p1;
+  L.range_stride = kKinds * L.tab;
+  L.nr = (1 << (std::max(dd->max_du, dd->max_dv) + 1)) - 1;
+  L.nu = nu;
+  L.nv = nv;
+  L.basis = basis;
+  a.lib_n = 2LL * L.nr * kKinds * L.p1 * L.p1;
+  w.lib_range_stride = L.range_stride;
+  w.lib_nr = L.nr;
+  if ((rc = bind(k++, size_t(a.lib_n), &w.lib))) return rc;
+
+  // tile sums of the four prefix sums, one region each (two run side by side)
+  a.n_grp_items = (long long)G1 + 1;
+  a.tiles_nested = scan_tiles(ne + 1);
+  a.tiles_dof = scan_tiles(nd, ScanRounds<DofGroupScan>::value);
+  a.tiles_mem = scan_tiles(nbs + 1);
+  a.tiles_grp = scan_tiles(a.n_grp_items, ScanRounds<GroupScan>::value);
+  if ((rc = bind(k++, size_t(a.tiles_nested + a.tiles_dof + a.tiles_mem + a.tiles_grp), &w.tile_sum))) return rc;
+  a.ts_nested = w.tile_sum;
+  a.ts_dof = a.ts_nested + a.tiles_nested;
+  a.ts_mem = a.ts_dof + a.tiles_dof;
+  a.ts_grp = a.ts_mem + a.tiles_mem;
+  a.slot_blocks =
+      unsigned(std::min<long long>(std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock), 1 << 16));
+
+  DevView& v = a.v;
+  v.n_elems = ne;
+  v.n_dofs = nd;
+  v.n_bs = nbs;
+  v.parent = dd->parent;
+  v.loc = dd->loc;
+  v.box = dd->box;
+  v.mat = dd->mat;
+  v.bs_off = dd->bs_off;
+  v.bs_dir = dd->bs_dir;
+  v.bs_i = dd->bs_i;
+  v.bs_j = dd->bs_j;
+  v.bs_dof = dd->bs_dof;
+  v.dof_bs = dd->dof_bs;
+  v.p1 = L.p1;
+  a.run_large = dd->large_state != 0;
+  return FEM2D_OK;
+}
+
+// Gauss-Legendre nodes and weights per axis in the context (a function of the
+// point counts only; host arithmetic)
+void ensure_glq(fem2d_ctx* ctx, int nu, int nv) {
+  if (ctx->glq_n[0] != nu || ctx->glq_n[1] != nv) {
+    ctx->glq = GlqArgs{};
+    fem2d::gauss_legendre_nodes(nu, ctx->glq.u_pts, ctx->glq.u_wts);
+    fem2d::gauss_legendre_nodes(nv, ctx->glq.v_pts, ctx->glq.v_wts);
+    ctx->glq_n[0] = nu;
+    ctx->glq_n[1] = nv;
+  }
+}
+
+// Part (a): preparation and symbolic phase, ending with the CSR layout and the
+// status block (nnz, error bits, block-path groups, row groups) at status.
+// With fine_events, events between the stages time them and the describe
+// kernel.
+int launch_prep(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, long long* status, bool fine_events) {
+  const Work& w = a.w;
+  const DevView& v = a.v;
+  const int ne = v.n_elems;
+  const long long nd = v.n_dofs, nbs = v.n_bs;
+  const int B = 256;
+  auto scan_grid = [&](long long tiles) { return unsigned(std::max(1LL, std::min(tiles, 8LL * ctx->n_cu))); };
+  k_init<<<grid_for(ne, B), B, 0, st>>>(w, ne, nd);
+  const unsigned gram_blocks = grid_for(a.lib_n, B), prep_blocks = grid_for(ne, B);
+  k_gram_elem_specs<<<gram_blocks + prep_blocks + a.slot_blocks + grid_for(nd, B), B, 0, st>>>(
+      v, w, gram_blocks, prep_blocks, a.slot_blocks, a.L, ctx->glq);
+  HIP_TRY_AT(hipGetLastError(), "after the Elem preparation kernels");
+  // nested-list offsets and row groups side by side
+  const unsigned gn = scan_grid(a.tiles_nested), gd = scan_grid(a.tiles_dof), gm = scan_grid(a.tiles_mem);
+  k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, a.ts_nested, gn, nd, DofGroupScan{v, w},
+                                             a.ts_dof);
+  k_scan_apply2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, a.ts_nested, gn, nd, DofGroupScan{v, w},
+                                            a.ts_dof);
+  // nested lists beside the first pass of the membership numbering
+  const unsigned fill_blocks = grid_for(ne, B);
+  k_nested_members<<<fill_blocks + gm, kBlock, 0, st>>>(v, w, fill_blocks, nbs + 1, MembershipScan{v, w}, a.ts_mem);
+  k_scan_apply<<<gm, kBlock, 0, st>>>(nbs + 1, MembershipScan{v, w}, a.ts_mem);
+  HIP_TRY_AT(hipGetLastError(), "after the nested-list and row-group kernels");
+  if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[1], st));
+  // symbolic: per-group candidate sets, row lengths and records, then row_ptr
+  if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[4], st));
+  k_describe_wave<<<unsigned(ctx->count_grid), kBlock, 0, st>>>(v, w);
+  if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[5], st));
+  const unsigned large_grid = unsigned(std::max(1, ctx->n_cu * 2));
+  if (a.run_large) k_count_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w);
+  HIP_TRY_AT(hipGetLastError(), "after the describe kernels");
+  // CSR layout and the status block
+  const GroupScan gs{w, a.n_grp_items, status};
+  k_scan_reduce<<<scan_grid(a.tiles_grp), kBlock, 0, st>>>(a.n_grp_items, gs, a.ts_grp);
+  k_scan_apply<<<scan_grid(a.tiles_grp), kBlock, 0, st>>>(a.n_grp_items, gs, a.ts_grp);
+  HIP_TRY_AT(hipGetLastError(), "after the CSR layout scan");
+  if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[2], st));
+  return FEM2D_OK;
+}
+
+// Part (b): the streaming fill (columns and values; with too small output
+// buffers it writes nothing)
+int launch_stream_fill(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, const FillOut& o) {
+  const int pi = o.precision;
+  const bool lds = lib_in_lds(o.precision, a.lib_n);
+  const unsigned grid = unsigned(lds ? ctx->fill_grid_lds[pi] : ctx->fill_grid[pi]);
+  launch_fill(lds, o.a_is_cc != 0, o.b_is_cc != 0, grid, st, a.w, a.L, o);
+  HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
+  return FEM2D_OK;
+}
+
+// Part (c): the block-path fill (Domains with large row groups)
+int launch_large_fill(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, const FillOut& o) {
+  const unsigned large_grid = unsigned(std::max(1, ctx->n_cu * 2));
+  k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(a.v, a.w, a.L, o);
+  HIP_TRY_AT(hipGetLastError(), "after the block-path fill kernel");
+  return FEM2D_OK;
+}
+
+// resident blocks of the streaming fill of one stored precision (the fill
+// assigns its chunks statically: exactly these are launched)
+template <int P>
+int fill_grids(fem2d_ctx* ctx) {
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, P>,
+                                                       Store<P>::kBlock, 0));
+  ctx->fill_grid[P] = std::max(1, per_cu) * ctx->n_cu;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, P>,
+                                                       Store<P>::kBlock, 0));
+  ctx->fill_grid_lds[P] = std::max(1, per_cu) * ctx->n_cu;
+  return FEM2D_OK;
+}
+
+FillOut make_fill_out(int precision, int ai, int bi, long long capacity, int* col, void* a_val, void* b_val,
+                      unsigned* res) {
+  FillOut o;
+  o.col = col;
+  o.aval = a_val;
+  o.bval = b_val;
+  o.res = res;
+  o.precision = precision;
+  o.capacity = capacity;
+  o.a_is_cc = ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
+  o.b_is_cc = bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
+  return o;
+}
+
+// CSR entries per fill chunk for a stored precision
+int chunk_of(int precision) {
+  switch (precision) {
+    case FEM2D_PRECISION_FP32: return Store<FEM2D_PRECISION_FP32>::kChunk;
+    case FEM2D_PRECISION_FP32R: return Store<FEM2D_PRECISION_FP32R>::kChunk;
+    default: return Store<FEM2D_PRECISION_FP64>::kChunk;
+  }
 }
 
 }  // namespace
@@ -2152,20 +2445,9 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_describe_wave, kBlock, 0));
   ctx->count_grid = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, double>,
-                                                   FillShape<double>::kBlock, 0));
-  ctx->fill_grid[0] = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, float>,
-                                                       FillShape<float>::kBlock, 0));
-  ctx->fill_grid[1] = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, float>,
-                                                       FillShape<float>::kBlock, 0));
-  ctx->fill_grid_lds[1] = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, double>,
-                                                   FillShape<double>::kBlock, 0));
-  ctx->fill_grid_lds[0] = std::max(1, per_cu) * ctx->n_cu;
+  if (int rc = fill_grids<FEM2D_PRECISION_FP64>(ctx)) return rc;
+  if (int rc = fill_grids<FEM2D_PRECISION_FP32>(ctx)) return rc;
+  if (int rc = fill_grids<FEM2D_PRECISION_FP32R>(ctx)) return rc;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fill_large),
@@ -2180,7 +2462,7 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& b : ctx->bufs)
     if (b.p) (void)hipFree(b.p);
-  for (DevBuf* b : {&ctx->out_col, &ctx->out_a, &ctx->out_b, &ctx->field_scratch})
+  for (DevBuf* b : {&ctx->out_col, &ctx->out_a, &ctx->out_b, &ctx->out_r, &ctx->field_scratch})
     if (b->p) (void)hipFree(b->p);
   for (auto& g : ctx->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
@@ -2340,120 +2622,24 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     return FEM2D_ERR_INVALID_ARG;
   }
   if (int rc = check_common(dd->cc, dd->n_dofs, glq_u, glq_v, basis, ai, bi, precision)) return rc;
-  if (dd->n_bs == 0 || dd->n_elems == 0) {
-    set_error("domain view has DoFs but no basis specs", 0);
-    return FEM2D_ERR_INVALID_VIEW;
-  }
+  int nu = 0, nv = 0;
+  if (int rc = check_assembly(dd, glq_u, glq_v, &nu, &nv)) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   // whatever happens below, the output buffers no longer hold the previous
   // assembly's GEP: its handles become stale
   ctx->out_generation = 0;
-  const int ne = dd->n_elems;
   const long long nd = dd->n_dofs;
-  const long long nbs = dd->n_bs;
-  const int nu = glq_u != 0 ? glq_u : default_ngq(dd->max_i);
-  const int nv = glq_v != 0 ? glq_v : default_ngq(dd->max_j);
-  if (nu > kMaxGlq || nv > kMaxGlq || nu < 1 || nv < 1) {
-    set_error("more Gauss points per axis than this build supports (64)", 0);
-    return FEM2D_ERR_UNSUPPORTED;
-  }
-  if (dd->max_i > kMaxOrder || dd->max_j > kMaxOrder) {
-    set_error("expansion order above MAX_POLYNOMIAL_ORDER", 0);
-    return FEM2D_ERR_INVALID_VIEW;
-  }
-  if (dd->max_du > kMaxRelDepth || dd->max_dv > kMaxRelDepth) return err_to_status(kErrDepth);
 
   // work buffers (grow-only, reused across calls)
-  Work w{};
-  w.n_dofs = nd;
-  int rc = FEM2D_OK;
-  int k = 0;
-  const size_t E1 = size_t(std::max(ne, 1)), N1 = size_t(nd + 1), B1 = size_t(nbs + 1);
-  if ((rc = slot(ctx, k++, E1, &w.geo)) || (rc = slot(ctx, k++, E1, &w.depth)) ||
-      (rc = slot(ctx, k++, E1, &w.n_desc)) || (rc = slot(ctx, k++, E1, &w.desc_cursor)) ||
-      (rc = slot(ctx, k++, size_t(ne + 1), &w.nested_ptr)) ||
-      (rc = slot(ctx, k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) ||
-      (rc = slot(ctx, k++, E1, &w.dyad)) || (rc = slot(ctx, k++, B1, &w.spec_run)) || (rc = slot(ctx, k++, B1, &w.spec_pos)) ||
-      (rc = slot(ctx, k++, B1, &w.bs_elem)) ||
-      (rc = slot(ctx, k++, B1, &w.bs_pack)) || (rc = slot(ctx, k++, N1, &w.dof_e1)) ||
-      (rc = slot(ctx, k++, N1, &w.dof_e2)) || (rc = slot(ctx, k++, N1, &w.dof_t1)) ||
-      (rc = slot(ctx, k++, N1, &w.dof_t2)) || (rc = slot(ctx, k++, N1, &w.dof_flag)) ||
-      (rc = slot(ctx, k++, N1, &w.dof_grp)) || (rc = slot(ctx, k++, N1, &w.g_start)) ||
-      (rc = slot(ctx, k++, N1, &w.g_e1)) || (rc = slot(ctx, k++, N1, &w.g_e2)) ||
-      (rc = slot(ctx, k++, B1, &w.mem_idx)) || (rc = slot(ctx, k++, B1, &w.mem_group)) ||
-      (rc = slot(ctx, k++, B1, &w.mem_bsoff)) || (rc = slot(ctx, k++, N1, &w.row_len)) ||
-      (rc = slot(ctx, k++, N1, &w.row_ptr)) || 
-      (rc = slot(ctx, k++, size_t(8), &w.err)) || (rc = slot(ctx, k++, N1, &w.large_list)) ||
-      (rc = slot(ctx, k++, N1, &w.is_large)))
-    return rc;
-  w.n_large = w.err + 1;
-  const size_t G1 = size_t(std::max(1LL, dd->n_groups));
-  w.meta_cap = int(G1);
-  if ((rc = slot(ctx, k++, G1, &w.meta_hdr)) || (rc = slot(ctx, k++, G1 * kWS * kCoef, &w.meta_coef)) ||
-      (rc = slot(ctx, k++, G1 * kWC, &w.meta_col)) ||
-      (rc = slot(ctx, k++, G1 * kWRows, &w.meta_row)) || (rc = slot(ctx, k++, G1 + 1, &w.seg_start)))
-    return rc;
-  const int chunk_slot = k++;
-
-  LibShape L;
-  L.p1 = std::max(dd->max_i, dd->max_j) + 1;
-  L.tab = L.p1 * L.p1;
-  L.range_stride = kKinds * L.tab;
-  L.nr = (1 << (std::max(dd->max_du, dd->max_dv) + 1)) - 1;
-  L.nu = nu;
-  L.nv = nv;
-  L.basis = basis;
-  const long long lib_n = 2LL * L.nr * kKinds * L.p1 * L.p1;
-  w.lib_range_stride = L.range_stride;
-  w.lib_nr = L.nr;
-  if ((rc = slot(ctx, k++, size_t(lib_n), &w.lib))) return rc;
-
-  // tile sums of the four prefix sums, one region each (two run side by side)
-  const long long n_grp_items = (long long)G1 + 1;
-  const long long tiles_nested = scan_tiles(ne + 1), tiles_dof = scan_tiles(nd, ScanRounds<DofGroupScan>::value),
-                  tiles_mem = scan_tiles(nbs + 1),
-                  tiles_grp = scan_tiles(n_grp_items, ScanRounds<GroupScan>::value);
-  if ((rc = slot(ctx, k++, size_t(tiles_nested + tiles_dof + tiles_mem + tiles_grp), &w.tile_sum))) return rc;
-  long long* ts_nested = w.tile_sum;
-  long long* ts_dof = ts_nested + tiles_nested;
-  long long* ts_mem = ts_dof + tiles_dof;
-  long long* ts_grp = ts_mem + tiles_mem;
-  auto scan_grid = [&](long long tiles) { return unsigned(std::max(1LL, std::min(tiles, 8LL * ctx->n_cu))); };
-  const unsigned slot_blocks =
-      unsigned(std::min<long long>(std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock), 1 << 16));
-
+  Assembly a;
+  int rc = bind_work(dd, nu, nv, basis,
+                     [&](int k, size_t count, auto** ptr) { return slot(ctx, k, count, ptr); }, a);
+  if (!rc) rc = slot(ctx, kRowPtrSlot, size_t(nd + 1), &a.w.row_ptr);
+  if (rc) return rc;
+  Work& w = a.w;
+  const bool run_large = a.run_large;
   hipStream_t st = ctx->stream;
-  // the previous call's readback of the pinned staging buffers has completed
-  // (every call ends with a stream synchronisation)
-  // Gauss-Legendre nodes and weights per axis (a function of the point counts only)
-  if (ctx->glq_n[0] != nu || ctx->glq_n[1] != nv) {
-    ctx->glq = GlqArgs{};
-    fem2d::gauss_legendre_nodes(nu, ctx->glq.u_pts, ctx->glq.u_wts);
-    fem2d::gauss_legendre_nodes(nv, ctx->glq.v_pts, ctx->glq.v_wts);
-    ctx->glq_n[0] = nu;
-    ctx->glq_n[1] = nv;
-  }
-
-  DevView v;
-  v.n_elems = ne;
-  v.n_dofs = nd;
-  v.n_bs = nbs;
-  v.parent = dd->parent;
-  v.loc = dd->loc;
-  v.box = dd->box;
-  v.mat = dd->mat;
-  v.bs_off = dd->bs_off;
-  v.bs_dir = dd->bs_dir;
-  v.bs_i = dd->bs_i;
-  v.bs_j = dd->bs_j;
-  v.bs_dof = dd->bs_dof;
-  v.dof_bs = dd->dof_bs;
-  v.p1 = L.p1;
-
-  const int B = 256;
-  const unsigned large_grid = unsigned(std::max(1, ctx->n_cu * 2));
-  // the block path runs unless this Domain is known to have no large groups
-  const bool run_large = dd->large_state != 0;
+  ensure_glq(ctx, nu, nv);
 
   // The assembly in three parts: (a) preparation and symbolic phase, ending
   // with the CSR layout and the status block, (b) the streaming fill, (c) the
@@ -2462,51 +2648,6 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   // and the buffers stay the same, and timed end to end; kernel by kernel,
   // events between the stages also time the stages and the describe and fill
   // kernels on their own.
-  auto part_a = [&](const FillOut& o, bool fine_events) -> int {
-    k_init<<<grid_for(ne, B), B, 0, st>>>(w, ne, nd);
-    const unsigned gram_blocks = grid_for(lib_n, B), prep_blocks = grid_for(ne, B);
-    k_gram_elem_specs<<<gram_blocks + prep_blocks + slot_blocks + grid_for(nd, B), B, 0, st>>>(
-        v, w, gram_blocks, prep_blocks, slot_blocks, L, ctx->glq);
-    HIP_TRY_AT(hipGetLastError(), "after the Elem preparation kernels");
-    // nested-list offsets and row groups side by side
-    const unsigned gn = scan_grid(tiles_nested), gd = scan_grid(tiles_dof), gm = scan_grid(tiles_mem);
-    k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{v, w}, ts_dof);
-    k_scan_apply2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, ts_nested, gn, nd, DofGroupScan{v, w}, ts_dof);
-    // nested lists beside the first pass of the membership numbering
-    const unsigned fill_blocks = grid_for(ne, B);
-    k_nested_members<<<fill_blocks + gm, kBlock, 0, st>>>(v, w, fill_blocks, nbs + 1, MembershipScan{v, w}, ts_mem);
-    k_scan_apply<<<gm, kBlock, 0, st>>>(nbs + 1, MembershipScan{v, w}, ts_mem);
-    HIP_TRY_AT(hipGetLastError(), "after the nested-list and row-group kernels");
-    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[1], st));
-    // symbolic: per-group candidate sets, row lengths and records, then row_ptr
-    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[4], st));
-    k_describe_wave<<<unsigned(ctx->count_grid), kBlock, 0, st>>>(v, w);
-    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[5], st));
-    if (run_large) k_count_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w);
-    HIP_TRY_AT(hipGetLastError(), "after the describe kernels");
-    // CSR layout and the status block
-    const GroupScan gs{w, n_grp_items, ctx->d_status};
-    k_scan_reduce<<<scan_grid(tiles_grp), kBlock, 0, st>>>(n_grp_items, gs, ts_grp);
-    k_scan_apply<<<scan_grid(tiles_grp), kBlock, 0, st>>>(n_grp_items, gs, ts_grp);
-    HIP_TRY_AT(hipGetLastError(), "after the CSR layout scan");
-    if (fine_events) HIP_TRY(hipEventRecord(ctx->ev[2], st));
-    return FEM2D_OK;
-  };
-  // numeric: columns and values; with too small output buffers the fill
-  // writes nothing and the host grows them and runs the assembly again
-  auto part_b = [&](const FillOut& o) -> int {
-    const int pi = o.fp32 ? 1 : 0;
-    const bool lds = lib_n <= (o.fp32 ? FillShape<float>::kLibLds : FillShape<double>::kLibLds);
-    const unsigned grid = unsigned(lds ? ctx->fill_grid_lds[pi] : ctx->fill_grid[pi]);
-    launch_fill(lds, o.a_is_cc != 0, o.b_is_cc != 0, grid, st, w, L, o);
-    HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
-    return FEM2D_OK;
-  };
-  auto part_c = [&](const FillOut& o) -> int {
-    k_fill_large<<<large_grid, kBlock, sizeof(GroupShared), st>>>(v, w, L, o);
-    HIP_TRY_AT(hipGetLastError(), "after the block-path fill kernel");
-    return FEM2D_OK;
-  };
   // record the work of parts() into ctx->graph_exec[i]
   auto record = [&](int i, const std::function<int()>& part) -> int {
     if (ctx->graph_exec[i]) (void)hipGraphExecDestroy(ctx->graph_exec[i]);
@@ -2532,20 +2673,16 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   long long nnz = 0;
   int n_large = 0;
   bool fine_timing = false;
+  const bool need_res = precision == FEM2D_PRECISION_FP32R;
   for (int attempt = 0; attempt < 2; ++attempt) {
-    FillOut o;
-    o.col = static_cast<int*>(ctx->out_col.p);
-    o.aval = static_cast<double*>(ctx->out_a.p);
-    o.bval = static_cast<double*>(ctx->out_b.p);
-    o.aval32 = static_cast<float*>(ctx->out_a.p);
-    o.bval32 = static_cast<float*>(ctx->out_b.p);
-    o.fp32 = precision == FEM2D_PRECISION_FP32 ? 1 : 0;
-    o.capacity = ctx->out_capacity;
-    o.a_is_cc = ai == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
-    o.b_is_cc = bi == FEM2D_INTEGRAL_CURLCURL ? 1 : 0;
-    w.chunk = o.fp32 ? FillShape<float>::kChunk : FillShape<double>::kChunk;
+    // FP32R keeps its residuals in a buffer of their own, grown with the others
+    if (need_res && (rc = ensure(ctx->out_r, size_t(std::max(1LL, ctx->out_capacity)) * sizeof(uint32_t))))
+      return rc;
+    const FillOut o = make_fill_out(precision, ai, bi, ctx->out_capacity, static_cast<int*>(ctx->out_col.p),
+                                    ctx->out_a.p, ctx->out_b.p, static_cast<unsigned*>(ctx->out_r.p));
+    w.chunk = chunk_of(precision);
     w.chunk_cap = o.capacity / w.chunk + 1;
-    if ((rc = slot(ctx, chunk_slot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
+    if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
     const bool fill = o.capacity > 0;
     const bool tail = fill && run_large;  // part (c) has work
     if (ctx->use_graphs) {
@@ -2554,9 +2691,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       if (!ctx->graph_exec[0] || !(key == ctx->graph_key)) {
         ctx->graph_key = GraphKey{};
         if ((rc = record(0, [&] {
-               int prc = part_a(o, false);
-               if (prc == FEM2D_OK && fill) prc = part_b(o);
-               if (prc == FEM2D_OK && tail) prc = part_c(o);
+               int prc = launch_prep(ctx, st, a, ctx->d_status, false);
+               if (prc == FEM2D_OK && fill) prc = launch_stream_fill(ctx, st, a, o);
+               if (prc == FEM2D_OK && tail) prc = launch_large_fill(ctx, st, a, o);
                return prc;
              })))
           return rc;
@@ -2569,11 +2706,11 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       fine_timing = false;
     } else {
       HIP_TRY(hipEventRecord(ctx->ev[0], st));
-      if ((rc = part_a(o, true))) return rc;
+      if ((rc = launch_prep(ctx, st, a, ctx->d_status, true))) return rc;
       HIP_TRY(hipEventRecord(ctx->ev[6], st));
-      if (fill && (rc = part_b(o))) return rc;
+      if (fill && (rc = launch_stream_fill(ctx, st, a, o))) return rc;
       HIP_TRY(hipEventRecord(ctx->ev[7], st));
-      if (tail && (rc = part_c(o))) return rc;
+      if (tail && (rc = launch_large_fill(ctx, st, a, o))) return rc;
       HIP_TRY(hipEventRecord(ctx->ev[3], st));
       fine_timing = true;
     }
@@ -2624,12 +2761,13 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   out->nnz = nnz;
   out->row_ptr = w.row_ptr;
   out->col = static_cast<int32_t*>(ctx->out_col.p);
-  const bool fp32 = precision == FEM2D_PRECISION_FP32;
+  const bool f32 = precision != FEM2D_PRECISION_FP64;  // FP32 and FP32R store FP32 values
   out->precision = precision;
-  out->a_val = fp32 ? nullptr : static_cast<double*>(ctx->out_a.p);
-  out->b_val = fp32 ? nullptr : static_cast<double*>(ctx->out_b.p);
-  out->a_val32 = fp32 ? static_cast<float*>(ctx->out_a.p) : nullptr;
-  out->b_val32 = fp32 ? static_cast<float*>(ctx->out_b.p) : nullptr;
+  out->a_val = f32 ? nullptr : static_cast<double*>(ctx->out_a.p);
+  out->b_val = f32 ? nullptr : static_cast<double*>(ctx->out_b.p);
+  out->a_val32 = f32 ? static_cast<float*>(ctx->out_a.p) : nullptr;
+  out->b_val32 = f32 ? static_cast<float*>(ctx->out_b.p) : nullptr;
+  out->res16 = precision == FEM2D_PRECISION_FP32R ? static_cast<uint32_t*>(ctx->out_r.p) : nullptr;
   ctx->out_generation = ++g_gep_generation;
   out->generation = ctx->out_generation;
   return FEM2D_OK;
@@ -2857,15 +2995,44 @@ int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d
       return 0;
     };
   };
+  // FP32R residuals applied to values already widened from their FP32 parts
+  auto add_residuals = [&](double* da, double* db) {
+    return [da, db, threads](const unsigned char* p, size_t off, size_t len) -> int {
+      const size_t k0 = off / sizeof(uint32_t);
+      fem2d::parallel_ranges(len / sizeof(uint32_t), threads, size_t(1) << 20, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) {
+          uint32_t r;
+          std::memcpy(&r, p + i * sizeof(uint32_t), sizeof(uint32_t));
+          da[k0 + i] = fem2d::fp32r_join(float(da[k0 + i]), fem2d::fp32r_res_a(r));
+          db[k0 + i] = fem2d::fp32r_join(float(db[k0 + i]), fem2d::fp32r_res_b(r));
+        }
+      });
+      return 0;
+    };
+  };
   hipError_t e = hipSuccess;
   const size_t nnz = size_t(std::max<int64_t>(0, src->nnz));
-  const bool fp32 = src->precision == FEM2D_PRECISION_FP32;
+  // each value array in the width its precision stores (FP32 widened to double)
+  if (src->precision != FEM2D_PRECISION_FP64 && src->precision != FEM2D_PRECISION_FP32 &&
+      src->precision != FEM2D_PRECISION_FP32R) {
+    fem2d_gep_csr_free(out);
+    set_error("unknown precision", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  const bool f32 = src->precision == FEM2D_PRECISION_FP32, f32r = src->precision == FEM2D_PRECISION_FP32R;
+  if ((f32 || f32r) && src->nnz > 0 && (!src->a_val32 || !src->b_val32 || (f32r && !src->res16))) {
+    fem2d_gep_csr_free(out);
+    set_error("GEP arrays missing", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
   int rc = ps.run(src->row_ptr, size_t(src->n + 1) * sizeof(int64_t), copy_to(out->row_ptr), &e);
   if (!rc) rc = ps.run(src->col, nnz * sizeof(int32_t), copy_to(out->col), &e);
-  if (!rc && fp32) rc = ps.run(src->a_val32, nnz * sizeof(float), widen_to(out->a_val), &e);
-  if (!rc && fp32) rc = ps.run(src->b_val32, nnz * sizeof(float), widen_to(out->b_val), &e);
-  if (!rc && !fp32) rc = ps.run(src->a_val, nnz * sizeof(double), copy_to(out->a_val), &e);
-  if (!rc && !fp32) rc = ps.run(src->b_val, nnz * sizeof(double), copy_to(out->b_val), &e);
+  if (!rc && (f32 || f32r)) rc = ps.run(src->a_val32, nnz * sizeof(float), widen_to(out->a_val), &e);
+  if (!rc && (f32 || f32r)) rc = ps.run(src->b_val32, nnz * sizeof(float), widen_to(out->b_val), &e);
+  // FP32R: each widened value gains its residual (fp32r_join), both matrices from one word
+  if (!rc && f32r) rc = ps.run(src->res16, nnz * sizeof(uint32_t), add_residuals(out->a_val, out->b_val), &e);
+  if (!rc && !f32 && !f32r) rc = ps.run(src->a_val, nnz * sizeof(double), copy_to(out->a_val), &e);
+  if (!rc && !f32 && !f32r) rc = ps.run(src->b_val, nnz * sizeof(double), copy_to(out->b_val), &e);
   if (rc) {
     fem2d_gep_csr_free(out);
     return rc == -1 ? hip_fail(e, "GEP download") : rc;

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/fem2d.h"
+#include "fp32r.hpp"
 #include "host/capi_internal.hpp"
 
 namespace {
@@ -50,10 +51,11 @@ __global__ void k_set_int(int* __restrict__ p, int v) {
 // Full symmetric A and B (column-major, leading dimension n) from the stored
 // upper triangle, one wavefront per row. An entry outside the upper triangle
 // is not written and sets *err.
+// V: the stored value type; res: FP32R residuals (fp32r.hpp), null otherwise.
 template <class V>
 __global__ void k_expand(long long n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                         const V* __restrict__ a, const V* __restrict__ b, double* __restrict__ A,
-                         double* __restrict__ B, int* __restrict__ err) {
+                         const V* __restrict__ a, const V* __restrict__ b, const uint32_t* __restrict__ res,
+                         double* __restrict__ A, double* __restrict__ B, int* __restrict__ err) {
   const int lane = threadIdx.x % kWave;
   const long long wave = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave;
   const long long waves = (gridDim.x * (long long)blockDim.x) / kWave;
@@ -65,7 +67,12 @@ __global__ void k_expand(long long n, const int64_t* __restrict__ row_ptr, const
         atomicOr(err, 1);
         continue;
       }
-      const double av = double(a[k]), bv = double(b[k]);
+      double av = double(a[k]), bv = double(b[k]);
+      if (res) {
+        const uint32_t r = res[k];
+        av = fem2d::fp32r_join(float(av), fem2d::fp32r_res_a(r));
+        bv = fem2d::fp32r_join(float(bv), fem2d::fp32r_res_b(r));
+      }
       A[r + c * n] = av;
       A[c + r * n] = av;
       B[r + c * n] = bv;
@@ -146,13 +153,15 @@ int read_info(const rocblas_int* d_info, hipStream_t st, rocblas_int* h_info) {
 // Dense A and B of a device GEP.
 int expand(const fem2d_gep_device* g, hipStream_t st, DevMem& m, double** A, double** B) {
   const long long n = g->n;
-  const bool fp32 = g->precision == FEM2D_PRECISION_FP32;
+  const bool f32r = g->precision == FEM2D_PRECISION_FP32R;
+  const bool fp32 = g->precision == FEM2D_PRECISION_FP32 || f32r;  // FP32 value arrays
   if (g->precision != FEM2D_PRECISION_FP64 && !fp32) {
     set_error("unknown precision", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
-  if (g->nnz < 0 || !g->row_ptr || (g->nnz > 0 && (!g->col || (fp32 ? !g->a_val32 || !g->b_val32
-                                                                      : !g->a_val || !g->b_val)))) {
+  if (g->nnz < 0 || !g->row_ptr ||
+      (g->nnz > 0 && (!g->col || (fp32 ? !g->a_val32 || !g->b_val32 : !g->a_val || !g->b_val) ||
+                      (f32r && !g->res16)))) {
     set_error("GEP arrays missing", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
@@ -168,9 +177,11 @@ int expand(const fem2d_gep_device* g, hipStream_t st, DevMem& m, double** A, dou
   EIG_HIP(hipGetLastError());
   const unsigned row_blocks = unsigned(std::max<long long>(1, std::min<long long>((n + 3) / 4, 1 << 16)));
   if (fp32)
-    k_expand<<<row_blocks, kBlock, 0, st>>>(n, g->row_ptr, g->col, g->a_val32, g->b_val32, *A, *B, err);
+    k_expand<<<row_blocks, kBlock, 0, st>>>(n, g->row_ptr, g->col, g->a_val32, g->b_val32, f32r ? g->res16 : nullptr,
+                                            *A, *B, err);
   else
-    k_expand<<<row_blocks, kBlock, 0, st>>>(n, g->row_ptr, g->col, g->a_val, g->b_val, *A, *B, err);
+    k_expand<<<row_blocks, kBlock, 0, st>>>(n, g->row_ptr, g->col, g->a_val, g->b_val,
+                                            static_cast<const uint32_t*>(nullptr), *A, *B, err);
   EIG_HIP(hipGetLastError());
   int h_err = 0;
   EIG_HIP(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../../include/fem2d.h"
+#include "fp32r.hpp"
 #include "host/capi_internal.hpp"
 #include "pinned_staging.hpp"
 
@@ -89,10 +90,11 @@ __global__ void k_petsc_rows(long long n, long long nnz, const int64_t* __restri
 
 // Full rows in big-endian words: lower part from the sorted strict entries,
 // then the stored upper row (one wavefront per row).
-// V: the GEP's stored precision (FP32 values are widened to the format's f64)
+// V: the GEP's stored value type (FP32 values are widened to the format's f64);
+// res: FP32R residuals (fp32r.hpp), null otherwise
 template <class V>
 __global__ void k_petsc_fill(long long n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                             const V* __restrict__ a, const V* __restrict__ b,
+                             const V* __restrict__ a, const V* __restrict__ b, const uint32_t* __restrict__ res,
                              const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ row_of,
                              const int64_t* __restrict__ lower_start, uint32_t* __restrict__ col_be,
                              uint64_t* __restrict__ a_be, uint64_t* __restrict__ b_be) {
@@ -114,8 +116,14 @@ __global__ void k_petsc_fill(long long n, const int64_t* __restrict__ row_ptr, c
         c = uint32_t(col[k]);
       }
       col_be[out0 + j] = be32(c);
-      a_be[out0 + j] = be64(double(a[k]));
-      b_be[out0 + j] = be64(double(b[k]));
+      double av = double(a[k]), bv = double(b[k]);
+      if (res) {
+        const uint32_t rr = res[k];
+        av = fem2d::fp32r_join(float(av), fem2d::fp32r_res_a(rr));
+        bv = fem2d::fp32r_join(float(bv), fem2d::fp32r_res_b(rr));
+      }
+      a_be[out0 + j] = be64(av);
+      b_be[out0 + j] = be64(bv);
     }
   }
 }
@@ -194,6 +202,19 @@ int fem2d_gep_device_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep
     set_error("matrix too large for the PETSc binary format's 32-bit sizes", 0);
     return FEM2D_ERR_UNSUPPORTED;
   }
+  {
+    const int p = gep->precision;
+    const bool f32r = p == FEM2D_PRECISION_FP32R, f32 = p == FEM2D_PRECISION_FP32 || f32r;
+    if (p != FEM2D_PRECISION_FP64 && !f32) {
+      set_error("unknown precision", 0);
+      return FEM2D_ERR_INVALID_ARG;
+    }
+    if (!gep->row_ptr || (nnz > 0 && (!gep->col || (f32 ? !gep->a_val32 || !gep->b_val32 : !gep->a_val || !gep->b_val) ||
+                                      (f32r && !gep->res16)))) {
+      set_error("GEP arrays missing", 0);
+      return FEM2D_ERR_INVALID_ARG;
+    }
+  }
   PETSC_TRY(hipSetDevice(fem2d::capi::ctx_device(ctx)));
   hipStream_t st = static_cast<hipStream_t>(fem2d_ctx_stream(ctx));
   DevMem m;
@@ -242,12 +263,14 @@ int fem2d_gep_device_print_to_petsc_binary_files(fem2d_ctx* ctx, const fem2d_gep
   PETSC_TRY(m.alloc(&a_be, size_t(full)));
   PETSC_TRY(m.alloc(&b_be, size_t(full)));
   if (n > 0) {
-    if (gep->precision == FEM2D_PRECISION_FP32)
-      k_petsc_fill<<<std::max(1u, row_waves), block, 0, st>>>(n, gep->row_ptr, gep->col, gep->a_val32, gep->b_val32,
-                                                              sidx, row_of, lower_start, col_be, a_be, b_be);
+    if (gep->precision != FEM2D_PRECISION_FP64)
+      k_petsc_fill<<<std::max(1u, row_waves), block, 0, st>>>(
+          n, gep->row_ptr, gep->col, gep->a_val32, gep->b_val32,
+          gep->precision == FEM2D_PRECISION_FP32R ? gep->res16 : nullptr, sidx, row_of, lower_start, col_be, a_be, b_be);
     else
-      k_petsc_fill<<<std::max(1u, row_waves), block, 0, st>>>(n, gep->row_ptr, gep->col, gep->a_val, gep->b_val, sidx,
-                                                              row_of, lower_start, col_be, a_be, b_be);
+      k_petsc_fill<<<std::max(1u, row_waves), block, 0, st>>>(n, gep->row_ptr, gep->col, gep->a_val, gep->b_val,
+                                                              static_cast<const uint32_t*>(nullptr), sidx, row_of,
+                                                              lower_start, col_be, a_be, b_be);
     PETSC_TRY(hipGetLastError());
   }
   PETSC_TRY(hipStreamSynchronize(st));

@@ -18,10 +18,13 @@ from .errors import check
 
 MIN_GLQ_ORDER = 4  # galerkin.rs:13
 
-# precision of the assembled values (fem2d_precision): FP64 is the reference's; FP32
-# keeps the FP64 arithmetic and stores the A and B values rounded to FP32
+# precision of the assembled values (fem2d_precision): FP64 is the reference's; the
+# mixed modes keep the FP64 arithmetic and round at the store: FP32 stores A and B as
+# FP32 (12 B per entry), FP32R as FP32 plus a 16-bit residual each (16 B per entry,
+# relative entry error <= 2^-39)
 FP64 = N.PRECISION_FP64
 FP32 = N.PRECISION_FP32
+FP32R = N.PRECISION_FP32R
 
 
 # -- generic arguments of galerkin_sample_gep_hcurl ------------------------
@@ -216,7 +219,8 @@ def galerkin_sample_gep_hcurl(domain: Domain, glq_grid_dim: Optional[Sequence[in
     """Fill A (``AI``) and B (``BI``) over the Domain's basis space on the GPU (galerkin.rs:33-187).
 
     Raises WrongContinuityCondition, EmptyDOFSet or InvalidGLQSettings in the reference's order.
-    ``precision`` FP32 stores the values rounded to FP32 (returned widened to float64)."""
+    ``precision`` FP32 stores A and B rounded to FP32, FP32R as FP32 plus a 16-bit residual (returned
+    widened to float64)."""
     ctx = ctx or default_context()
     gu, gv = _glq(glq_grid_dim)
     out = N.GepCsr()

@@ -72,12 +72,20 @@ enum fem2d_basis { FEM2D_BASIS_HIERPOLY = 0, FEM2D_BASIS_KOL = 1 };
 /* HierCurlIntegral implementations (integrals.rs): CurlCurl and L2Inner */
 enum fem2d_integral { FEM2D_INTEGRAL_CURLCURL = 0, FEM2D_INTEGRAL_L2INNER = 1 };
 
-/* Precision of the assembled values. FP64: the reference's. FP32 (mixed): the same
- * FP64 arithmetic, the A and B values stored rounded to FP32, so a stored entry
- * takes 12 bytes of HBM instead of 20 (the pattern is the same). The separable
- * (sum-factorised) contraction leaves the assembly bound by these writes. Host
- * results (fem2d_gep_csr) hold FP32 values widened to double. */
-enum fem2d_precision { FEM2D_PRECISION_FP64 = 0, FEM2D_PRECISION_FP32 = 1 };
+/* Precision of the assembled values. FP64: the reference's. The mixed modes keep
+ * the same FP64 arithmetic and round values once, at the store; the pattern is
+ * the same.
+ * FP32: A and B stored as FP32, 12 bytes of HBM per stored entry instead of 20
+ *   (relative entry error <= 2^-24; the Rayleigh quotients of the lowest modes
+ *   move by up to ~1e-7).
+ * FP32R: A and B stored as FP32 plus a 16-bit residual each, 16 bytes per entry:
+ *   x = hi + q * 2^(e - 38) with hi = fl32(x), e the binary exponent of hi and q
+ *   a signed 16-bit integer, so the relative entry error is <= 2^-39 and hi alone
+ *   is the FP32 matrix.
+ * The separable (sum-factorised) contraction leaves the assembly bound by these
+ * writes. Host results (fem2d_gep_csr) hold the stored values widened to double
+ * (FP32R: hi plus its residual). */
+enum fem2d_precision { FEM2D_PRECISION_FP64 = 0, FEM2D_PRECISION_FP32 = 1, FEM2D_PRECISION_FP32R = 2 };
 
 /* BasisDir (basis_spec.rs:259-268) */
 enum fem2d_basis_dir { FEM2D_DIR_U = 0, FEM2D_DIR_V = 1, FEM2D_DIR_W = 2 };
@@ -175,12 +183,14 @@ typedef struct fem2d_gep_device {
   int64_t nnz;
   int64_t* row_ptr;
   int32_t* col;
-  double* a_val;     /* FEM2D_PRECISION_FP64 values (NULL for FP32) */
+  double* a_val;     /* FEM2D_PRECISION_FP64 values, else NULL */
   double* b_val;
   int32_t precision; /* fem2d_precision of the values */
-  float* a_val32;    /* FEM2D_PRECISION_FP32 values (NULL for FP64) */
+  float* a_val32;    /* FEM2D_PRECISION_FP32 and _FP32R values (FP32R: the FP32 parts), else NULL */
   float* b_val32;
   uint64_t generation; /* assembly that produced the handle; 0 for caller-owned arrays */
+  uint32_t* res16;   /* FEM2D_PRECISION_FP32R: per entry, A's 16-bit residual in the low half and B's
+                        in the high half (two's complement), else NULL */
 } fem2d_gep_device;
 
 int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* domain, fem2d_device_domain** out);

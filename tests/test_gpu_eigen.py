@@ -119,6 +119,9 @@ def test_waveguide_cap_and_dense_te10(ctx):
     # FP32-stored matrices: the same mode within the storage rounding
     p32 = F.solve_gep_dense(dd.assemble(w.glq, precision=F.FP32), 10.0, ctx=ctx)
     assert abs(p32.value - pair.value) / pair.value < 1e-6
+    # FP32R-stored matrices (FP32 parts plus residuals, expanded on the device): within 1e-10
+    p32r = F.solve_gep_dense(dd.assemble(w.glq, precision=F.FP32R), 10.0, ctx=ctx)
+    assert abs(p32r.value - pair.value) / pair.value < 1e-10
 
 
 def test_dense_solve_4496_dofs(ctx):

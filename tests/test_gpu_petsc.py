@@ -97,3 +97,19 @@ def test_missing_directory_is_an_io_error(ctx, tmp_path):
     g = F.galerkin_sample_gep_hcurl(dom, glq, ctx=ctx)
     with pytest.raises(OSError):
         g.print_to_petsc_binary_files(str(tmp_path / "absent"), "x", ctx=ctx)
+
+
+@pytest.mark.parametrize("precision", [F.FP32, F.FP32R])
+def test_device_gep_files_of_mixed_storage(ctx, tmp_path, precision):
+    """A GEP stored in a mixed mode: the files hold the stored values widened to f64 (FP32R: the
+    FP32 part plus its residual), the same values the download returns."""
+    print(f"[petsc] mixed storage precision={precision}", flush=True)
+    dom, glq = cases.small_hp_irregular()
+    dd = F.DeviceDomain(dom, ctx)
+    dev = dd.assemble(glq, F.KOLShapeFn, precision=precision)
+    d = _outdir(tmp_path)
+    dd.print_to_petsc_binary_files(dev, d, "mixed")
+    g = dd.download(dev)
+    for which in "ab":
+        got = open(os.path.join(d, "tmp", f"mixed_{which}.dat"), "rb").read()
+        assert got == _expected(g, which), which
