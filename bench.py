@@ -21,13 +21,16 @@ host arrays on both sides. `domain_build` times the host Domain construction
 from the workload's Mesh, data-parallel and sequential. `eigensolve` times the
 dense GPU eigensolves of assembled GEPs (with LAPACK on the host beside them);
 `petsc_writer` the PETSc AIJ files of the assembled GEP (with the oracle's writer
-beside it); `--fields` would time the field evaluation over every leaf of the Domain, whose
-GPU path is switched off in this build.
+beside it); `fields` the field evaluation over every leaf of the Domain; `sparse_eigensolve`
+the sparse shift-invert eigensolve of the workload's GEP near the TE10 cutoff;
+`other_workloads` the assembly of the hp-irregular (KOL, 1.78 M DoFs) and 256-guide
+configurations at full size, each with its fill roofline and tuned chunk size.
 `cpu_baseline` times the oracle (the C++
 restatement of the reference algorithm) on a bounded sample of the same
 workload on the host cores. `mixed_precision` repeats the measurement with the values
-stored as FP32 (FEM2D_PRECISION_FP32: FP64 arithmetic, 12 B per stored entry) and states its
-largest relative entry error against the FP64 values of the whole matrix.
+stored in the mixed modes (FP32: 12 B per stored entry; FP32R: FP32 plus a 16-bit residual,
+16 B) and states their largest relative entry errors against the FP64 values of the whole
+matrix. `assembly_roofline` sets the whole assembly's device time against the output bytes.
 """
 
 from __future__ import annotations
@@ -126,10 +129,12 @@ def _max_rel_errors(g32, g64):
 
 
 def mixed_precision(dd, w, args, nnz: int, ms_fp64: float, fill_fp64_ms: float):
-    """The same assembly with FEM2D_PRECISION_FP32: FP64 arithmetic, the A and B values stored
-    as FP32 (12 B per stored entry instead of 20). Timed like the FP64 steps (graph replays;
-    the fill kernel kernel by kernel); its values are compared with the FP64 GPU values of the
-    whole matrix."""
+    """The same assembly in the mixed storage modes, FP64 arithmetic in both:
+    FEM2D_PRECISION_FP32 (A and B stored as FP32, 12 B per stored entry instead of 20) and
+    FEM2D_PRECISION_FP32R (FP32 plus a 16-bit residual per value, 16 B per entry). Timed like the
+    FP64 steps (graph replays; the fill kernel kernel by kernel); their values are compared with
+    the FP64 GPU values of the whole matrix. The byte ratio bounds the fill's speed-up over FP64
+    (20/12 and 20/16), the write-bound kernel's roofline."""
     import numpy as np
 
     import fem_2d_amd as F
@@ -140,35 +145,136 @@ def mixed_precision(dd, w, args, nnz: int, ms_fp64: float, fill_fp64_ms: float):
         return dd.assemble(w.glq, w.basis, F.CurlCurl, F.L2Inner, precision=precision)
 
     g64 = dd.download(step(F.FP64))
-    for _ in range(args.warmup):
-        step(F.FP32)
+    out_modes = {}
+    for name, prec, bpe, desc in (("fp32", F.FP32, 12, "A and B stored as FP32 (FEM2D_PRECISION_FP32)"),
+                                  ("fp32r", F.FP32R, 16, "A and B stored as FP32 plus a 16-bit residual "
+                                                          "(FEM2D_PRECISION_FP32R)")):
+        for _ in range(args.warmup):
+            step(prec)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = step(prec)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        ctx.set_graphs(False)
+        step(prec)
+        fill_ms = []
+        for _ in range(args.steps):
+            step(prec)
+            fill_ms.append(ctx.last_timing()["fill_kernel_ms"])
+        ctx.set_graphs(True)
+        gm = dd.download(step(prec))
+        assert int(out.nnz) == nnz and gm.nnz == nnz
+        fill = float(np.mean(fill_ms))
+        achieved = bpe * nnz / (fill * 1e-3) / 1e9
+        out_modes[name] = {
+            "precision": desc + ", fp64 arithmetic",
+            "bytes_per_entry": bpe,
+            "value": nnz / (ms * 1e-3),
+            "unit": "nonzeros/s",
+            "ms_per_step": ms,
+            "speedup_vs_fp64": ms_fp64 / ms,
+            "fill_kernel_ms": fill,
+            "fill_speedup_vs_fp64": fill_fp64_ms / fill,
+            "fill_speedup_ceiling": BYTES_PER_ENTRY / bpe,
+            "fill_roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS},
+            "max_rel_entry_error_vs_fp64": _max_rel_errors(gm, g64),
+        }
+    return out_modes
+
+
+def other_workload(name: str, steps: int = 10, warmup: int = 3):
+    """Assembly throughput of another configuration at its full size: the timed graph replays,
+    the fill kernel kernel by kernel (its HBM roofline), and the chunk size the autotuning chose."""
+    import numpy as np
+
+    import fem_2d_amd as F
+
+    w = build_workload(name)
+    ctx = F.Context(0)
+    dd = F.DeviceDomain(w.domain, ctx)
+
+    def step():
+        return dd.assemble(w.glq, w.basis, F.CurlCurl, F.L2Inner)
+
+    for _ in range(warmup):
+        out = step()
     ctx.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = step(F.FP32)
+    for _ in range(steps):
+        out = step()
     ctx.synchronize()
-    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    nnz = int(out.nnz)
+    total = []
+    for _ in range(steps):
+        step()
+        total.append(ctx.last_timing()["total_ms"])
     ctx.set_graphs(False)
-    step(F.FP32)
-    fill_ms = []
-    for _ in range(args.steps):
-        step(F.FP32)
-        fill_ms.append(ctx.last_timing()["fill_kernel_ms"])
+    step()
+    fill, describe = [], []
+    for _ in range(steps):
+        step()
+        t = ctx.last_timing()
+        fill.append(t["fill_kernel_ms"])
+        describe.append(t["count_kernel_ms"])
     ctx.set_graphs(True)
-    g32 = dd.download(step(F.FP32))
-    assert int(out.nnz) == nnz and g32.nnz == nnz
-    fill = float(np.mean(fill_ms))
-    return {
-        "precision": "fp32 values, fp64 arithmetic (FEM2D_PRECISION_FP32)",
-        "value": nnz / (ms * 1e-3),
-        "unit": "nonzeros/s",
-        "ms_per_step": ms,
-        "speedup_vs_fp64": ms_fp64 / ms,
-        "fill_kernel_ms": fill,
-        "fill_speedup_vs_fp64": fill_fp64_ms / fill,
-        "fill_achieved_GBs": 12 * nnz / (fill * 1e-3) / 1e9,
-        "max_rel_entry_error_vs_fp64": _max_rel_errors(g32, g64),
-    }
+    t = ctx.last_timing()
+    fill_ms = float(np.mean(fill))
+    dev_ms = float(np.mean(total))
+    achieved = BYTES_PER_ENTRY * nnz / (fill_ms * 1e-3) / 1e9
+    ctx.close()
+    return {"value": nnz / (ms * 1e-3), "unit": "nonzeros/s", "ms_per_step": ms, "n_dofs": int(out.n),
+            "nnz_upper": nnz, "glq": list(w.glq), "basis": w.basis.__name__, "large_groups": int(t["n_large"]),
+            "fill_chunk": int(t["fill_chunk"]), "fill_lib_in_lds": bool(t["fill_lib_lds"]),
+            "device_ms": {"total": dev_ms, "describe_kernel": float(np.mean(describe)), "fill_kernel": fill_ms},
+            "roofline": {"kernel": "k_fill_flat", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_launch": BYTES_PER_ENTRY * nnz},
+            "assembly_roofline_frac": BYTES_PER_ENTRY * nnz / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def sparse_eigensolve(w):
+    """The sparse shift-invert eigensolve (the slepc_solve_gep stand-in) on the workload's
+    device-resident GEP near the TE10 cutoff: one call (ordering, block factorisation, iteration);
+    beside it, the GPU call and the oracle's ARPACK shift-invert (scipy, host) on the 19 296-DoF
+    hp-irregular test GEP."""
+    import numpy as np
+
+    import fem_2d_amd as F
+    from oracle import eigen as OE
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+
+    ctx = F.Context(0)
+    dd = F.DeviceDomain(w.domain, ctx)
+    dev = dd.assemble(w.glq, w.basis)
+    t0 = time.perf_counter()
+    pair = F.solve_gep_sparse(dev, 10.0, ctx=ctx)
+    ms = (time.perf_counter() - t0) * 1e3
+    out = {"solver": "block-tridiagonal level-set ordering, dense Schur-block inverses (rocSOLVER getrf/getri, "
+                     "rocBLAS gemm/gemv), shift-invert iteration in residual-correction form; FP64",
+           w.name: {"n": int(dev.n), "ms": ms, "eigenvalue": pair.value,
+                    "rel_err_vs_te10": abs(pair.value - np.pi ** 2) / np.pi ** 2}}
+    dom, glq = cases.small_hp_irregular()
+    dds = F.DeviceDomain(dom, ctx)
+    devs = dds.assemble(glq, F.KOLShapeFn)
+    F.solve_gep_sparse(devs, 5.0, ctx=ctx)
+    devs = dds.assemble(glq, F.KOLShapeFn)
+    t0 = time.perf_counter()
+    p = F.solve_gep_sparse(devs, 5.0, ctx=ctx)
+    gpu_ms = (time.perf_counter() - t0) * 1e3
+    A, B = dds.download(dds.assemble(glq, F.KOLShapeFn)).to_scipy()
+    t0 = time.perf_counter()
+    lam, _ = OE.sparse_solve_gep(A, B, 5.0)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    out["small_hp_irregular"] = {"n": int(A.shape[0]), "gpu_ms": gpu_ms, "eigenvalue": p.value,
+                                 "cpu_baseline": {"ms": cpu_ms, "eigenvalue": lam, "cores": 1,
+                                                  "kind": "ARPACK shift-invert Lanczos (scipy eigsh, SuperLU)"}}
+    ctx.close()
+    return out
 
 
 def host_boundary(w, nnz: int, calls: int = 3):
@@ -384,8 +490,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-eigen", action="store_true", help="skip timing the dense eigensolves")
     ap.add_argument("--no-petsc", action="store_true", help="skip timing the PETSc binary writer")
-    ap.add_argument("--fields", action="store_true",
-                    help="time the field evaluation (its GPU path is switched off in this build)")
+    ap.add_argument("--no-fields", action="store_true", help="skip timing the field evaluation")
+    ap.add_argument("--no-sparse-eigen", action="store_true",
+                    help="skip timing the sparse eigensolve of the workload's GEP")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="skip the hp-irregular and 256-guide assembly measurements")
     ap.add_argument("--no-domain-build", action="store_true",
                     help="skip timing the host Domain construction (parallel and sequential builders)")
     ap.add_argument("--no-mixed", action="store_true",
@@ -457,6 +566,7 @@ def main():
     ctx.set_graphs(True)
     fill_avg = float(np.mean(fill_ms))
     mixed = None if args.no_mixed else mixed_precision(dd, w, args, nnz, ms_per_step, fill_avg)
+    dev_total = float(np.mean(total_ms))
 
     # element pairs visited by the reference's loops (pattern-independent work count)
     a = w.domain.arrays()
@@ -498,10 +608,16 @@ def main():
             "basis": w.basis.__name__,
             "precision": "fp64",
             "large_groups": int(t_last["n_large"]),
+            "fill_chunk": int(t_last["fill_chunk"]),
+            "fill_lib_in_lds": bool(t_last["fill_lib_lds"]),
         },
         "element_pairs_per_s": pairs / (ms_per_step * 1e-3),
-        "device_ms": {"total": float(np.mean(total_ms)), "describe_kernel": float(np.mean(count_ms)),
+        "device_ms": {"total": dev_total, "describe_kernel": float(np.mean(count_ms)),
                       "fill_kernel": fill_avg},
+        # the whole assembly (every launch of the recorded graph) against the same output bytes
+        "assembly_roofline": {"achieved": BYTES_PER_ENTRY * nnz / (dev_total * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s",
+                              "frac": BYTES_PER_ENTRY * nnz / (dev_total * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "roofline": {
             "kernel": "k_fill_flat",
             "bound": "hbm",
@@ -524,8 +640,12 @@ def main():
         line["eigensolve"] = eigensolve()
     if not args.no_petsc:
         line["petsc_writer"] = petsc_writer(w)
-    if args.fields:
+    if not args.no_fields:
         line["fields"] = fields(w)
+    if not args.no_sparse_eigen:
+        line["sparse_eigensolve"] = sparse_eigensolve(w)
+    if not args.no_other_workloads and w.name == "waveguide_64x32":
+        line["other_workloads"] = {name: other_workload(name) for name in ("hp_irregular", "many_waveguides")}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, pairs, nnz, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -15,7 +15,7 @@ from .fields import UniformFieldSpace, xy_fields_array
 from .galerkin import (FP32, FP32R, FP64, GEP, MIN_GLQ_ORDER, Context, CurlCurl, DeviceDomain, HierCurlBasisFnSpace,
                        HierCurlIntegral, HierPoly, KOLShapeFn, L2Inner, ShapeFn, default_context,
                        galerkin_sample_gep_hcurl, gauss_quadrature_points)
-from .linalg import EigenPair, nalgebra_solve_gep, solve_gep_dense
+from .linalg import EigenPair, nalgebra_solve_gep, slepc_solve_gep, solve_gep_dense, solve_gep_sparse
 from .mesh import (MAX_POLYNOMIAL_ORDER, MIN_EDGE_LENGTH, Edge, Elem, HLevels, HRef, Mesh, Node, Orders, PolyOrders,
                    PRef)
 
@@ -26,7 +26,7 @@ __all__ = [
     "HierCurlIntegral", "HierPoly", "KOLShapeFn", "L2Inner", "ShapeFn", "default_context",
     "galerkin_sample_gep_hcurl", "gauss_quadrature_points", "MAX_POLYNOMIAL_ORDER", "MIN_EDGE_LENGTH", "Edge", "Elem",
     "HLevels", "HRef", "Mesh", "Node", "Orders", "PolyOrders", "PRef", "EigenPair", "nalgebra_solve_gep",
-    "solve_gep_dense", "NalgebraGEPError", "FailedToInvertB", "SpuriouslyConverged", "ProblemTooLarge",
+    "solve_gep_dense", "solve_gep_sparse", "slepc_solve_gep", "NalgebraGEPError", "FailedToInvertB", "SpuriouslyConverged", "ProblemTooLarge",
     "FailedToConverge", "UniformFieldSpace", "xy_fields_array", "UniformFieldError", "MismatchedSolutionSize",
     "MissingQuantity",
 ]

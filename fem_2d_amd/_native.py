@@ -93,6 +93,27 @@ class Timing(C.Structure):
         ("fill_kernel_ms", C.c_double),
         ("nnz", C.c_int64),
         ("n_large", C.c_int64),
+        ("fill_chunk", C.c_int64),
+        ("fill_lib_lds", C.c_int64),
+    ]
+
+
+class EigenStats(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("blocks", C.c_int32),
+        ("max_block", C.c_int32),
+        ("factor_bytes", C.c_double),
+        ("iterations", C.c_int32),
+        ("inner_iterations", C.c_int32),
+        ("residual", C.c_double),
+        ("max_inner_per_solve", C.c_double),
+        ("symbolic_ms", C.c_double),
+        ("factor_ms", C.c_double),
+        ("iterate_ms", C.c_double),
+        ("min_pivot", C.c_double),
+        ("precond_error", C.c_double),
+        ("bad_block", C.c_int32),
     ]
 
 
@@ -136,6 +157,8 @@ SIGNATURES = {
     "fem2d_ctx_destroy": (None, [vp]),
     "fem2d_ctx_stream": (vp, [vp]),
     "fem2d_ctx_synchronize": (C.c_int, [vp]),
+    "fem2d_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
+    "fem2d_ctx_last_eigen_stats": (C.c_int, [vp, P(EigenStats)]),
     "fem2d_galerkin_sample_gep_hcurl": (
         C.c_int, [vp, P(DomainView), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepCsr)]),
     "fem2d_gep_csr_free": (None, [P(GepCsr)]),
@@ -160,6 +183,8 @@ SIGNATURES = {
     "fem2d_gep_device_nalgebra_solve": (C.c_int, [vp, P(GepDevice), C.c_double, P(EigenPair)]),
     "fem2d_solve_gep_dense": (C.c_int, [vp, P(GepCsr), C.c_double, P(EigenPair)]),
     "fem2d_gep_device_solve_dense": (C.c_int, [vp, P(GepDevice), C.c_double, P(EigenPair)]),
+    "fem2d_solve_gep_sparse": (C.c_int, [vp, P(GepCsr), C.c_double, P(EigenPair)]),
+    "fem2d_gep_device_solve_sparse": (C.c_int, [vp, P(GepDevice), C.c_double, P(EigenPair)]),
     "fem2d_ctx_last_timing": (C.c_int, [vp, P(Timing)]),
     "fem2d_last_error": (C.c_char_p, []),
     "fem2d_last_error_detail": (C.c_int, []),

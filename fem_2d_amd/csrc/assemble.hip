@@ -44,7 +44,9 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fem2d.h"
@@ -117,7 +119,8 @@ constexpr int kFillWavesPerSimd = 6;
 // latency-bound kernels: at most 64 VGPRs, so that 8 wavefronts per SIMD are resident
 constexpr int kLatencyWavesPerSimd = 8;
 constexpr int kMaxDepth = 24;      // h-refinement depth below a base Elem
-constexpr int kMaxRelDepth = 12;   // dyadic halvings per axis covered by the Gram library
+constexpr int kMaxRanges = 16383;  // distinct relative sub-ranges per axis (14-bit ids in the block path)
+constexpr int kDenseRangeDepth = 12;  // deepest sub-range of the direct range map
 constexpr int kMaxOrder = 20;      // MAX_POLYNOMIAL_ORDER (mesh.rs:42)
 constexpr int kMaxGlq = 64;        // Gauss points per axis
 constexpr int kKinds = 5;          // Gram kinds
@@ -131,7 +134,7 @@ enum ErrBits : int {
   kErrView = 1,     // view violates Domain invariants
   kErrSize = 2,     // a per-group limit of this build was exceeded
   kErrContrib = 4,  // an entry with more than two contributions / a DoF with more than two specs
-  kErrDepth = 8,    // refinement deeper than kMaxDepth / kMaxRelDepth
+  kErrDepth = 8,    // refinement deeper than kMaxDepth
 };
 
 struct ElemGeo {
@@ -191,7 +194,14 @@ struct Work {
   int64_t* row_ptr;  // [N+1]
   double* lib;
   int lib_range_stride;  // library values per range: p1 * p1 * kKinds
-  int lib_nr;            // ranges per axis
+  int lib_nr;            // ranges of the x axis (the y axis's start at lib_nr)
+  const long long* ranges;  // [nr_u + nr_v] sorted keys depth << 32 | index of the sub-ranges, per axis
+  int lib_nr_v;             // ranges of the y axis
+  // for Domains at most kDenseRangeDepth halvings deep: each axis's library
+  // range by dyadic id 2^d - 1 + i (-1 where absent), the y axis's from
+  // range_map_v; null otherwise (the keys are searched)
+  const int* range_map;
+  int range_map_v;
   int* err;
   int* n_large;       // row groups handled by the block path
   int* large_list;
@@ -221,7 +231,8 @@ struct LibShape {
   int p1;      // orders 0..p1-1 per axis
   int tab;     // p1 * p1: values per kind and range
   int range_stride;  // kKinds * tab: values per range
-  int nr;      // ranges per axis
+  int nr;      // ranges of the x axis
+  int nr_v;    // ranges of the y axis
   int nu, nv;  // Gauss points per axis
   int basis;
 };
@@ -338,23 +349,42 @@ __device__ __forceinline__ double shape_pick(int which, int n, double x) {
   }
 }
 
-// dyadic sub-range id -> (scale, offset) of the mapped Gauss points
-__device__ __forceinline__ void range_of(int rid, double& s, double& c) {
-  int du = 31 - __clz(rid + 1);
-  int iu = rid + 1 - (1 << du);
-  s = ldexp(1.0, -du);
-  double lo = -1.0 + double(iu) * ldexp(1.0, 1 - du);
+// A dyadic sub-range of [-1, 1]: index i of the 2^d equal parts at depth d,
+// as the key d << 32 | i. The Gram library holds the sub-ranges that occur in
+// the Domain (the position of each Elem inside each of its ancestors, and the
+// whole range), sorted by key per axis.
+__host__ __device__ __forceinline__ long long range_key(int d, int i) { return ((long long)d << 32) | (unsigned)i; }
+
+// sub-range key -> (scale, offset) of the mapped Gauss points
+__device__ __forceinline__ void range_of(long long key, double& s, double& c) {
+  const int d = int(key >> 32);
+  const int i = int(key & 0xffffffffLL);
+  s = ldexp(1.0, -d);
+  const double lo = -1.0 + double(i) * ldexp(1.0, 1 - d);
   c = lo + s;
 }
 
-__device__ __forceinline__ int rel_range_id(int4 P, int4 Q, int axis) {
+// library range of Q's position inside its ancestor (or itself) P on one axis:
+// binary search of the axis's sorted keys; -1 if absent (an invalid view)
+__device__ __forceinline__ int rel_range_id(const Work& w, int4 P, int4 Q, int axis) {
   const int dp = axis == 0 ? P.x : P.z;
   const int ip = axis == 0 ? P.y : P.w;
   const int dq = axis == 0 ? Q.x : Q.z;
   const int iq = axis == 0 ? Q.y : Q.w;
   const int dd = dq - dp;
-  const int ii = iq - (ip << dd);
-  return (1 << dd) - 1 + ii;
+  if (w.range_map) {
+    const int rid = (1 << dd) - 1 + (iq - (ip << dd));
+    const int size = axis == 0 ? w.range_map_v : (1 << (kDenseRangeDepth + 1)) - 1;
+    return rid >= 0 && rid < size ? w.range_map[(axis == 0 ? 0 : w.range_map_v) + rid] : -1;
+  }
+  const long long key = range_key(dd, iq - (ip << dd));
+  const long long* keys = w.ranges + (axis == 0 ? 0 : w.lib_nr);
+  int lo = 0, hi = axis == 0 ? w.lib_nr : w.lib_nr_v;  // keys[lo] <= key < keys[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (keys[mid] <= key) lo = mid; else hi = mid;
+  }
+  return keys[lo] == key ? lo : -1;
 }
 
 // Block prefactors of one nested pair of Elems: P holds the p-side spec (the
@@ -370,8 +400,12 @@ __device__ __forceinline__ void block_coefs(const Work& w, int P, int Q, bool lo
   rv = 0;
   if (!local) {
     const int4 dp = w.dyad[P], dq = w.dyad[Q];
-    ru = rel_range_id(dp, dq, 0);
-    rv = rel_range_id(dp, dq, 1);
+    ru = rel_range_id(w, dp, dq, 0);
+    rv = rel_range_id(w, dp, dq, 1);
+    if (ru < 0 || rv < 0) {  // a position the upload did not see: the view is inconsistent
+      set_err(w.err, kErrView);
+      ru = rv = 0;
+    }
     su = ldexp(1.0, -(dq.x - dp.x));
     sv = ldexp(1.0, -(dq.z - dp.z));
   }
@@ -507,7 +541,6 @@ __device__ __forceinline__ void elem_prep_body(const DevView& v, const Work& w, 
       ++dv;
     }
   }
-  if (du > kMaxRelDepth || dv > kMaxRelDepth) set_err(w.err, kErrDepth);
   w.dyad[e] = make_int4(du, iu, dv, iv);
 
   const double* b = v.box + 4 * e;
@@ -900,21 +933,24 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
     s_wts[1][i] = glq.v_wts[i];
   }
   __syncthreads();
-  // layout [axis][range][kind][a][b]; the library holds fewer than 2^31 values
-  const unsigned per_axis = unsigned(L.nr) * unsigned(kKinds * L.p1 * L.p1);
+  // layout [axis][range][kind][a][b] (nr_u ranges on the x axis, nr_v on the
+  // y axis); the library holds fewer than 2^31 values
+  const unsigned stride = unsigned(kKinds * L.p1 * L.p1);
+  const unsigned x_part = unsigned(L.nr) * stride, total = unsigned(L.nr + L.nr_v) * stride;
   const unsigned t = unsigned(block) * blockDim.x + threadIdx.x;
-  if (block >= (2LL * per_axis + blockDim.x - 1) / blockDim.x || t >= 2u * per_axis) return;
+  if (block >= ((long long)total + blockDim.x - 1) / blockDim.x || t >= total) return;
   const unsigned p1 = unsigned(L.p1);
-  const int axis = t >= per_axis ? 1 : 0;
-  unsigned r = t - (axis ? per_axis : 0u);
+  const int axis = t >= x_part ? 1 : 0;
+  unsigned r = t - (axis ? x_part : 0u);
   const int b = int(r % p1);
   r /= p1;
   const int a = int(r % p1);
   r /= p1;
   const int kind = int(r % unsigned(kKinds));
   const int rid = int(r / unsigned(kKinds));
+  const long long key = w.ranges[(axis ? L.nr : 0) + rid];
   double s, c;
-  range_of(rid, s, c);
+  range_of(key, s, c);
   const int np = axis == 0 ? L.nu : L.nv;
   const double* pts = s_pts[axis];
   const double* wts = s_wts[axis];
@@ -931,7 +967,7 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
 #pragma unroll 1
   for (int m = 0; m < np; ++m) {
     const double xq = pts[m];
-    const double xp = rid == 0 ? xq : xq * s + c;
+    const double xp = key == 0 ? xq : xq * s + c;
     acc += wts[m] * shape_pick<kBasis>(fp, a, xp) * shape_pick<kBasis>(fq, b, xq);
   }
   w.lib[t] = acc;
@@ -1431,8 +1467,9 @@ __device__ __forceinline__ void store_entry(int* __restrict__ col_out, typename 
   }
 }
 
-// Streaming fill: the CSR is cut into chunks of kChunk consecutive entries
-// (w.chunk, which the layout scan used, equals Store<P>::kChunk);
+// Streaming fill: the CSR is cut into chunks of w.chunk consecutive entries
+// (the size the layout scan used: Store<P>::kChunk, or the one tuned for the
+// Domain);
 // wavefront v takes chunks v, v + waves, ..., so that the wavefronts write
 // neighbouring chunks at any time. For each row group a chunk meets, the
 // wavefront stages the group's source records in LDS and writes the entries
@@ -1454,7 +1491,7 @@ __global__ void __launch_bounds__(Store<P>::kBlock, kFillWavesPerSimd) k_fill_fl
   const long long nnz = w.row_ptr[w.n_dofs];
   if (nnz > o.capacity) return;  // the host grows the buffers and runs the fill again
   if (kLibInLds) {
-    const int n_lib = 2 * L.nr * L.range_stride;
+    const int n_lib = (L.nr + L.nr_v) * L.range_stride;
     for (int i = threadIdx.x; i < n_lib; i += blockDim.x) lib_lds[i] = w.lib[i];
     __syncthreads();
   }
@@ -1473,13 +1510,13 @@ __global__ void __launch_bounds__(Store<P>::kBlock, kFillWavesPerSimd) k_fill_fl
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
   const int waves = int((gridDim.x * blockDim.x) / kWave);
-  constexpr int kChunk = Store<P>::kChunk;
-  const long long n_chunks = min((nnz + kChunk - 1) / kChunk, w.chunk_cap);
+  const long long chunk = w.chunk;  // the layout scan's chunk size (default or tuned per Domain)
+  const long long n_chunks = min((nnz + chunk - 1) / chunk, w.chunk_cap);
   const double* __restrict__ lib = kLibInLds ? lib_lds : w.lib;
   const unsigned tab = unsigned(L.tab);
   for (long long ch = wave; ch < n_chunks; ch += waves) {
-    const long long pbase = ch * kChunk;
-    const int len = int(min((long long)kChunk, nnz - pbase));
+    const long long pbase = ch * chunk;
+    const int len = int(min(chunk, nnz - pbase));
     const long long pb = uniform(pbase);
     int* __restrict__ col_out = o.col + pb;
     VA* __restrict__ a_out = static_cast<VA*>(o.aval) + pb;
@@ -1596,8 +1633,27 @@ struct GroupShared {
   int doff0[kMaxDist], doff1[kMaxDist];
   int row_rel[kMaxRows + 1];
   unsigned int row_bs[2][kMaxRows];
+  int wave_sum[kWavesPerBlock];  // block-wide scans
   int n_src, n_raw, n_dist, total, u_self, fail;
 };
+
+// Exclusive block-wide prefix sum of one int per thread (kBlock threads);
+// *total receives the sum. Two barriers; every thread must call it.
+__device__ __forceinline__ int block_excl_scan(int x, int* wave_sum, int* total) {
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const int incl = wave_incl_scan(x, lane);
+  if (lane == kWave - 1) wave_sum[wid] = incl;
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int i = 0; i < kWavesPerBlock; ++i) {
+    before += i < wid ? wave_sum[i] : 0;
+    all += wave_sum[i];
+  }
+  __syncthreads();  // wave_sum is reused by the next scan
+  *total = all;
+  return before + incl - x;
+}
 
 static_assert(sizeof(GroupShared) <= 64 * 1024, "row-group LDS footprint must fit 64 KiB");
 
@@ -1674,42 +1730,94 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
       __syncthreads();
     }
   }
+  // dedupe, all threads: each takes 4 consecutive sorted keys; a key whose row
+  // group differs from its predecessor's starts a distinct group (first
+  // contribution), a repeat is the group's second contribution (a third is
+  // an error). Distinct groups are numbered by a block-wide prefix sum.
+  static_assert(kMaxRaw == 4 * kBlock && kMaxDist == 4 * kBlock, "four keys / groups per thread");
+  int h4[4], r4[4];
+  bool new4[4], valid4[4];
+  int n_new = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = 4 * tid + k;
+    valid4[k] = t < n_raw;
+    const unsigned long long kt = valid4[k] ? S.key[t] : 0ull;
+    h4[k] = int(kt >> 20);
+    r4[k] = int(kt & 0xFFFFF);
+    new4[k] = valid4[k] && (t == 0 || int(S.key[t - 1] >> 20) != h4[k]);
+    n_new += new4[k] ? 1 : 0;
+  }
+  int nd = 0;
+  const int first = block_excl_scan(n_new, S.wave_sum, &nd);
+  if (nd > kMaxDist) {  // block-uniform
+    if (tid == 0) set_err(w.err, kErrSize);
+    return false;
+  }
+  {
+    int d = first;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (new4[k]) {
+        const int gs = w.g_start[h4[k]];
+        S.dstart[d] = gs;
+        S.cum[d] = w.g_start[h4[k] + 1] - gs;  // the group's size, prefix-summed below
+        S.ds0[d] = S.raw_src[r4[k]];
+        S.doff0[d] = S.raw_off[r4[k]];
+        S.ds1[d] = -1;
+        S.doff1[d] = 0;
+        ++d;
+      }
+  }
   if (tid == 0) {
-    int nd = 0, run = 0, self = -1, prev = -1;
-    for (int t = 0; t < n_raw; ++t) {
-      const int h = int(S.key[t] >> 20);
-      const int r = int(S.key[t] & 0xFFFFF);
-      if (h != prev) {
-        if (nd >= kMaxDist) {
-          S.fail = 1;
-          break;
+    S.fail = 0;
+    S.u_self = -1;
+  }
+  __syncthreads();
+  {
+    int d = first;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (new4[k]) {
+        ++d;
+      } else if (valid4[k]) {
+        const int t = 4 * tid + k;
+        if (t >= 2 && int(S.key[t - 2] >> 20) == h4[k]) {
+          S.fail = 2;  // a third contribution to one candidate group
+        } else {
+          S.ds1[d - 1] = S.raw_src[r4[k]];
+          S.doff1[d - 1] = S.raw_off[r4[k]];
         }
-        const int gs = w.g_start[h];
-        S.dstart[nd] = gs;
-        S.cum[nd] = run;
-        S.ds0[nd] = S.raw_src[r];
-        S.doff0[nd] = S.raw_off[r];
-        S.ds1[nd] = -1;
-        S.doff1[nd] = 0;
-        if (h == g) self = run;
-        run += w.g_start[h + 1] - gs;
-        ++nd;
-        prev = h;
-      } else {
-        if (S.ds1[nd - 1] >= 0) {
-          S.fail = 2;
-          break;
-        }
-        S.ds1[nd - 1] = S.raw_src[r];
-        S.doff1[nd - 1] = S.raw_off[r];
       }
     }
-    S.cum[nd] = run;
-    S.n_dist = nd;
-    S.total = run;
-    S.u_self = self;
-    if (self < 0 && !S.fail) S.fail = 3;
   }
+  __syncthreads();
+  // the candidate columns before each distinct group; the group itself
+  int sz[4], n_cols = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    sz[k] = 4 * tid + k < nd ? S.cum[4 * tid + k] : 0;
+    n_cols += sz[k];
+  }
+  int total = 0;
+  int run = block_excl_scan(n_cols, S.wave_sum, &total);
+  const int gs_self = w.g_start[g];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int d = 4 * tid + k;
+    if (d < nd) {
+      S.cum[d] = run;
+      if (S.dstart[d] == gs_self) S.u_self = run;
+    }
+    run += sz[k];
+  }
+  if (tid == 0) {
+    S.cum[nd] = total;
+    S.n_dist = nd;
+    S.total = total;
+  }
+  __syncthreads();
+  if (tid == 0 && !S.fail && S.u_self < 0) S.fail = 3;  // the group is not among its own candidates
   __syncthreads();
   const int fail = S.fail;
   __syncthreads();
@@ -1969,10 +2077,12 @@ struct GraphKey {
   unsigned long long alloc_generation = 0;
   bool run_large = true;
   int precision = -1;
+  int chunk = 0;
+  int lib_lds = -1;
   bool operator==(const GraphKey& o) const {
     return domain == o.domain && basis == o.basis && ai == o.ai && bi == o.bi && nu == o.nu && nv == o.nv &&
            capacity == o.capacity && alloc_generation == o.alloc_generation && run_large == o.run_large &&
-           precision == o.precision;
+           precision == o.precision && chunk == o.chunk && lib_lds == o.lib_lds;
   }
 };
 
@@ -1988,12 +2098,20 @@ struct fem2d_device_domain {
   long long n_dofs = 0;
   long long n_bs = 0;
   int max_i = 0, max_j = 0;
-  int max_du = 0, max_dv = 0;
+  int nr_u = 0, nr_v = 0;      // distinct sub-ranges (Gram library ranges) per axis
+  long long* ranges = nullptr;  // [nr_u + nr_v] their keys, sorted per axis (device)
+  int* range_map = nullptr;     // direct map by dyadic id for shallow Domains (device), else null
+  int range_map_v = 0;          // offset of the y axis's map (the x axis's size)
   long long nested_cap = 0;  // sum over Elems of 1 + 2 * depth: entries of the nested lists
   long long n_groups = 0;    // row groups of the view's DoF signatures (sizes the per-group records)
   // whether any row group needs the block path: a property of the Domain,
   // learned by the first assembly (-1 unknown, 0 none, 1 some)
   mutable int large_state = -1;
+  // the streaming fill's chunk size and library placement (1 LDS, 0 global
+  // memory) per stored precision, chosen by timing the candidates on this
+  // Domain (chunk 0: not tuned yet)
+  mutable int tuned_chunk[kPrecisions] = {};
+  mutable int tuned_lds[kPrecisions] = {-1, -1, -1};
   // device arrays
   int* parent = nullptr;
   unsigned char* loc = nullptr;
@@ -2030,11 +2148,13 @@ struct fem2d_ctx {
   int fill_grid[kPrecisions] = {};
   int fill_grid_lds[kPrecisions] = {};
   bool use_graphs = true;
+  bool autotune = true;  // time the fill's chunk sizes on a Domain's first assembly
   hipGraphExec_t graph_exec[1] = {nullptr};  // the last recorded assembly
   GraphKey graph_key;
   void* solver = nullptr;  // eigensolver handle, made by the first solve
   // generation of the latest assembly whose outputs the buffers hold (0: none)
   unsigned long long out_generation = 0;
+  fem2d_eigen_stats eigen_stats{};  // of the last sparse eigensolve
 };
 
 namespace {
@@ -2044,6 +2164,7 @@ std::atomic<unsigned long long> g_gep_generation{0};
 }  // namespace
 
 int fem2d::capi::ctx_device(fem2d_ctx* ctx) { return ctx->device; }
+fem2d_eigen_stats* fem2d::capi::ctx_eigen_stats(fem2d_ctx* ctx) { return &ctx->eigen_stats; }
 
 int fem2d::capi::check_gep_handle(fem2d_ctx* ctx, const fem2d_gep_device* gep) {
   if (gep->generation != 0 && gep->generation != ctx->out_generation) {
@@ -2177,7 +2298,7 @@ int err_to_status(long long bits) {
     return FEM2D_ERR_INVALID_VIEW;
   }
   if (bits & kErrDepth) {
-    set_error("refinement deeper than this build supports (24 levels, 12 halvings per axis)", 0);
+    set_error("refinement deeper than this build supports (24 levels below a base Elem)", 0);
     return FEM2D_ERR_UNSUPPORTED;
   }
   set_error("a row group exceeds this build's limits (128 nested Elems, 1024 memberships or rows)", 0);
@@ -2185,9 +2306,9 @@ int err_to_status(long long bits) {
 }
 
 // ---------------------------------------------------------------------------
-// One assembly: its device arrays, launch shapes and kernel sequence; the
-// arrays are bound by a binder (fem2d_assemble_device: the context's
-// grow-only slots)
+// One assembly: its device arrays, launch shapes and kernel sequence, shared
+// by the assembly entry points (here fem2d_assemble_device, with its arrays in
+// the context's grow-only slots)
 // ---------------------------------------------------------------------------
 
 struct Assembly {
@@ -2200,6 +2321,7 @@ struct Assembly {
   long long *ts_nested = nullptr, *ts_dof = nullptr, *ts_mem = nullptr, *ts_grp = nullptr;
   unsigned slot_blocks = 0;
   bool run_large = true;  // launch the block path (unless the Domain is known to need none)
+  int lib_lds = -1;       // the fill's library copy: -1 in LDS when it fits, 0 global memory, 1 LDS
 };
 
 // Option and size checks after check_common; *nu, *nv receive the Gauss point counts.
@@ -2218,7 +2340,10 @@ int check_assembly(const fem2d_device_domain* dd, int glq_u, int glq_v, int* nu,
     set_error("expansion order above MAX_POLYNOMIAL_ORDER", 0);
     return FEM2D_ERR_INVALID_VIEW;
   }
-  if (dd->max_du > kMaxRelDepth || dd->max_dv > kMaxRelDepth) return err_to_status(kErrDepth);
+  if (dd->nr_u > kMaxRanges || dd->nr_v > kMaxRanges) {
+    set_error("more distinct refinement sub-ranges per axis than this build supports (16383)", 0);
+    return FEM2D_ERR_UNSUPPORTED;
+  }
   return FEM2D_OK;
 }
 
@@ -2260,13 +2385,18 @@ int bind_work(const fem2d_device_domain* dd, int nu, int nv, int basis, Bind&& b
   L.p1 = std::max(dd->max_i, dd->max_j) + 1;
   L.tab = L.p1 * L.p1;
   L.range_stride = kKinds * L.tab;
-  L.nr = (1 << (std::max(dd->max_du, dd->max_dv) + 1)) - 1;
+  L.nr = dd->nr_u;
+  L.nr_v = dd->nr_v;
   L.nu = nu;
   L.nv = nv;
   L.basis = basis;
-  a.lib_n = 2LL * L.nr * kKinds * L.p1 * L.p1;
+  a.lib_n = (long long)(L.nr + L.nr_v) * kKinds * L.p1 * L.p1;
   w.lib_range_stride = L.range_stride;
   w.lib_nr = L.nr;
+  w.lib_nr_v = L.nr_v;
+  w.ranges = dd->ranges;
+  w.range_map = dd->range_map;
+  w.range_map_v = dd->range_map_v;
   if ((rc = bind(k++, size_t(a.lib_n), &w.lib))) return rc;
 
   // tile sums of the four prefix sums, one region each (two run side by side)
@@ -2362,7 +2492,7 @@ int launch_prep(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, long long* st
 // buffers it writes nothing)
 int launch_stream_fill(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, const FillOut& o) {
   const int pi = o.precision;
-  const bool lds = lib_in_lds(o.precision, a.lib_n);
+  const bool lds = lib_in_lds(o.precision, a.lib_n) && a.lib_lds != 0;
   const unsigned grid = unsigned(lds ? ctx->fill_grid_lds[pi] : ctx->fill_grid[pi]);
   launch_fill(lds, o.a_is_cc != 0, o.b_is_cc != 0, grid, st, a.w, a.L, o);
   HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
@@ -2405,7 +2535,10 @@ FillOut make_fill_out(int precision, int ai, int bi, long long capacity, int* co
   return o;
 }
 
-// CSR entries per fill chunk for a stored precision
+// chunk sizes the fill's autotuning times on a Domain's first assembly
+constexpr int kChunkCandidates[] = {128, 256, 512, 1024};
+
+// CSR entries per fill chunk for a stored precision (before any tuning)
 int chunk_of(int precision) {
   switch (precision) {
     case FEM2D_PRECISION_FP32: return Store<FEM2D_PRECISION_FP32>::kChunk;
@@ -2485,6 +2618,24 @@ int fem2d_ctx_set_graphs(fem2d_ctx* ctx, int enable) {
   return FEM2D_OK;
 }
 
+int fem2d_ctx_last_eigen_stats(fem2d_ctx* ctx, fem2d_eigen_stats* out) {
+  if (!ctx || !out) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  *out = ctx->eigen_stats;
+  return FEM2D_OK;
+}
+
+int fem2d_ctx_set_autotune(fem2d_ctx* ctx, int enable) {
+  if (!ctx) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  ctx->autotune = enable != 0;
+  return FEM2D_OK;
+}
+
 int fem2d_ctx_synchronize(fem2d_ctx* ctx) {
   if (!ctx) {
     set_error("null argument", 0);
@@ -2545,22 +2696,59 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
   dd->n_elems = ne;
   dd->n_dofs = v->n_dofs;
   dd->n_bs = nbs;
-  // host metadata: maximum orders (Mesh::max_expansion_orders, mesh.rs:626-630)
-  // and the deepest dyadic subdivision per axis (sizes the Gram library)
+  // host metadata: maximum orders (Mesh::max_expansion_orders, mesh.rs:626-630),
+  // and the sub-ranges of the Gram library: the position of each Elem inside
+  // each of its ancestors, per axis (h_refinement.rs:247-279 halves the range
+  // per HRefLoc), plus the whole range
+  std::vector<long long> keys_u(1, range_key(0, 0)), keys_v(1, range_key(0, 0));
   for (int e = 0; e < ne; ++e) {
     dd->max_i = std::max<int>(dd->max_i, v->orders[2 * e]);
     dd->max_j = std::max<int>(dd->max_j, v->orders[2 * e + 1]);
-    int du = 0, dv = 0, steps = 0;
+    unsigned char locs[kMaxDepth + 1];
+    int steps = 0;
     for (int x = e; x >= 0 && v->parent[x] >= 0 && v->parent[x] < ne && steps <= kMaxDepth;
-         x = v->parent[x], ++steps) {
-      const int l = v->loc[x];
-      if (l <= FEM2D_LOC_E) ++du;
-      if (l <= FEM2D_LOC_NE || l >= FEM2D_LOC_S) ++dv;
-    }
-    dd->max_du = std::max(dd->max_du, du);
-    dd->max_dv = std::max(dd->max_dv, dv);
+         x = v->parent[x], ++steps)
+      locs[steps] = v->loc[x];
     dd->nested_cap += 1 + 2LL * steps;
+    if (steps > kMaxDepth) continue;  // rejected by the device checks
+    // Elem e inside its ancestor t levels up: the steps locs[t-1] (topmost) .. locs[0]
+    for (int t = 1; t <= steps; ++t) {
+      int du = 0, iu = 0, dv = 0, iv = 0;
+      for (int k = t - 1; k >= 0; --k) {
+        const int l = locs[k];
+        if (l <= FEM2D_LOC_E) {
+          iu = 2 * iu + (l == FEM2D_LOC_SE || l == FEM2D_LOC_NE || l == FEM2D_LOC_E ? 1 : 0);
+          ++du;
+        }
+        if (l <= FEM2D_LOC_NE || l >= FEM2D_LOC_S) {
+          iv = 2 * iv + (l == FEM2D_LOC_NW || l == FEM2D_LOC_NE || l == FEM2D_LOC_N ? 1 : 0);
+          ++dv;
+        }
+      }
+      keys_u.push_back(range_key(du, iu));
+      keys_v.push_back(range_key(dv, iv));
+    }
   }
+  for (auto* k : {&keys_u, &keys_v}) {
+    std::sort(k->begin(), k->end());
+    k->erase(std::unique(k->begin(), k->end()), k->end());
+  }
+  dd->nr_u = int(std::min<size_t>(keys_u.size(), size_t(1) << 30));
+  dd->nr_v = int(std::min<size_t>(keys_v.size(), size_t(1) << 30));
+  // shallow Domains (every sub-range at most kDenseRangeDepth halvings deep):
+  // a direct map by dyadic id, one load instead of a search per lookup
+  std::vector<int> range_map;
+  if (int(keys_u.back() >> 32) <= kDenseRangeDepth && int(keys_v.back() >> 32) <= kDenseRangeDepth) {
+    const int su = (1 << (int(keys_u.back() >> 32) + 1)) - 1;
+    const int sv = (1 << (kDenseRangeDepth + 1)) - 1;
+    range_map.assign(size_t(su + sv), -1);
+    for (size_t k = 0; k < keys_u.size(); ++k)
+      range_map[size_t((1 << int(keys_u[k] >> 32)) - 1 + int(keys_u[k] & 0xffffffffLL))] = int(k);
+    for (size_t k = 0; k < keys_v.size(); ++k)
+      range_map[size_t(su + (1 << int(keys_v[k] >> 32)) - 1 + int(keys_v[k] & 0xffffffffLL))] = int(k);
+    dd->range_map_v = su;
+  }
+  keys_u.insert(keys_u.end(), keys_v.begin(), keys_v.end());
   {
     // row groups: runs of DoFs whose basis specs lie on the same Elems (spec
     // lists are visited in Elem order, so the first Elem is the smaller one)
@@ -2584,7 +2772,9 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
       (rc = upload(off.data(), off.size(), &dd->bs_off)) ||
       (rc = upload(v->bs_dir, size_t(nbs), &dd->bs_dir)) || (rc = upload(v->bs_i, size_t(nbs), &dd->bs_i)) ||
       (rc = upload(v->bs_j, size_t(nbs), &dd->bs_j)) || (rc = upload(v->bs_dof, size_t(nbs), &dd->bs_dof)) ||
-      (rc = upload(v->dof_bs, size_t(2 * v->n_dofs), &dd->dof_bs))) {
+      (rc = upload(v->dof_bs, size_t(2 * v->n_dofs), &dd->dof_bs)) ||
+      (rc = upload(keys_u.data(), keys_u.size(), &dd->ranges)) ||
+      (!range_map.empty() && (rc = upload(range_map.data(), range_map.size(), &dd->range_map)))) {
     fem2d_device_domain_free(dd);
     return rc;
   }
@@ -2610,7 +2800,7 @@ void fem2d_device_domain_free(fem2d_device_domain* dd) {
   (void)hipSetDevice(dd->device);
   for (void* p : {(void*)dd->parent, (void*)dd->loc, (void*)dd->box, (void*)dd->mat, (void*)dd->bs_off,
                   (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof, (void*)dd->dof_bs,
-                  (void*)dd->leaves})
+                  (void*)dd->leaves, (void*)dd->ranges, (void*)dd->range_map})
     if (p) (void)hipFree(p);
   delete dd;
 }
@@ -2636,6 +2826,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
                      [&](int k, size_t count, auto** ptr) { return slot(ctx, k, count, ptr); }, a);
   if (!rc) rc = slot(ctx, kRowPtrSlot, size_t(nd + 1), &a.w.row_ptr);
   if (rc) return rc;
+  a.lib_lds = dd->tuned_lds[precision];
   Work& w = a.w;
   const bool run_large = a.run_large;
   hipStream_t st = ctx->stream;
@@ -2680,14 +2871,14 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       return rc;
     const FillOut o = make_fill_out(precision, ai, bi, ctx->out_capacity, static_cast<int*>(ctx->out_col.p),
                                     ctx->out_a.p, ctx->out_b.p, static_cast<unsigned*>(ctx->out_r.p));
-    w.chunk = chunk_of(precision);
+    w.chunk = dd->tuned_chunk[precision] > 0 ? dd->tuned_chunk[precision] : chunk_of(precision);
     w.chunk_cap = o.capacity / w.chunk + 1;
     if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
     const bool fill = o.capacity > 0;
     const bool tail = fill && run_large;  // part (c) has work
     if (ctx->use_graphs) {
       const GraphKey key{dd->serial, basis,     ai,       bi, nu, nv, o.capacity, g_alloc_generation.load(),
-                         run_large,  precision};
+                         run_large,  precision, w.chunk, a.lib_lds};
       if (!ctx->graph_exec[0] || !(key == ctx->graph_key)) {
         ctx->graph_key = GraphKey{};
         if ((rc = record(0, [&] {
@@ -2756,6 +2947,45 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     return fem2d_assemble_device(ctx, dd, basis, ai, bi, glq_u, glq_v, precision, out);
   }
   dd->large_state = n_large > 0 ? 1 : 0;
+  ctx->timing.fill_chunk = w.chunk;
+  ctx->timing.fill_lib_lds = lib_in_lds(precision, a.lib_n) && a.lib_lds != 0 ? 1 : 0;
+  if (ctx->autotune && dd->tuned_chunk[precision] == 0) {
+    // the first assembly of this Domain in this precision: time the streaming
+    // fill at each candidate chunk size and library placement (best of two
+    // launches each, after the layout scan for that size), keep the fastest,
+    // and assemble again with it
+    int best = w.chunk, best_lds = -1;
+    float best_ms = HUGE_VALF;
+    const FillOut o = make_fill_out(precision, ai, bi, ctx->out_capacity, static_cast<int*>(ctx->out_col.p),
+                                    ctx->out_a.p, ctx->out_b.p, static_cast<unsigned*>(ctx->out_r.p));
+    const bool fits = lib_in_lds(precision, a.lib_n);
+    for (int place = fits ? 1 : 0; place >= 0; --place) {
+      a.lib_lds = place;
+      for (int c : kChunkCandidates) {
+        w.chunk = c;
+        w.chunk_cap = o.capacity / c + 1;
+        if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
+        for (int rep = 0; rep < 2; ++rep) {
+          if ((rc = launch_prep(ctx, st, a, ctx->d_status, false))) return rc;
+          HIP_TRY(hipEventRecord(ctx->ev[6], st));
+          if ((rc = launch_stream_fill(ctx, st, a, o))) return rc;
+          HIP_TRY(hipEventRecord(ctx->ev[7], st));
+          HIP_TRY(hipStreamSynchronize(st));
+          float ms = 0.0f;
+          HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
+          if (ctx->h_status[1]) return err_to_status(ctx->h_status[1]);
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+            best_lds = place;
+          }
+        }
+      }
+    }
+    dd->tuned_chunk[precision] = best;
+    dd->tuned_lds[precision] = best_lds;
+    return fem2d_assemble_device(ctx, dd, basis, ai, bi, glq_u, glq_v, precision, out);
+  }
 
   out->n = nd;
   out->nnz = nnz;
@@ -2772,6 +3002,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   out->generation = ctx->out_generation;
   return FEM2D_OK;
 }
+
 
 int fem2d_device_malloc(fem2d_ctx* ctx, int64_t bytes, void** out) {
   if (!ctx || !out || bytes < 0) {

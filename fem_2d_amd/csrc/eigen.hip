@@ -28,15 +28,14 @@
 #include <vector>
 
 #include "../../include/fem2d.h"
+#include "eigen_common.hpp"
 #include "fp32r.hpp"
 #include "host/capi_internal.hpp"
 
 namespace {
 
-using fem2d::capi::set_error;
+using namespace fem2d::eig;
 
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
 constexpr double kSpurious = 1e-12;  // nalgebra_solve.rs:24
 
 __global__ void k_zero(double* __restrict__ p, long long count) {
@@ -69,9 +68,9 @@ __global__ void k_expand(long long n, const int64_t* __restrict__ row_ptr, const
       }
       double av = double(a[k]), bv = double(b[k]);
       if (res) {
-        const uint32_t r = res[k];
-        av = fem2d::fp32r_join(float(av), fem2d::fp32r_res_a(r));
-        bv = fem2d::fp32r_join(float(bv), fem2d::fp32r_res_b(r));
+        const uint32_t q = res[k];
+        av = fem2d::fp32r_join(float(av), fem2d::fp32r_res_a(q));
+        bv = fem2d::fp32r_join(float(bv), fem2d::fp32r_res_b(q));
       }
       A[r + c * n] = av;
       A[c + r * n] = av;
@@ -88,66 +87,6 @@ __global__ void k_lower_to_upper(long long n, double* __restrict__ M) {
     const long long i = idx % n, j = idx / n;
     if (i > j) M[j + i * n] = M[i + j * n];
   }
-}
-
-unsigned grid_for(long long count) {
-  return unsigned(std::max<long long>(1, std::min<long long>((count + kBlock - 1) / kBlock, 1 << 16)));
-}
-
-struct DevMem {
-  std::vector<void*> ptrs;
-  ~DevMem() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  template <class T>
-  hipError_t alloc(T** p, size_t count) {
-    void* q = nullptr;
-    const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 8);
-    if (e == hipSuccess) ptrs.push_back(q);
-    *p = static_cast<T*>(q);
-    return e;
-  }
-};
-
-int hip_fail(hipError_t e, const char* what) {
-  set_error(std::string(what) + ": " + hipGetErrorString(e), 0);
-  return e == hipErrorOutOfMemory ? FEM2D_ERR_OOM : FEM2D_ERR_HIP;
-}
-
-int blas_fail(rocblas_status s, const char* what) {
-  set_error(std::string(what) + ": " + rocblas_status_to_string(s), 0);
-  return s == rocblas_status_memory_error ? FEM2D_ERR_OOM : FEM2D_ERR_HIP;
-}
-
-#define EIG_HIP(expr)                                 \
-  do {                                                \
-    hipError_t _e = (expr);                           \
-    if (_e != hipSuccess) return hip_fail(_e, #expr); \
-  } while (0)
-
-#define EIG_BLAS(expr)                                               \
-  do {                                                               \
-    rocblas_status _s = (expr);                                      \
-    if (_s != rocblas_status_success) return blas_fail(_s, #expr);   \
-  } while (0)
-
-// The context's rocBLAS handle (made on first use, bound to the context's stream).
-int solver_handle(fem2d_ctx* ctx, hipStream_t st, rocblas_handle* out) {
-  void** slot = fem2d::capi::ctx_solver_slot(ctx);
-  if (!*slot) {
-    rocblas_handle h = nullptr;
-    EIG_BLAS(rocblas_create_handle(&h));
-    *slot = h;
-  }
-  *out = static_cast<rocblas_handle>(*slot);
-  EIG_BLAS(rocblas_set_stream(*out, st));
-  return FEM2D_OK;
-}
-
-int read_info(const rocblas_int* d_info, hipStream_t st, rocblas_int* h_info) {
-  EIG_HIP(hipMemcpyAsync(h_info, d_info, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
-  EIG_HIP(hipStreamSynchronize(st));
-  return FEM2D_OK;
 }
 
 // Dense A and B of a device GEP.
@@ -225,25 +164,6 @@ int pick(const std::vector<double>& D, const double* V, long long n, double targ
   out->value = D[best];
   out->n = n;
   out->vector = v;
-  return FEM2D_OK;
-}
-
-int check_device_args(fem2d_ctx* ctx, const fem2d_gep_device* gep, fem2d_eigenpair* out) {
-  if (!ctx || !gep || !out) {
-    set_error("null argument", 0);
-    return FEM2D_ERR_INVALID_ARG;
-  }
-  out->value = 0.0;
-  out->n = 0;
-  out->vector = nullptr;
-  if (int rc = fem2d::capi::check_gep_handle(ctx, gep)) return rc;
-  if (gep->n < 0) {
-    set_error("negative GEP size", 0);
-    return FEM2D_ERR_INVALID_ARG;
-  }
-  out->value = 0.0;
-  out->n = 0;
-  out->vector = nullptr;
   return FEM2D_OK;
 }
 
@@ -337,16 +257,7 @@ int inverse_iteration(rocblas_handle h, hipStream_t st, long long n, const doubl
   rocblas_int info = 0;
   if (int rc = read_info(d_info, st, &info)) return rc;
   if (info != 0) return 1;  // the target is an eigenvalue to working precision
-  // a fixed start vector with components in [0.5, 1.5) (SplitMix64)
-  std::vector<double> x0(static_cast<size_t>(n));
-  unsigned long long state = 0x9E3779B97F4A7C15ull;
-  for (auto& v : x0) {
-    unsigned long long z = (state += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    v = 0.5 + double(z >> 11) * 0x1.0p-53;
-  }
+  const std::vector<double> x0 = start_vector(n);
   EIG_HIP(hipMemcpyAsync(x, x0.data(), size_t(n) * sizeof(double), hipMemcpyHostToDevice, st));
   EIG_HIP(hipStreamSynchronize(st));  // x0 is read before any return below
   const double one = 1.0, zero = 0.0;
@@ -466,51 +377,6 @@ int dense_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem2
   EIG_HIP(hipMemcpyAsync(hD.data(), D, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, st));
   EIG_HIP(hipStreamSynchronize(st));
   return pick(hD, A, n, target, st, out);
-}
-
-// A host CSR on the device (checked on the host before any device read).
-int upload(const fem2d_gep_csr* gep, DevMem& m, fem2d_gep_device* d) {
-  if (gep->n < 0 || gep->nnz < 0 || !gep->row_ptr || (gep->nnz > 0 && (!gep->col || !gep->a_val || !gep->b_val))) {
-    set_error("GEP sizes or arrays invalid", 0);
-    return FEM2D_ERR_INVALID_ARG;
-  }
-  for (long long r = 0; r < gep->n; ++r)
-    if (gep->row_ptr[r + 1] < gep->row_ptr[r]) {
-      set_error("row_ptr decreases", 0);
-      return FEM2D_ERR_INVALID_ARG;
-    }
-  if (gep->row_ptr[0] != 0 || gep->row_ptr[gep->n] != gep->nnz) {
-    set_error("row_ptr does not span [0, nnz]", 0);
-    return FEM2D_ERR_INVALID_ARG;
-  }
-  *d = fem2d_gep_device{};
-  d->n = gep->n;
-  d->nnz = gep->nnz;
-  d->precision = FEM2D_PRECISION_FP64;
-  EIG_HIP(m.alloc(&d->row_ptr, size_t(gep->n + 1)));
-  EIG_HIP(m.alloc(&d->col, size_t(gep->nnz)));
-  EIG_HIP(m.alloc(&d->a_val, size_t(gep->nnz)));
-  EIG_HIP(m.alloc(&d->b_val, size_t(gep->nnz)));
-  EIG_HIP(hipMemcpy(d->row_ptr, gep->row_ptr, size_t(gep->n + 1) * 8, hipMemcpyHostToDevice));
-  if (gep->nnz > 0) {
-    EIG_HIP(hipMemcpy(d->col, gep->col, size_t(gep->nnz) * 4, hipMemcpyHostToDevice));
-    EIG_HIP(hipMemcpy(d->a_val, gep->a_val, size_t(gep->nnz) * 8, hipMemcpyHostToDevice));
-    EIG_HIP(hipMemcpy(d->b_val, gep->b_val, size_t(gep->nnz) * 8, hipMemcpyHostToDevice));
-  }
-  return FEM2D_OK;
-}
-
-template <class Solve>
-int solve_host(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target, fem2d_eigenpair* out, Solve solve) {
-  if (!ctx || !gep || !out) {
-    set_error("null argument", 0);
-    return FEM2D_ERR_INVALID_ARG;
-  }
-  EIG_HIP(hipSetDevice(fem2d::capi::ctx_device(ctx)));
-  DevMem m;
-  fem2d_gep_device d;
-  if (int rc = upload(gep, m, &d)) return rc;
-  return solve(ctx, &d, target, out);
 }
 
 }  // namespace

@@ -5,6 +5,7 @@
 
 struct fem2d_ctx;
 struct fem2d_gep_device;
+struct fem2d_eigen_stats;
 
 namespace fem2d {
 
@@ -21,6 +22,8 @@ int ctx_device(fem2d_ctx* ctx);
 void** ctx_solver_slot(fem2d_ctx* ctx);
 // destroys a handle held in that slot
 void release_solver(void* handle);
+// the context's statistics of its last sparse eigensolve
+fem2d_eigen_stats* ctx_eigen_stats(fem2d_ctx* ctx);
 // FEM2D_OK for a device GEP handle that may be read on this context: one the
 // caller built over arrays of its own (generation 0), or the latest assembly
 // of this context; FEM2D_ERR_INVALID_ARG (message set) for a stale handle

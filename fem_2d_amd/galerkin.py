@@ -93,6 +93,17 @@ class Context:
         """Record each assembly once as a HIP graph and replay it (default), or launch kernel by kernel."""
         check(self._lib.fem2d_ctx_set_graphs(self._h, 1 if enable else 0))
 
+    def set_autotune(self, enable: bool) -> None:
+        """Time the entry fill's chunk sizes on each Domain's first assembly and keep the fastest
+        (default), or keep the default chunk size."""
+        check(self._lib.fem2d_ctx_set_autotune(self._h, 1 if enable else 0))
+
+    def last_eigen_stats(self) -> dict:
+        """Statistics of the last sparse eigensolve on this context."""
+        s = N.EigenStats()
+        check(self._lib.fem2d_ctx_last_eigen_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in N.EigenStats._fields_}
+
     def synchronize(self) -> None:
         """Wait for all work on the context's device."""
         check(self._lib.fem2d_ctx_synchronize(self._h))
@@ -106,7 +117,7 @@ class Context:
         check(self._lib.fem2d_ctx_last_timing(self._h, C.byref(t)))
         return dict(total_ms=t.total_ms, prep_ms=t.prep_ms, symbolic_ms=t.symbolic_ms, fill_ms=t.fill_ms,
                     n_groups=t.n_groups, count_kernel_ms=t.count_kernel_ms, fill_kernel_ms=t.fill_kernel_ms,
-                    nnz=t.nnz, n_large=t.n_large)
+                    nnz=t.nnz, n_large=t.n_large, fill_chunk=t.fill_chunk, fill_lib_lds=t.fill_lib_lds)
 
 
 _default_ctx: Optional[Context] = None

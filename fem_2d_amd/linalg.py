@@ -1,9 +1,10 @@
-"""Dense generalized eigensolves of an assembled GEP on the GPU: the drop-ins for
-``nalgebra_solve_gep`` (reference src/fem_problem/linalg/nalgebra_solve.rs:14-50) and, up to
-the dense solver's size limit, ``slepc_solve_gep`` (linalg/slepc_solve.rs:14-68).
+"""Generalized eigensolves of an assembled GEP on the GPU: the drop-ins for
+``nalgebra_solve_gep`` (reference src/fem_problem/linalg/nalgebra_solve.rs:14-50) and
+``slepc_solve_gep`` (linalg/slepc_solve.rs:14-68): dense up to the dense solver's size limit,
+sparse (block-tridiagonal level-set ordering, dense Schur blocks) beyond.
 
-Both run in libfem2d_amd.so (rocSOLVER on the context's stream after the library's own
-CSR-to-dense kernel); there is no CPU fallback. Each takes either a host :class:`GEP` or a
+All run in libfem2d_amd.so (rocSOLVER / rocBLAS on the context's stream after the library's
+own kernels); there is no CPU fallback. Each takes either a host :class:`GEP` or a
 device-resident GEP as :meth:`DeviceDomain.assemble` returns it, which is solved in HBM without
 a download."""
 
@@ -67,3 +68,27 @@ def solve_gep_dense(gep: Union[GEP, N.GepDevice], target_eigenvalue: float,
     (shift-invert inverse iteration; the whole spectrum when that is inconclusive), its vector
     B-normalised. Raises ProblemTooLarge, FailedToInvertB, SpuriouslyConverged or FailedToConverge."""
     return _solve("fem2d_solve_gep_dense", "fem2d_gep_device_solve_dense", gep, target_eigenvalue, ctx)
+
+
+def solve_gep_sparse(gep: Union[GEP, N.GepDevice], target_eigenvalue: float,
+                     ctx: Optional[Context] = None) -> EigenPair:
+    """The eigenpair of A x = lambda B x nearest ``target_eigenvalue`` for GEPs of any size whose
+    factor fits the device memory, standing in for slepc_solve_gep (slepc_solve.rs:14-68):
+    shift-invert inverse iteration on a block-tridiagonal (breadth-first level set) ordering of
+    A - target B, its dense Schur-complement blocks inverted on the GPU; the vector B-normalised.
+    Raises ProblemTooLarge or FailedToConverge."""
+    return _solve("fem2d_solve_gep_sparse", "fem2d_gep_device_solve_sparse", gep, target_eigenvalue, ctx)
+
+
+# the dense solve is used up to this many DoFs by slepc_solve_gep, the sparse one beyond
+SLEPC_DENSE_LIMIT = 4096
+
+
+def slepc_solve_gep(gep: Union[GEP, N.GepDevice], target_eigenvalue: float,
+                    ctx: Optional[Context] = None) -> EigenPair:
+    """slepc_solve_gep (slepc_solve.rs:14-68) without the SLEPc process: the eigenpair nearest
+    ``target_eigenvalue``, from the dense solve up to SLEPC_DENSE_LIMIT DoFs and from the sparse
+    one beyond."""
+    if gep.n <= SLEPC_DENSE_LIMIT:
+        return solve_gep_dense(gep, target_eigenvalue, ctx)
+    return solve_gep_sparse(gep, target_eigenvalue, ctx)

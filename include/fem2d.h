@@ -153,6 +153,12 @@ int fem2d_ctx_synchronize(fem2d_ctx* ctx);
  * replayed (default 1) or launched kernel by kernel (0). Results are the same;
  * graphs remove the per-kernel launch gaps of the ~20-kernel pipeline. */
 int fem2d_ctx_set_graphs(fem2d_ctx* ctx, int enable);
+/* Whether the first assembly of a Domain in a precision times the entry fill at
+ * each candidate chunk size (128, 256, 512, 1024 CSR entries per work item) and
+ * library placement (an LDS copy per block when it fits, or global memory) and
+ * keeps the fastest for later assemblies of that Domain (default 1). The tuned
+ * shape changes timing only, never results. */
+int fem2d_ctx_set_autotune(fem2d_ctx* ctx, int enable);
 
 /* Replaces galerkin_sample_gep_hcurl (galerkin.rs:33-187).
  * basis: fem2d_basis; a_integral / b_integral: fem2d_integral (the AI / BI
@@ -254,6 +260,44 @@ int fem2d_solve_gep_dense(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double targe
 int fem2d_gep_device_solve_dense(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
                                  fem2d_eigenpair* out);
 
+/* The same problem for GEPs of any size whose factor fits the device memory,
+ * standing in for slepc_solve_gep (slepc_solve.rs:14-68), which runs SLEPc's
+ * shift-and-invert around the target: the eigenpair nearest target_eigenvalue,
+ * the vector B-normalised. The matrix graph is ordered by breadth-first level
+ * sets (host, from the pattern), which makes A - target B block tridiagonal;
+ * the SPD matrix A + tau B (tau = max(|target|, 1)) is block-Cholesky factored
+ * on the device (rocSOLVER / rocBLAS, dense Schur-complement blocks) and
+ * preconditions GMRES solves with A - target B inside a shift-invert inverse
+ * iteration that runs against the sparse matrices in place. It converges when the relative residual
+ * |A x - lambda B x| / max(|A x|, |target| |B x|) reaches 1e-12 (1e-10 once the
+ * Rayleigh quotient stops moving). Device memory: 8 bytes times the sum of the
+ * squared block widths (about 21 GB for the 445 380-DoF benchmark GEP).
+ * Errors: FEM2D_ERR_PROBLEM_TOO_LARGE (the blocks do not fit the free device
+ * memory), FEM2D_ERR_FAILED_TO_INVERT_B (A + tau B is not positive definite),
+ * FEM2D_ERR_SPURIOUSLY_CONVERGED (no DoFs), FEM2D_ERR_NOT_CONVERGED. */
+int fem2d_solve_gep_sparse(fem2d_ctx* ctx, const fem2d_gep_csr* gep, double target_eigenvalue,
+                           fem2d_eigenpair* out);
+/* Statistics of the last sparse eigensolve on a context (also after a failure). */
+typedef struct fem2d_eigen_stats {
+  int64_t n;                 /* DoFs */
+  int32_t blocks;            /* blocks of the level ordering */
+  int32_t max_block;         /* widest block */
+  double factor_bytes;       /* device bytes of the kept block Cholesky factors */
+  int32_t iterations;        /* outer (shift-invert) iterations */
+  int32_t inner_iterations;  /* preconditioned GMRES steps over all outer iterations */
+  double residual;           /* final relative residual */
+  double max_inner_per_solve; /* most GMRES steps of one inner solve */
+  double symbolic_ms;        /* host ordering and entry lists */
+  double factor_ms;          /* block factorisation on the device */
+  double iterate_ms;         /* the iteration */
+  double min_pivot;          /* smallest squared diagonal entry of the Cholesky factors */
+  double precond_error;      /* |M (A + tau B) z - z| / |z| for a fixed z: the factorisation's accuracy */
+  int32_t bad_block;         /* first block that is not positive definite, -1 for none */
+} fem2d_eigen_stats;
+int fem2d_ctx_last_eigen_stats(fem2d_ctx* ctx, fem2d_eigen_stats* out);
+int fem2d_gep_device_solve_sparse(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target_eigenvalue,
+                                  fem2d_eigenpair* out);
+
 /* ---------------------------------------------------------------------------
  * Field evaluation: UniformFieldSpace::xy_fields (fields.rs:63-127)
  * ------------------------------------------------------------------------- */
@@ -299,6 +343,8 @@ typedef struct fem2d_timing {
   double fill_kernel_ms;  /* the wave-per-group entry-writing kernel alone */
   int64_t nnz;
   int64_t n_large;    /* row groups assembled one block per group (large column sets) */
+  int64_t fill_chunk; /* CSR entries per work item of the entry fill (default or tuned) */
+  int64_t fill_lib_lds; /* 1 when the entry fill read the Gram library from an LDS copy */
 } fem2d_timing;
 int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out);
 

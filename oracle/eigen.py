@@ -9,7 +9,12 @@ CPU restatements of the two eigensolves that follow the assembly, on dense numpy
   magnitude -> SpuriouslyConverged (24-26); else the first eigenvalue with the smallest distance to
   the target and its eigenvector (28-46; nalgebra's vectors have unit 2-norm).
 * the symmetric-definite problem A x = lambda B x that slepc_solve_gep (slepc_solve.rs:14-68)
-  hands to SLEPc: LAPACK dsygvd through scipy, the eigenpair nearest the target, B-normalised.
+  hands to SLEPc: LAPACK dsygvd through scipy, the eigenpair nearest the target, B-normalised;
+  and, for sparse matrices of any size, ARPACK's shift-invert Lanczos around the target (scipy
+  eigsh), SLEPc's own method for a target eigenvalue.
+
+The sparse restatement is pinned against the dense one on small GEPs, the SLEPc eigenpair below
+and the TE10 cutoff pi^2 of the 1096-DoF waveguide (tests/test_oracle_sparse_eigen.py).
 
 Pinned by the reference's known answers, which the tests check against these functions: the
 nalgebra eigenvalue 2.6479657 of lib.rs:45-80 (tolerance 1e-6) - which only the lower-triangle
@@ -76,3 +81,25 @@ def dense_solve_gep(A: np.ndarray, B: np.ndarray, target: float):
         raise SpuriouslyConverged()
     k = _nearest(w, target)
     return float(w[k]), V[:, k]
+
+
+def sparse_solve_gep(A, B, target: float):
+    """(value, B-normalised vector) of A x = lambda B x nearest target for sparse symmetric A, B
+    (scipy.sparse matrices): ARPACK's shift-invert Lanczos (scipy eigsh with sigma = target, a
+    sparse LU of A - target B), the way slepc_solve_gep's SLEPc run is driven with a target."""
+    import scipy.sparse.linalg as spl
+
+    w, V = spl.eigsh(A.tocsc(), k=1, M=B.tocsc(), sigma=target, which="LM", tol=1e-13,
+                     v0=np.ones(A.shape[0]))
+    x = V[:, 0]
+    x = x / np.sqrt(x @ (B @ x))
+    return float(w[0]), x
+
+
+def sparse_eigenvalues_near(A, B, sigma: float, k: int):
+    """The k eigenvalues of A x = lambda B x nearest sigma (ARPACK shift-invert), ascending."""
+    import scipy.sparse.linalg as spl
+
+    w = spl.eigsh(A.tocsc(), k=k, M=B.tocsc(), sigma=sigma, which="LM", tol=1e-13,
+                  v0=np.ones(A.shape[0]), return_eigenvectors=False)
+    return np.sort(w)

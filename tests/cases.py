@@ -81,6 +81,22 @@ def proper_edge_order_mesh():
     return F.Domain.from_mesh(m), (6, 6)
 
 
+def deep_corner():
+    """A 2 x 2 square refined 15 levels deep toward its SW corner (T; the 16th would take its edges
+    below MIN_EDGE_LENGTH, mesh.rs:34-36), with U and V refinements on two leaves 14 levels down;
+    orders (2, 3)."""
+    m = F.Mesh.rect_grid(1, 1, 0.0, 2.0, 0.0, 2.0)
+    m.set_global_expansion_orders(F.Orders(2, 3))
+    target = 0
+    for _ in range(15):
+        m.h_refine_elems([target], F.HRef.T)
+        target = m.elem(target).children[0]
+    uncles = m.elem(m.elem(m.elem(target).parent).parent).children  # leaves 14 levels down
+    m.h_refine_elems([uncles[1]], F.HRef.u())
+    m.h_refine_elems([uncles[2]], F.HRef.v())
+    return F.Domain.from_mesh(m), (5, 6)
+
+
 def waveguide_small():
     w = W.waveguide_small()
     return w.domain, w.glq
@@ -103,6 +119,7 @@ ALL = {
     "mixed_refinements": mixed_refinements,
     "create_domain": create_domain_case,
     "deep_unit": deep_unit,
+    "deep_corner": deep_corner,
     "proper_edge_order": proper_edge_order_mesh,
     "waveguide_small": waveguide_small,
     "small_hp_irregular": small_hp_irregular,
