@@ -78,6 +78,7 @@ class EigenPair(C.Structure):
     _fields_ = [("value", C.c_double), ("n", C.c_int64), ("vector", C.POINTER(C.c_double))]
 
 
+NOALLOC_SCRATCH_BYTES = 64 << 20  # FEM2D_NOALLOC_SCRATCH_BYTES
 NALGEBRA_MAX_DENSE_SIZE = 1000  # FEM2D_NALGEBRA_MAX_DENSE_SIZE (nalgebra_solve.rs:6)
 DENSE_GEP_MAX_SIZE = 32768  # FEM2D_DENSE_GEP_MAX_SIZE
 
@@ -167,6 +168,8 @@ SIGNATURES = {
     "fem2d_assemble_device": (
         C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepDevice)]),
     "fem2d_gep_device_download": (C.c_int, [vp, P(GepDevice), P(GepCsr)]),
+    "fem2d_assemble_device_noalloc": (
+        C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i64, vp, vp, vp, vp, vp, P(i64)]),
     "fem2d_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
     "fem2d_gep_device_print_to_petsc_binary_files": (C.c_int, [vp, P(GepDevice), C.c_char_p, C.c_char_p]),
     "fem2d_gep_csr_print_to_petsc_binary_files": (C.c_int, [vp, P(GepCsr), C.c_char_p, C.c_char_p]),

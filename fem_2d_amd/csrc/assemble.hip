@@ -2161,6 +2161,37 @@ namespace {
 // generations of device GEP handles, unique in the process
 std::atomic<unsigned long long> g_gep_generation{0};
 
+// Work memory of fem2d_assemble_device_noalloc: one static array per device,
+// held until the process ends. Host code never passes `scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): the
+// device address of each device's copy is looked up once, with that device
+// current, by fem2d_ctx_create, and only that address is used.
+constexpr size_t kScratchBytes = size_t(FEM2D_NOALLOC_SCRATCH_BYTES);
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// device address of this device's scratch array (the device is current)
+int register_scratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return FEM2D_OK;  // the no-allocation entry point rejects it
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return FEM2D_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fem2d::capi::set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e), 0);
+    return e == hipErrorOutOfMemory ? FEM2D_ERR_OOM : FEM2D_ERR_HIP;
+  }
+  g_scratch_ptr[device] = p;
+  return FEM2D_OK;
+}
+
+void* scratch_of(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 }  // namespace
 
 int fem2d::capi::ctx_device(fem2d_ctx* ctx) { return ctx->device; }
@@ -2307,8 +2338,8 @@ int err_to_status(long long bits) {
 
 // ---------------------------------------------------------------------------
 // One assembly: its device arrays, launch shapes and kernel sequence, shared
-// by the assembly entry points (here fem2d_assemble_device, with its arrays in
-// the context's grow-only slots)
+// by fem2d_assemble_device (arrays in the context's grow-only slots) and
+// fem2d_assemble_device_noalloc (arrays carved from the static scratch array)
 // ---------------------------------------------------------------------------
 
 struct Assembly {
@@ -2585,6 +2616,10 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fill_large),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GroupShared))));
+  if (int rc = register_scratch(device)) {
+    fem2d_ctx_destroy(ctx);
+    return rc;
+  }
   *out = ctx;
   return FEM2D_OK;
 }
@@ -3003,6 +3038,87 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   return FEM2D_OK;
 }
 
+int fem2d_assemble_device_noalloc(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int ai, int bi,
+                                  int glq_u, int glq_v, int precision, int64_t capacity, int64_t* row_ptr,
+                                  int32_t* col, void* a_val, void* b_val, uint32_t* res16, int64_t* nnz_out) {
+  if (!ctx || !dd || !nnz_out) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  *nnz_out = -1;
+  if (int rc = check_common(dd->cc, dd->n_dofs, glq_u, glq_v, basis, ai, bi, precision)) return rc;
+  int nu = 0, nv = 0;
+  if (int rc = check_assembly(dd, glq_u, glq_v, &nu, &nv)) return rc;
+  if (capacity < 0 || !row_ptr ||
+      (capacity > 0 && (!col || !a_val || !b_val || (precision == FEM2D_PRECISION_FP32R && !res16)))) {
+    set_error("output arrays missing or negative capacity", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  unsigned char* base = static_cast<unsigned char*>(scratch_of(ctx->device));
+  if (!base) {
+    set_error("no scratch array registered for this device", 0);
+    return FEM2D_ERR_UNSUPPORTED;
+  }
+  // carve the work arrays from the scratch array (256-byte aligned); the
+  // total is known only at the end, so overflow is noted and reported after
+  size_t used = 0;
+  auto carve = [&](size_t bytes) -> void* {
+    const size_t at = (used + 255) & ~size_t(255);
+    used = at + std::max<size_t>(bytes, 8);
+    return used <= kScratchBytes ? base + at : nullptr;
+  };
+  Assembly a;
+  int rc = bind_work(dd, nu, nv, basis,
+                     [&](int, size_t count, auto** ptr) {
+                       using T = std::remove_pointer_t<std::remove_pointer_t<decltype(ptr)>>;
+                       *ptr = static_cast<T*>(carve(count * sizeof(T)));
+                       return FEM2D_OK;
+                     },
+                     a);
+  if (rc) return rc;
+  a.lib_lds = dd->tuned_lds[precision];
+  Work& w = a.w;
+  w.row_ptr = row_ptr;
+  w.chunk = dd->tuned_chunk[precision] > 0 ? dd->tuned_chunk[precision] : chunk_of(precision);
+  w.chunk_cap = capacity / w.chunk + 1;
+  w.chunk_first = static_cast<int*>(carve(size_t(w.chunk_cap) * sizeof(int)));
+  if (used > kScratchBytes) {
+    set_error("the assembly of this Domain needs " + std::to_string(used) + " bytes of work memory; the " +
+                  "no-allocation entry point has " + std::to_string(kScratchBytes) + " (FEM2D_NOALLOC_SCRATCH_BYTES)",
+              0);
+    return FEM2D_ERR_UNSUPPORTED;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  ensure_glq(ctx, nu, nv);
+  const FillOut o = make_fill_out(precision, ai, bi, capacity, col, a_val, b_val, res16);
+  // kernel by kernel on the context's stream: k_init clears every counter
+  // and flag the kernels accumulate into, so the array's earlier contents do
+  // not matter
+  if ((rc = launch_prep(ctx, st, a, ctx->d_status, false))) return rc;
+  if (capacity > 0 && (rc = launch_stream_fill(ctx, st, a, o))) return rc;
+  if (capacity > 0 && a.run_large && (rc = launch_large_fill(ctx, st, a, o))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  const volatile long long* stat = ctx->h_status;
+  if (stat[1]) return err_to_status(stat[1]);
+  const long long nnz = stat[0];
+  const int n_large = int(stat[2]);
+  if (!a.run_large && n_large > 0) {
+    // cannot happen for an unchanged Domain; assemble again with the block path
+    dd->large_state = 1;
+    return fem2d_assemble_device_noalloc(ctx, dd, basis, ai, bi, glq_u, glq_v, precision, capacity, row_ptr, col,
+                                         a_val, b_val, res16, nnz_out);
+  }
+  dd->large_state = n_large > 0 ? 1 : 0;
+  *nnz_out = nnz;
+  if (nnz > capacity) {
+    set_error("output capacity " + std::to_string(capacity) + " is below the " + std::to_string(nnz) +
+                  " stored entries of this GEP (returned in nnz); nothing was written beyond row_ptr",
+              0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  return FEM2D_OK;
+}
 
 int fem2d_device_malloc(fem2d_ctx* ctx, int64_t bytes, void** out) {
   if (!ctx || !out || bytes < 0) {

@@ -205,6 +205,29 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
                           int glq_u, int glq_v, int precision, fem2d_gep_device* out);
 int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d_gep_csr* out);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the assembly of fem2d_assemble_device, same
+ * kernels and results, with no hipMalloc, hipFree or graph instantiation in the
+ * call. The caller passes the outputs: row_ptr[n_dofs + 1] (int64), col[capacity]
+ * (int32), a_val / b_val[capacity] (double for FEM2D_PRECISION_FP64, float for
+ * FEM2D_PRECISION_FP32 and _FP32R) and, for FP32R only, res16[capacity] (NULL
+ * otherwise). The work memory is a static device array of
+ * FEM2D_NOALLOC_SCRATCH_BYTES per device, held until the process ends; a Domain
+ * whose assembly needs more (about 7.3 KB per row group plus ~100 B per DoF)
+ * is rejected with FEM2D_ERR_UNSUPPORTED before any device work. *nnz receives
+ * the GEP's stored entries; when they exceed capacity the call returns
+ * FEM2D_ERR_INVALID_ARG after writing row_ptr and nothing else, so the caller can
+ * size its buffers and call again. Every call on one device shares that device's
+ * array: calls that overlap on one device (from several contexts or threads)
+ * race and must be serialised by the caller. The kernels run on the context's
+ * stream; the call returns after they finish. */
+#define FEM2D_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int fem2d_assemble_device_noalloc(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int a_integral,
+                                  int b_integral, int glq_u, int glq_v, int precision, int64_t capacity,
+                                  int64_t* row_ptr, int32_t* col, void* a_val, void* b_val, uint32_t* res16,
+                                  int64_t* nnz);
+
 /* PETSc AIJ binary files of a GEP, replacing GEP::print_to_petsc_binary_files
  * (linalg.rs:44-52) with AIJMatrixBinary (sparse_matrix.rs:184-265): writes
  * "{dir}/tmp/{prefix}_a.dat" and "{dir}/tmp/{prefix}_b.dat", each the full
