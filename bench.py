@@ -254,10 +254,16 @@ def sparse_eigensolve(w):
     t0 = time.perf_counter()
     pair = F.solve_gep_sparse(dev, 10.0, ctx=ctx)
     ms = (time.perf_counter() - t0) * 1e3
-    out = {"solver": "block-tridiagonal level-set ordering, dense Schur-block inverses (rocSOLVER getrf/getri, "
-                     "rocBLAS gemm/gemv), shift-invert iteration in residual-correction form; FP64",
+    st = ctx.last_eigen_stats()
+    out = {"solver": "block-tridiagonal level-set ordering; block Cholesky of A + tau B (rocSOLVER potrf/trtri, "
+                     "rocBLAS trsm/syrk/trmv) preconditioning GMRES inside a residual-correction shift-invert "
+                     "iteration; FP64",
            w.name: {"n": int(dev.n), "ms": ms, "eigenvalue": pair.value,
-                    "rel_err_vs_te10": abs(pair.value - np.pi ** 2) / np.pi ** 2}}
+                    "rel_err_vs_te10": abs(pair.value - np.pi ** 2) / np.pi ** 2,
+                    "residual": st["residual"], "blocks": st["blocks"], "max_block": st["max_block"],
+                    "factor_gb": st["factor_bytes"] / 1e9, "symbolic_ms": st["symbolic_ms"],
+                    "factor_ms": st["factor_ms"], "iterate_ms": st["iterate_ms"],
+                    "outer_iterations": st["iterations"], "gmres_steps": st["inner_iterations"]}}
     dom, glq = cases.small_hp_irregular()
     dds = F.DeviceDomain(dom, ctx)
     devs = dds.assemble(glq, F.KOLShapeFn)
