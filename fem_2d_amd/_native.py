@@ -159,6 +159,7 @@ SIGNATURES = {
     "fem2d_ctx_stream": (vp, [vp]),
     "fem2d_ctx_synchronize": (C.c_int, [vp]),
     "fem2d_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
+    "fem2d_ctx_set_fill_chunk": (C.c_int, [vp, C.c_int]),
     "fem2d_ctx_last_eigen_stats": (C.c_int, [vp, P(EigenStats)]),
     "fem2d_galerkin_sample_gep_hcurl": (
         C.c_int, [vp, P(DomainView), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepCsr)]),

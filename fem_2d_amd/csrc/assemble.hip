@@ -481,11 +481,7 @@ __device__ __forceinline__ void pair_values(const double* __restrict__ lib, unsi
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void init_body(const Work& w, int n_elems, long long n_dofs, long long t) {
-  if (t < n_elems) {
-    w.depth[t] = 0;
-    w.n_desc[t] = 0;
-    w.desc_cursor[t] = 0;
-  }
+  if (t < n_elems) w.desc_cursor[t] = 0;
   if (t == 0) {
     *w.err = 0;
     *w.n_large = 0;
@@ -923,8 +919,15 @@ struct GroupScan {
 
 // 1D Gram library: lib[axis][range][kind][a][b] = sum_m w_m f_a(x'_m) g_b(x_m)
 // with x'_m = x_m * s + c mapped into the dyadic sub-range (glq.rs:238-249).
+// The block's values share few (range, order, point) shape evaluations: with
+// tab (dynamic LDS of gram_table_bytes) the block first evaluates each of them
+// once (N, T and T' of order n at point m, on its sub-ranges and at the
+// unmapped points), then every value is a dot product of two table rows; the
+// values are bit for bit those of evaluating the shapes per term (same
+// functions, same points, same products in the same order).
 template <int kBasis>
-__device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, const GlqArgs& glq, long long block) {
+__device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, const GlqArgs& glq, long long block,
+                                          double* tab) {
   __shared__ double s_pts[2][kMaxGlq], s_wts[2][kMaxGlq];
   for (int i = threadIdx.x; i < kMaxGlq; i += blockDim.x) {
     s_pts[0][i] = glq.u_pts[i];
@@ -937,9 +940,51 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
   // y axis); the library holds fewer than 2^31 values
   const unsigned stride = unsigned(kKinds * L.p1 * L.p1);
   const unsigned x_part = unsigned(L.nr) * stride, total = unsigned(L.nr + L.nr_v) * stride;
-  const unsigned t = unsigned(block) * blockDim.x + threadIdx.x;
-  if (block >= ((long long)total + blockDim.x - 1) / blockDim.x || t >= total) return;
+  if (block >= ((long long)total + blockDim.x - 1) / blockDim.x) return;  // block-uniform
   const unsigned p1 = unsigned(L.p1);
+  const unsigned t0 = unsigned(block) * blockDim.x;
+  const unsigned R0 = t0 / stride;  // the block's first range (over both axes)
+  const int npm = max(L.nu, L.nv);
+  // shape index of the table: N, T, T' (N' is not used by the library)
+  auto slot = [](int f) { return f == 0 ? 0 : f == 2 ? 1 : 2; };
+  if (tab) {
+    const unsigned t1 = min(t0 + blockDim.x, total);
+    const int nR = int((t1 - 1) / stride - R0) + 1;
+    // rows: nR sub-ranges, then the x and y axes' unmapped points
+    const int items = (nR + 2) * int(p1) * npm;
+    for (int i = threadIdx.x; i < items; i += blockDim.x) {
+      const int m = i % npm;
+      const int r = i / npm;
+      const int n = r % int(p1);
+      const int row = r / int(p1);
+      int axis;
+      double x;
+      if (row < nR) {
+        const unsigned R = R0 + unsigned(row);
+        axis = R >= unsigned(L.nr) ? 1 : 0;
+        if (m >= (axis == 0 ? L.nu : L.nv)) continue;
+        const long long key = w.ranges[R];
+        double sc, c;
+        range_of(key, sc, c);
+        const double xq = s_pts[axis][m];
+        x = key == 0 ? xq : xq * sc + c;
+      } else {
+        axis = row - nR;
+        if (m >= (axis == 0 ? L.nu : L.nv)) continue;
+        x = s_pts[axis][m];
+      }
+      double N, Nd, T, Td;
+      if (kBasis == FEM2D_BASIS_KOL) kol(n, x, N, Nd, T, Td);
+      else hier_poly(n, x, N, Nd, T, Td);
+      double* out = tab + (size_t(row) * 3 * p1 + n) * npm + m;
+      out[0] = N;
+      out[size_t(p1) * npm] = T;
+      out[2 * size_t(p1) * npm] = Td;
+    }
+    __syncthreads();
+  }
+  const unsigned t = t0 + threadIdx.x;
+  if (t >= total) return;
   const int axis = t >= x_part ? 1 : 0;
   unsigned r = t - (axis ? x_part : 0u);
   const int b = int(r % p1);
@@ -964,13 +1009,31 @@ __device__ __forceinline__ void gram_body(const Work& w, const LibShape& L, cons
     default: fp = 2; fq = 2; break;
   }
   double acc = 0.0;
+  if (tab) {
+    const unsigned R = t / stride;
+    const int nR = int((min(t0 + blockDim.x, total) - 1) / stride - R0) + 1;
+    const double* P = tab + ((size_t(R - R0) * 3 + slot(fp)) * p1 + a) * npm;
+    const double* Q = tab + ((size_t(nR + axis) * 3 + slot(fq)) * p1 + b) * npm;
 #pragma unroll 1
-  for (int m = 0; m < np; ++m) {
-    const double xq = pts[m];
-    const double xp = key == 0 ? xq : xq * s + c;
-    acc += wts[m] * shape_pick<kBasis>(fp, a, xp) * shape_pick<kBasis>(fq, b, xq);
+    for (int m = 0; m < np; ++m) acc += wts[m] * P[m] * Q[m];
+  } else {
+#pragma unroll 1
+    for (int m = 0; m < np; ++m) {
+      const double xq = pts[m];
+      const double xp = key == 0 ? xq : xq * s + c;
+      acc += wts[m] * shape_pick<kBasis>(fp, a, xp) * shape_pick<kBasis>(fq, b, xq);
+    }
   }
   w.lib[t] = acc;
+}
+
+// LDS bytes of gram_body's shape table for 256-thread blocks (0: too large,
+// the values evaluate their shapes per term)
+size_t gram_table_bytes(const LibShape& L) {
+  const long long stride = (long long)kKinds * L.p1 * L.p1;
+  const long long ranges = (kBlock + stride - 1) / stride + 1;  // most sub-ranges one block meets
+  const size_t bytes = size_t(ranges + 2) * 3 * size_t(L.p1) * size_t(std::max(L.nu, L.nv)) * sizeof(double);
+  return bytes <= 48 * 1024 ? bytes : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -984,25 +1047,30 @@ __global__ void __launch_bounds__(kBlock) k_init(Work w, int n_elems, long long 
   init_body(w, n_elems, n_dofs, blockIdx.x * (long long)kBlock + threadIdx.x);
 }
 
-// The Gram library (needed from the describe kernel on; its blocks come first
-// so that they are dispatched first, being the longest); Elem geometry and
-// depth; per-spec owners and packed specs; the DoFs' spec lists checked
-__global__ void __launch_bounds__(kBlock) k_gram_elem_specs(DevView v, Work w, unsigned gram_blocks,
-                                                            unsigned prep_blocks, unsigned spec_blocks, LibShape L,
-                                                            GlqArgs glq) {
-  if (blockIdx.x < gram_blocks) {
-    // one instance per shape-function space (no evaluation of the other's recurrence)
-    if (L.basis == FEM2D_BASIS_KOL) gram_body<FEM2D_BASIS_KOL>(w, L, glq, blockIdx.x);
-    else gram_body<FEM2D_BASIS_HIERPOLY>(w, L, glq, blockIdx.x);
-  } else if (blockIdx.x < gram_blocks + prep_blocks) {
-    elem_prep_body(v, w, int((blockIdx.x - gram_blocks) * kBlock + threadIdx.x));
-  } else if (blockIdx.x < gram_blocks + prep_blocks + spec_blocks) {
-    const unsigned first = gram_blocks + prep_blocks;
-    spec_body(v, w, (long long)(blockIdx.x - first) * kWavesPerBlock + threadIdx.x / kWave,
+// The Gram library of one assembly (a function of the Domain's dyadic
+// sub-ranges, the shape-function space and the Gauss points)
+__global__ void __launch_bounds__(kBlock) k_gram(Work w, LibShape L, GlqArgs glq, int use_table) {
+  extern __shared__ double gram_tab[];
+  double* tab = use_table ? gram_tab : nullptr;
+  // one instance per shape-function space (no evaluation of the other's recurrence)
+  if (L.basis == FEM2D_BASIS_KOL) gram_body<FEM2D_BASIS_KOL>(w, L, glq, blockIdx.x, tab);
+  else gram_body<FEM2D_BASIS_HIERPOLY>(w, L, glq, blockIdx.x, tab);
+}
+
+// Domain data derived once, when the Domain is uploaded (the device Domain is
+// immutable afterwards): per Elem its depth, descendant count, position inside
+// its base Elem and geometry; per basis spec its owning Elem, packed spec and
+// position in the Elem's list (the reference's BasisSpec carries these); and
+// the checks that the view's DoF and spec relations agree.
+__global__ void __launch_bounds__(kBlock) k_domain_prep(DevView v, Work w, unsigned prep_blocks,
+                                                        unsigned spec_blocks) {
+  if (blockIdx.x < prep_blocks) {
+    elem_prep_body(v, w, int(blockIdx.x * kBlock + threadIdx.x));
+  } else if (blockIdx.x < prep_blocks + spec_blocks) {
+    spec_body(v, w, (long long)(blockIdx.x - prep_blocks) * kWavesPerBlock + threadIdx.x / kWave,
               (long long)spec_blocks * kWavesPerBlock);
   } else {
-    const unsigned first = gram_blocks + prep_blocks + spec_blocks;
-    dof_check_body(v, w, (long long)(blockIdx.x - first) * kBlock + threadIdx.x);
+    dof_check_body(v, w, (long long)(blockIdx.x - prep_blocks - spec_blocks) * kBlock + threadIdx.x);
   }
 }
 
@@ -1469,9 +1537,10 @@ __device__ __forceinline__ void store_entry(int* __restrict__ col_out, typename 
 
 // Streaming fill: the CSR is cut into chunks of w.chunk consecutive entries
 // (the size the layout scan used: Store<P>::kChunk, or the one tuned for the
-// Domain);
-// wavefront v takes chunks v, v + waves, ..., so that the wavefronts write
-// neighbouring chunks at any time. For each row group a chunk meets, the
+// Domain); wavefront v takes chunks v, v + waves, ..., so that the wavefronts
+// work on neighbouring chunks at any time and share each row group's records
+// in L2 (one contiguous range per wavefront, with no such sharing, ran 2.5-4x
+// slower). For each row group a chunk meets, the
 // wavefront stages the group's source records in LDS and writes the entries
 // with consecutive lanes on consecutive entries (coalesced, non-temporal
 // stores), reading the group's column and row records and the Gram library (a
@@ -2125,6 +2194,14 @@ struct fem2d_device_domain {
   int* dof_bs = nullptr;
   int* leaves = nullptr;  // Elems without children, ascending (the field grids)
   long long n_leaves = 0;
+  // derived on the device at upload (k_domain_prep)
+  ElemGeo* geo = nullptr;
+  int* depth = nullptr;
+  int* n_desc = nullptr;
+  int4* dyad = nullptr;
+  int* bs_elem = nullptr;
+  unsigned* bs_pack = nullptr;
+  unsigned* spec_pos = nullptr;
 };
 
 struct fem2d_ctx {
@@ -2149,6 +2226,7 @@ struct fem2d_ctx {
   int fill_grid_lds[kPrecisions] = {};
   bool use_graphs = true;
   bool autotune = true;  // time the fill's chunk sizes on a Domain's first assembly
+  int fill_chunk = 0;     // fill chunk size forced for every assembly (0: tuned or default)
   hipGraphExec_t graph_exec[1] = {nullptr};  // the last recorded assembly
   GraphKey graph_key;
   void* solver = nullptr;  // eigensolver handle, made by the first solve
@@ -2350,7 +2428,6 @@ struct Assembly {
   long long n_grp_items = 0;  // row groups the records are sized for, plus one
   long long tiles_nested = 0, tiles_dof = 0, tiles_mem = 0, tiles_grp = 0;
   long long *ts_nested = nullptr, *ts_dof = nullptr, *ts_mem = nullptr, *ts_grp = nullptr;
-  unsigned slot_blocks = 0;
   bool run_large = true;  // launch the block path (unless the Domain is known to need none)
   int lib_lds = -1;       // the fill's library copy: -1 in LDS when it fits, 0 global memory, 1 LDS
 };
@@ -2382,6 +2459,67 @@ int check_assembly(const fem2d_device_domain* dd, int glq_u, int glq_v, int* nu,
 // chunk_first (sized by the output capacity) through bind(k, count, &ptr),
 // k numbering the arrays; fills in the view, the library shape and the
 // prefix-sum tile layout.
+// The device Domain's input arrays as the kernels take them
+DevView dev_view(const fem2d_device_domain* dd) {
+  DevView v{};
+  v.n_elems = dd->n_elems;
+  v.n_dofs = dd->n_dofs;
+  v.n_bs = dd->n_bs;
+  v.parent = dd->parent;
+  v.loc = dd->loc;
+  v.box = dd->box;
+  v.mat = dd->mat;
+  v.bs_off = dd->bs_off;
+  v.bs_dir = dd->bs_dir;
+  v.bs_i = dd->bs_i;
+  v.bs_j = dd->bs_j;
+  v.bs_dof = dd->bs_dof;
+  v.dof_bs = dd->dof_bs;
+  v.p1 = std::max(dd->max_i, dd->max_j) + 1;
+  return v;
+}
+
+// k_domain_prep into the device Domain's derived arrays (allocated here); the
+// view's relation checks surface as the call's status
+int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
+  const int ne = dd->n_elems;
+  const size_t E1 = size_t(std::max(ne, 1)), B1 = size_t(std::max(dd->n_bs, 1LL));
+  if (int rc = alloc_device(E1, &dd->geo)) return rc;
+  if (int rc = alloc_device(E1, &dd->depth)) return rc;
+  if (int rc = alloc_device(E1, &dd->n_desc)) return rc;
+  if (int rc = alloc_device(E1, &dd->dyad)) return rc;
+  if (int rc = alloc_device(B1, &dd->bs_elem)) return rc;
+  if (int rc = alloc_device(B1, &dd->bs_pack)) return rc;
+  if (int rc = alloc_device(B1, &dd->spec_pos)) return rc;
+  hipStream_t st = ctx->stream;
+  Work w{};
+  w.geo = dd->geo;
+  w.depth = dd->depth;
+  w.n_desc = dd->n_desc;
+  w.dyad = dd->dyad;
+  w.bs_elem = dd->bs_elem;
+  w.bs_pack = dd->bs_pack;
+  w.spec_pos = dd->spec_pos;
+  if (int rc = alloc_device(2, &w.err)) return rc;
+  struct FreeErr {
+    int* p;
+    ~FreeErr() { (void)hipFree(p); }
+  } free_err{w.err};
+  HIP_TRY(hipMemsetAsync(dd->depth, 0, E1 * sizeof(int), st));
+  HIP_TRY(hipMemsetAsync(dd->n_desc, 0, E1 * sizeof(int), st));
+  HIP_TRY(hipMemsetAsync(w.err, 0, sizeof(int), st));
+  const unsigned prep_blocks = grid_for(ne, kBlock);
+  const unsigned spec_blocks =
+      unsigned(std::min<long long>(std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock), 1 << 16));
+  k_domain_prep<<<prep_blocks + spec_blocks + grid_for(dd->n_dofs, kBlock), kBlock, 0, st>>>(
+      dev_view(dd), w, prep_blocks, spec_blocks);
+  HIP_TRY_AT(hipGetLastError(), "after the Domain preparation kernel");
+  int err = 0;
+  HIP_TRY(hipMemcpyAsync(&err, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return err ? err_to_status(err) : FEM2D_OK;
+}
+
 template <class Bind>
 int bind_work(const fem2d_device_domain* dd, int nu, int nv, int basis, Bind&& bind, Assembly& a) {
   const int ne = dd->n_elems;
@@ -2393,11 +2531,17 @@ int bind_work(const fem2d_device_domain* dd, int nu, int nv, int basis, Bind&& b
   int rc = FEM2D_OK;
   int k = 0;
   const size_t E1 = size_t(std::max(ne, 1)), N1 = size_t(nd + 1), B1 = size_t(nbs + 1);
-  if ((rc = bind(k++, E1, &w.geo)) || (rc = bind(k++, E1, &w.depth)) || (rc = bind(k++, E1, &w.n_desc)) ||
-      (rc = bind(k++, E1, &w.desc_cursor)) || (rc = bind(k++, size_t(ne + 1), &w.nested_ptr)) ||
-      (rc = bind(k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) || (rc = bind(k++, E1, &w.dyad)) ||
-      (rc = bind(k++, B1, &w.spec_run)) || (rc = bind(k++, B1, &w.spec_pos)) || (rc = bind(k++, B1, &w.bs_elem)) ||
-      (rc = bind(k++, B1, &w.bs_pack)) || (rc = bind(k++, N1, &w.dof_e1)) || (rc = bind(k++, N1, &w.dof_e2)) ||
+  // the Domain's derived arrays (read only by the assembly)
+  w.geo = dd->geo;
+  w.depth = dd->depth;
+  w.n_desc = dd->n_desc;
+  w.dyad = dd->dyad;
+  w.bs_elem = dd->bs_elem;
+  w.bs_pack = dd->bs_pack;
+  w.spec_pos = dd->spec_pos;
+  if ((rc = bind(k++, E1, &w.desc_cursor)) || (rc = bind(k++, size_t(ne + 1), &w.nested_ptr)) ||
+      (rc = bind(k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) ||
+      (rc = bind(k++, B1, &w.spec_run)) || (rc = bind(k++, N1, &w.dof_e1)) || (rc = bind(k++, N1, &w.dof_e2)) ||
       (rc = bind(k++, N1, &w.dof_t1)) || (rc = bind(k++, N1, &w.dof_t2)) || (rc = bind(k++, N1, &w.dof_flag)) ||
       (rc = bind(k++, N1, &w.dof_grp)) || (rc = bind(k++, N1, &w.g_start)) || (rc = bind(k++, N1, &w.g_e1)) ||
       (rc = bind(k++, N1, &w.g_e2)) || (rc = bind(k++, B1, &w.mem_idx)) || (rc = bind(k++, B1, &w.mem_group)) ||
@@ -2441,24 +2585,8 @@ int bind_work(const fem2d_device_domain* dd, int nu, int nv, int basis, Bind&& b
   a.ts_dof = a.ts_nested + a.tiles_nested;
   a.ts_mem = a.ts_dof + a.tiles_dof;
   a.ts_grp = a.ts_mem + a.tiles_mem;
-  a.slot_blocks =
-      unsigned(std::min<long long>(std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock), 1 << 16));
 
-  DevView& v = a.v;
-  v.n_elems = ne;
-  v.n_dofs = nd;
-  v.n_bs = nbs;
-  v.parent = dd->parent;
-  v.loc = dd->loc;
-  v.box = dd->box;
-  v.mat = dd->mat;
-  v.bs_off = dd->bs_off;
-  v.bs_dir = dd->bs_dir;
-  v.bs_i = dd->bs_i;
-  v.bs_j = dd->bs_j;
-  v.bs_dof = dd->bs_dof;
-  v.dof_bs = dd->dof_bs;
-  v.p1 = L.p1;
+  a.v = dev_view(dd);
   a.run_large = dd->large_state != 0;
   return FEM2D_OK;
 }
@@ -2487,10 +2615,9 @@ int launch_prep(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, long long* st
   const int B = 256;
   auto scan_grid = [&](long long tiles) { return unsigned(std::max(1LL, std::min(tiles, 8LL * ctx->n_cu))); };
   k_init<<<grid_for(ne, B), B, 0, st>>>(w, ne, nd);
-  const unsigned gram_blocks = grid_for(a.lib_n, B), prep_blocks = grid_for(ne, B);
-  k_gram_elem_specs<<<gram_blocks + prep_blocks + a.slot_blocks + grid_for(nd, B), B, 0, st>>>(
-      v, w, gram_blocks, prep_blocks, a.slot_blocks, a.L, ctx->glq);
-  HIP_TRY_AT(hipGetLastError(), "after the Elem preparation kernels");
+  const size_t gram_lds = gram_table_bytes(a.L);
+  k_gram<<<grid_for(a.lib_n, B), B, gram_lds, st>>>(w, a.L, ctx->glq, gram_lds > 0 ? 1 : 0);
+  HIP_TRY_AT(hipGetLastError(), "after the Gram library kernel");
   // nested-list offsets and row groups side by side
   const unsigned gn = scan_grid(a.tiles_nested), gd = scan_grid(a.tiles_dof), gm = scan_grid(a.tiles_mem);
   k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, a.ts_nested, gn, nd, DofGroupScan{v, w},
@@ -2569,6 +2696,7 @@ FillOut make_fill_out(int precision, int ai, int bi, long long capacity, int* co
 // chunk sizes the fill's autotuning times on a Domain's first assembly
 constexpr int kChunkCandidates[] = {128, 256, 512, 1024};
 
+
 // CSR entries per fill chunk for a stored precision (before any tuning)
 int chunk_of(int precision) {
   switch (precision) {
@@ -2577,6 +2705,9 @@ int chunk_of(int precision) {
     default: return Store<FEM2D_PRECISION_FP64>::kChunk;
   }
 }
+
+// w.chunk for a chunk size (0: the stored precision's default)
+void set_chunk(Work& w, int chunk, int precision) { w.chunk = chunk > 0 ? chunk : chunk_of(precision); }
 
 }  // namespace
 
@@ -2668,6 +2799,19 @@ int fem2d_ctx_set_autotune(fem2d_ctx* ctx, int enable) {
     return FEM2D_ERR_INVALID_ARG;
   }
   ctx->autotune = enable != 0;
+  return FEM2D_OK;
+}
+
+int fem2d_ctx_set_fill_chunk(fem2d_ctx* ctx, int chunk) {
+  if (!ctx) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  if (chunk != 0 && (chunk < kWave || chunk > 4096 || (chunk & (chunk - 1)) != 0)) {
+    set_error("fill chunk: 0, or a power of two from 64 to 4096", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  ctx->fill_chunk = chunk;
   return FEM2D_OK;
 }
 
@@ -2809,7 +2953,8 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
       (rc = upload(v->bs_j, size_t(nbs), &dd->bs_j)) || (rc = upload(v->bs_dof, size_t(nbs), &dd->bs_dof)) ||
       (rc = upload(v->dof_bs, size_t(2 * v->n_dofs), &dd->dof_bs)) ||
       (rc = upload(keys_u.data(), keys_u.size(), &dd->ranges)) ||
-      (!range_map.empty() && (rc = upload(range_map.data(), range_map.size(), &dd->range_map)))) {
+      (!range_map.empty() && (rc = upload(range_map.data(), range_map.size(), &dd->range_map))) ||
+      (rc = domain_prep(ctx, dd))) {
     fem2d_device_domain_free(dd);
     return rc;
   }
@@ -2835,7 +2980,8 @@ void fem2d_device_domain_free(fem2d_device_domain* dd) {
   (void)hipSetDevice(dd->device);
   for (void* p : {(void*)dd->parent, (void*)dd->loc, (void*)dd->box, (void*)dd->mat, (void*)dd->bs_off,
                   (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof, (void*)dd->dof_bs,
-                  (void*)dd->leaves, (void*)dd->ranges, (void*)dd->range_map})
+                  (void*)dd->leaves, (void*)dd->ranges, (void*)dd->range_map, (void*)dd->geo, (void*)dd->depth,
+                  (void*)dd->n_desc, (void*)dd->dyad, (void*)dd->bs_elem, (void*)dd->bs_pack, (void*)dd->spec_pos})
     if (p) (void)hipFree(p);
   delete dd;
 }
@@ -2906,7 +3052,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
       return rc;
     const FillOut o = make_fill_out(precision, ai, bi, ctx->out_capacity, static_cast<int*>(ctx->out_col.p),
                                     ctx->out_a.p, ctx->out_b.p, static_cast<unsigned*>(ctx->out_r.p));
-    w.chunk = dd->tuned_chunk[precision] > 0 ? dd->tuned_chunk[precision] : chunk_of(precision);
+    set_chunk(w, ctx->fill_chunk ? ctx->fill_chunk : dd->tuned_chunk[precision], precision);
     w.chunk_cap = o.capacity / w.chunk + 1;
     if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
     const bool fill = o.capacity > 0;
@@ -2984,7 +3130,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   dd->large_state = n_large > 0 ? 1 : 0;
   ctx->timing.fill_chunk = w.chunk;
   ctx->timing.fill_lib_lds = lib_in_lds(precision, a.lib_n) && a.lib_lds != 0 ? 1 : 0;
-  if (ctx->autotune && dd->tuned_chunk[precision] == 0) {
+  if (ctx->autotune && !ctx->fill_chunk && dd->tuned_chunk[precision] == 0) {
     // the first assembly of this Domain in this precision: time the streaming
     // fill at each candidate chunk size and library placement (best of two
     // launches each, after the layout scan for that size), keep the fastest,
@@ -2997,8 +3143,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     for (int place = fits ? 1 : 0; place >= 0; --place) {
       a.lib_lds = place;
       for (int c : kChunkCandidates) {
-        w.chunk = c;
-        w.chunk_cap = o.capacity / c + 1;
+        set_chunk(w, c, precision);
+        w.chunk_cap = o.capacity / w.chunk + 1;
         if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
         for (int rep = 0; rep < 2; ++rep) {
           if ((rc = launch_prep(ctx, st, a, ctx->d_status, false))) return rc;
@@ -3079,7 +3225,7 @@ int fem2d_assemble_device_noalloc(fem2d_ctx* ctx, const fem2d_device_domain* dd,
   a.lib_lds = dd->tuned_lds[precision];
   Work& w = a.w;
   w.row_ptr = row_ptr;
-  w.chunk = dd->tuned_chunk[precision] > 0 ? dd->tuned_chunk[precision] : chunk_of(precision);
+  set_chunk(w, ctx->fill_chunk ? ctx->fill_chunk : dd->tuned_chunk[precision], precision);
   w.chunk_cap = capacity / w.chunk + 1;
   w.chunk_first = static_cast<int*>(carve(size_t(w.chunk_cap) * sizeof(int)));
   if (used > kScratchBytes) {

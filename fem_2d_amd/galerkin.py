@@ -98,6 +98,11 @@ class Context:
         (default), or keep the default chunk size."""
         check(self._lib.fem2d_ctx_set_autotune(self._h, 1 if enable else 0))
 
+    def set_fill_chunk(self, chunk: int) -> None:
+        """Forces the fill's chunk size (a power of two from 64 to 4096 CSR entries) for later
+        assemblies, with no tuning; 0 returns to the tuned or default size."""
+        check(self._lib.fem2d_ctx_set_fill_chunk(self._h, int(chunk)))
+
     def last_eigen_stats(self) -> dict:
         """Statistics of the last sparse eigensolve on this context."""
         s = N.EigenStats()

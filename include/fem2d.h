@@ -159,6 +159,11 @@ int fem2d_ctx_set_graphs(fem2d_ctx* ctx, int enable);
  * keeps the fastest for later assemblies of that Domain (default 1). The tuned
  * shape changes timing only, never results. */
 int fem2d_ctx_set_autotune(fem2d_ctx* ctx, int enable);
+/* Forces the streaming fill's chunk size (CSR entries per wavefront work item,
+ * a power of two from 64 to 4096) for every later assembly on the context, with
+ * no tuning while set; 0 returns to the tuned or default size. Results do not
+ * depend on it. */
+int fem2d_ctx_set_fill_chunk(fem2d_ctx* ctx, int chunk);
 
 /* Replaces galerkin_sample_gep_hcurl (galerkin.rs:33-187).
  * basis: fem2d_basis; a_integral / b_integral: fem2d_integral (the AI / BI
@@ -366,7 +371,7 @@ typedef struct fem2d_timing {
   double fill_kernel_ms;  /* the wave-per-group entry-writing kernel alone */
   int64_t nnz;
   int64_t n_large;    /* row groups assembled one block per group (large column sets) */
-  int64_t fill_chunk; /* CSR entries per work item of the entry fill (default or tuned) */
+  int64_t fill_chunk; /* CSR entries per work item of the entry fill (default, tuned or forced) */
   int64_t fill_lib_lds; /* 1 when the entry fill read the Gram library from an LDS copy */
 } fem2d_timing;
 int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out);

@@ -92,6 +92,39 @@ def test_block_path_after_smaller_domain(graphs):
     c.close()
 
 
+FILL_CHUNKS = [64, 128, 1024, 4096]
+
+
+@pytest.mark.parametrize("name,basis", [("small_hp_irregular", "kol"), ("waveguide_small", "hierpoly"),
+                                        ("deep_corner", "kol")])
+def test_fill_chunk_sizes_give_identical_gep(name, basis):
+    """Every chunk size of the streaming fill (64 to 4096 entries per work item) writes the same
+    GEP, bit for bit, as the default size; one of them is checked against the oracle too."""
+    print(f"[fill-chunk] case={name}", flush=True)
+    c = F.Context(0)
+    c.set_autotune(False)
+    dom, glq = cases.ALL[name]()
+    B = BASIS[basis]
+    dd = F.DeviceDomain(dom, c)
+    ref = dd.download(dd.assemble(glq, B))
+    for chunk in FILL_CHUNKS:
+        c.set_fill_chunk(chunk)
+        got = dd.download(dd.assemble(glq, B))
+        assert c.last_timing()["fill_chunk"] == chunk
+        np.testing.assert_array_equal(got.row_ptr, ref.row_ptr)
+        np.testing.assert_array_equal(got.col, ref.col)
+        assert np.array_equal(got.a_val.view(np.uint64), ref.a_val.view(np.uint64)), chunk
+        assert np.array_equal(got.b_val.view(np.uint64), ref.b_val.view(np.uint64)), chunk
+    c.set_fill_chunk(128)
+    gep = dd.download(dd.assemble(glq, B))
+    rc, oref = oracle.assemble(dom.view(), basis=B.code, glq=glq, n_threads=8)
+    assert rc == 0
+    assert_gep_matches(gep, oref)
+    with pytest.raises(ValueError):  # FEM2D_ERR_INVALID_ARG
+        c.set_fill_chunk(96)
+    c.close()
+
+
 def test_stale_device_handle_rejected(ctx):
     """A device GEP handle is valid until the next assembly on its context: afterwards the
     download, the PETSc writer and the eigensolves reject it (the buffers may have been
