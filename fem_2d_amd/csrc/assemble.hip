@@ -120,6 +120,11 @@ constexpr int kFillWavesPerSimd = 6;
 constexpr int kLatencyWavesPerSimd = 8;
 constexpr int kMaxDepth = 24;      // h-refinement depth below a base Elem
 constexpr int kMaxRanges = 16383;  // distinct relative sub-ranges per axis (14-bit ids in the block path)
+// spec_pos bits: a spec's index in its Elem's list, and flags
+constexpr unsigned kPosIndex = (1u << 29) - 1;
+constexpr unsigned kPosNextAdj = 1u << 29;  // the list's next spec belongs to this spec's DoF + 1
+constexpr unsigned kPosPrevAdj = 1u << 30;  // the list's previous spec belongs to this spec's DoF - 1
+constexpr unsigned kPosLast = 1u << 31;     // the last spec of the list
 constexpr int kDenseRangeDepth = 12;  // deepest sub-range of the direct range map
 constexpr int kMaxOrder = 20;      // MAX_POLYNOMIAL_ORDER (mesh.rs:42)
 constexpr int kMaxGlq = 64;        // Gauss points per axis
@@ -175,17 +180,15 @@ struct Work {
   int4* dyad;  // (du, iu, dv, iv): position of the Elem inside its base Elem
   int* bs_elem;       // per basis spec: owning Elem
   unsigned* bs_pack;  // per basis spec, packed by pack_spec
-  int* dof_e1;
-  int* dof_e2;
-  int* dof_t1;  // the DoF's basis spec on its first / second Elem (flat index; dof_t2 -1 for none)
-  int* dof_t2;
-  int* dof_flag;
-  int* dof_grp;  // inclusive scan of dof_flag
+  int4* dof_sig;  // per DoF (e1, t1, e2, t2): its basis specs t1, t2 (flat index) on its first and second
+                  // Elem e1 < e2, in Elem order (-1 for none)
+  int* dof_grp;  // row group of each DoF, plus one (inclusive scan of the group starts)
   int* g_start;  // [G+1]
   int* g_e1;
   int* g_e2;
   int2* spec_run;  // [n_bs] per basis spec: row group if a membership starts there (else -1), index in the Elem's list
-  unsigned* spec_pos;  // [n_bs] index in the Elem's list | last in the list << 31
+  unsigned* spec_pos;  // [n_bs] index k in the Elem's list | last in the list << 31 | the list's spec k - 1
+                       // belongs to this spec's DoF - 1 << 30 | spec k + 1 to its DoF + 1 << 29
   int* mem_idx;   // [n_bs+1] memberships before each basis spec; Elem X's are [mem_idx[bs_off[X]], mem_idx[bs_off[X+1]])
   long long* tile_sum;  // per-tile sums of the prefix sum being computed
   int* mem_group;
@@ -581,8 +584,11 @@ __device__ __forceinline__ void spec_body(const DevView& v, const Work& w, long 
     for (long long t = t0 + lane; t < t1; t += kWave) {
       w.bs_elem[t] = int(e);
       w.bs_pack[t] = pack_spec(v.bs_dir[t], v.bs_i[t], v.bs_j[t], v.p1);
-      w.spec_pos[t] = unsigned(t - t0) | (t + 1 == t1 ? 0x80000000u : 0u);
       const int d = v.bs_dof[t];
+      const bool prev_adj = t > t0 && v.bs_dof[t - 1] == d - 1;
+      const bool next_adj = t + 1 < t1 && v.bs_dof[t + 1] == d + 1;
+      w.spec_pos[t] = unsigned(t - t0) | (t + 1 == t1 ? kPosLast : 0u) | (prev_adj ? kPosPrevAdj : 0u) |
+                      (next_adj ? kPosNextAdj : 0u);
       // H(curl) specs are U or V directed with orders inside the Gram library
       if (d < 0 || d >= v.n_dofs || v.bs_dir[t] > FEM2D_DIR_V || v.bs_i[t] >= v.p1 || v.bs_j[t] >= v.p1 ||
           (v.dof_bs[2LL * d] != t && v.dof_bs[2LL * d + 1] != t))
@@ -602,31 +608,11 @@ __device__ __forceinline__ void dof_check_body(const DevView& v, const Work& w, 
   if (!ok_a || !ok_b) set_err(w.err, kErrView);
 }
 
-// One of a DoF's basis specs (flat index t, -1 for none): its Elem, its place
-// in the Elem's list (spec_pos) and the DoFs of its neighbours in that list.
-struct SpecInfo {
-  int t, e;
-  unsigned pos;
-  int prev_d, next_d;
-};
-
-__device__ __forceinline__ SpecInfo spec_info(const DevView& v, const Work& w, int t) {
-  const bool ok = t >= 0 && t < v.n_bs;
-  const int tt = ok ? t : 0;  // every load in bounds
-  SpecInfo s;
-  s.t = ok ? t : -1;
-  s.e = ok ? w.bs_elem[tt] : -1;
-  s.pos = w.spec_pos[tt];
-  s.prev_d = tt > 0 ? v.bs_dof[tt - 1] : -1;
-  s.next_d = tt + 1 < v.n_bs ? v.bs_dof[tt + 1] : -1;
-  return s;
-}
-
 // A DoF's signature: the Elems of its specs in increasing order and the specs
 // (flat indices); -1 where absent. Out-of-range spec indices (an invalid view,
-// flagged by dof_check_body) read as absent.
-__device__ __forceinline__ void dof_sig(const DevView& v, const Work& w, long long d, int& e1, int& t1, int& e2,
-                                        int& t2) {
+// rejected by dof_check_body) read as absent.
+__device__ __forceinline__ void dof_sig_body(const DevView& v, const Work& w, long long d) {
+  if (d >= v.n_dofs) return;
   int ta = v.dof_bs[2 * d], tb = v.dof_bs[2 * d + 1];
   if (ta < 0 || ta >= v.n_bs) ta = -1;
   if (tb < 0 || tb >= v.n_bs) tb = -1;
@@ -638,10 +624,7 @@ __device__ __forceinline__ void dof_sig(const DevView& v, const Work& w, long lo
     tb = te;
     eb = ee;
   }
-  e1 = ea;
-  t1 = ta;
-  e2 = eb;
-  t2 = tb;
+  w.dof_sig[d] = make_int4(ea, ta, eb, tb);
 }
 
 // ---------------------------------------------------------------------------
@@ -777,60 +760,49 @@ struct DofGroupScan {
   DevView v;
   Work w;
   struct Item {
-    long long n;     // 1 where a row group starts
-    SpecInfo a, b;   // the DoF's specs, on its first and second Elem
-    bool last;       // the next DoF starts another group (or there is none)
+    long long n;  // 1 where a row group starts
+    int4 sig;     // the DoF's (e1, t1, e2, t2)
+    bool last;    // the next DoF starts another group (or there is none)
   };
   __device__ Item load(long long d) const {
     Item it;
-    it.a = spec_info(v, w, v.dof_bs[2 * d]);
-    it.b = spec_info(v, w, v.dof_bs[2 * d + 1]);
-    if (it.a.e < 0 || (it.b.e >= 0 && it.b.e < it.a.e)) {
-      const SpecInfo x = it.a;
-      it.a = it.b;
-      it.b = x;
-    }
-    int p1, pt1, p2, pt2;
+    it.sig = w.dof_sig[d];
     it.n = 1;
     if (d > 0) {
-      dof_sig(v, w, d - 1, p1, pt1, p2, pt2);
-      it.n = (p1 != it.a.e || p2 != it.b.e) ? 1 : 0;
+      const int4 p = w.dof_sig[d - 1];
+      it.n = (p.x != it.sig.x || p.z != it.sig.z) ? 1 : 0;
     }
     it.last = true;
     if (d + 1 < v.n_dofs) {
-      dof_sig(v, w, d + 1, p1, pt1, p2, pt2);
-      it.last = p1 != it.a.e || p2 != it.b.e;
+      const int4 q = w.dof_sig[d + 1];
+      it.last = q.x != it.sig.x || q.z != it.sig.z;
     }
     return it;
   }
-  // spec s of DoF d in group g; first: d is the group's first DoF. The run
+  // spec t of DoF d in group g; first: d is the group's first DoF. The run
   // continues from the previous spec of the Elem's list when that spec is DoF
   // d - 1's and d is not a group's first, and on to the next one likewise.
-  __device__ void mark_spec(long long d, const SpecInfo& s, int g, bool first, bool last) const {
-    const int k = int(s.pos & 0x7fffffffu);
-    const bool start = k == 0 || s.prev_d != d - 1 || first;
-    const bool end = (s.pos >> 31) != 0u || s.next_d != d + 1 || last;
+  __device__ void mark_spec(int t, int g, bool first, bool last) const {
+    const unsigned pos = w.spec_pos[t];
+    const int k = int(pos & kPosIndex);
+    const bool start = (pos & kPosPrevAdj) == 0u || first;
+    const bool end = (pos & kPosNextAdj) == 0u || last;
     if ((start && !first) || (end && !last)) set_err(w.err, kErrView);  // a run that is not a whole group
-    w.spec_run[s.t] = make_int2(start ? g : -1, k);
+    w.spec_run[t] = make_int2(start ? g : -1, k);
   }
   __device__ void store(long long d, long long excl, const Item& it) const {
-    const int e1 = it.a.e, e2 = it.b.e;
+    const int e1 = it.sig.x, e2 = it.sig.z;
     if (e1 < 0 || (e2 >= 0 && e2 == e1)) set_err(w.err, kErrView);  // no spec, or two on one Elem
-    w.dof_e1[d] = e1;
-    w.dof_t1[d] = it.a.t;
-    w.dof_e2[d] = e2;
-    w.dof_t2[d] = it.b.t;
     const int g1 = int(excl + it.n);  // group id + 1
     w.dof_grp[d] = g1;
-    w.dof_flag[d] = int(it.n);
     if (it.n) {
       w.g_start[g1 - 1] = int(d);
       w.g_e1[g1 - 1] = e1;
       w.g_e2[g1 - 1] = e2;
     }
     if (d == w.n_dofs - 1) w.g_start[g1] = int(w.n_dofs);
-    if (e1 >= 0) mark_spec(d, it.a, g1 - 1, it.n != 0, it.last);
-    if (e2 >= 0) mark_spec(d, it.b, g1 - 1, it.n != 0, it.last);
+    if (e1 >= 0) mark_spec(it.sig.y, g1 - 1, it.n != 0, it.last);
+    if (e2 >= 0) mark_spec(it.sig.w, g1 - 1, it.n != 0, it.last);
   }
 };
 
@@ -1055,6 +1027,11 @@ __global__ void __launch_bounds__(kBlock) k_gram(Work w, LibShape L, GlqArgs glq
   // one instance per shape-function space (no evaluation of the other's recurrence)
   if (L.basis == FEM2D_BASIS_KOL) gram_body<FEM2D_BASIS_KOL>(w, L, glq, blockIdx.x, tab);
   else gram_body<FEM2D_BASIS_HIERPOLY>(w, L, glq, blockIdx.x, tab);
+}
+
+// per DoF the Elems and specs of its two basis specs (after k_domain_prep)
+__global__ void __launch_bounds__(kBlock) k_dof_sig(DevView v, Work w) {
+  dof_sig_body(v, w, blockIdx.x * (long long)kBlock + threadIdx.x);
 }
 
 // Domain data derived once, when the Domain is uploaded (the device Domain is
@@ -1372,8 +1349,8 @@ __global__ void __launch_bounds__(kBlock, kLatencyWavesPerSimd) k_describe_wave(
       }
       for (int q = lane; q < nrows; q += kWave) {
         const int r = r0 + q;
-        const int t2 = w.dof_t2[r];
-        w.meta_row[(long long)g * kWRows + q] = make_uint2(w.bs_pack[w.dof_t1[r]], t2 >= 0 ? w.bs_pack[t2] : 0u);
+        const int4 sg = w.dof_sig[r];
+        w.meta_row[(long long)g * kWRows + q] = make_uint2(w.bs_pack[sg.y], sg.w >= 0 ? w.bs_pack[sg.w] : 0u);
       }
       for (int t = lane; t < T; t += kWave) {
         const int u = u_self + t;
@@ -1953,8 +1930,9 @@ __global__ void __launch_bounds__(kBlock) k_fill_large(DevView v, Work w, LibSha
       S.row_rel[q] = q < nrows ? int(row_ptr[r0 + q] - seg0) : int(seg_len);
       if (q < nrows) {
         const int r = r0 + q;
-        S.row_bs[0][q] = w.bs_pack[w.dof_t1[r]];
-        S.row_bs[1][q] = e_k[1] >= 0 ? w.bs_pack[w.dof_t2[r]] : 0u;
+        const int4 sg = w.dof_sig[r];
+        S.row_bs[0][q] = w.bs_pack[sg.y];
+        S.row_bs[1][q] = e_k[1] >= 0 ? w.bs_pack[sg.w] : 0u;
       }
     }
     __syncthreads();
@@ -2202,6 +2180,7 @@ struct fem2d_device_domain {
   int* bs_elem = nullptr;
   unsigned* bs_pack = nullptr;
   unsigned* spec_pos = nullptr;
+  int4* dof_sig = nullptr;
 };
 
 struct fem2d_ctx {
@@ -2491,6 +2470,7 @@ int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
   if (int rc = alloc_device(B1, &dd->bs_elem)) return rc;
   if (int rc = alloc_device(B1, &dd->bs_pack)) return rc;
   if (int rc = alloc_device(B1, &dd->spec_pos)) return rc;
+  if (int rc = alloc_device(size_t(std::max(dd->n_dofs, 1LL)), &dd->dof_sig)) return rc;
   hipStream_t st = ctx->stream;
   Work w{};
   w.geo = dd->geo;
@@ -2500,6 +2480,7 @@ int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
   w.bs_elem = dd->bs_elem;
   w.bs_pack = dd->bs_pack;
   w.spec_pos = dd->spec_pos;
+  w.dof_sig = dd->dof_sig;
   if (int rc = alloc_device(2, &w.err)) return rc;
   struct FreeErr {
     int* p;
@@ -2513,7 +2494,9 @@ int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
       unsigned(std::min<long long>(std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock), 1 << 16));
   k_domain_prep<<<prep_blocks + spec_blocks + grid_for(dd->n_dofs, kBlock), kBlock, 0, st>>>(
       dev_view(dd), w, prep_blocks, spec_blocks);
-  HIP_TRY_AT(hipGetLastError(), "after the Domain preparation kernel");
+  // the DoF signatures read the specs' owners written above
+  k_dof_sig<<<grid_for(dd->n_dofs, kBlock), kBlock, 0, st>>>(dev_view(dd), w);
+  HIP_TRY_AT(hipGetLastError(), "after the Domain preparation kernels");
   int err = 0;
   HIP_TRY(hipMemcpyAsync(&err, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -2539,10 +2522,10 @@ int bind_work(const fem2d_device_domain* dd, int nu, int nv, int basis, Bind&& b
   w.bs_elem = dd->bs_elem;
   w.bs_pack = dd->bs_pack;
   w.spec_pos = dd->spec_pos;
+  w.dof_sig = dd->dof_sig;
   if ((rc = bind(k++, E1, &w.desc_cursor)) || (rc = bind(k++, size_t(ne + 1), &w.nested_ptr)) ||
       (rc = bind(k++, size_t(std::max(1LL, dd->nested_cap)), &w.nested)) ||
-      (rc = bind(k++, B1, &w.spec_run)) || (rc = bind(k++, N1, &w.dof_e1)) || (rc = bind(k++, N1, &w.dof_e2)) ||
-      (rc = bind(k++, N1, &w.dof_t1)) || (rc = bind(k++, N1, &w.dof_t2)) || (rc = bind(k++, N1, &w.dof_flag)) ||
+      (rc = bind(k++, B1, &w.spec_run)) ||
       (rc = bind(k++, N1, &w.dof_grp)) || (rc = bind(k++, N1, &w.g_start)) || (rc = bind(k++, N1, &w.g_e1)) ||
       (rc = bind(k++, N1, &w.g_e2)) || (rc = bind(k++, B1, &w.mem_idx)) || (rc = bind(k++, B1, &w.mem_group)) ||
       (rc = bind(k++, B1, &w.mem_bsoff)) || (rc = bind(k++, N1, &w.row_len)) || (rc = bind(k++, size_t(8), &w.err)) ||
@@ -2981,7 +2964,8 @@ void fem2d_device_domain_free(fem2d_device_domain* dd) {
   for (void* p : {(void*)dd->parent, (void*)dd->loc, (void*)dd->box, (void*)dd->mat, (void*)dd->bs_off,
                   (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof, (void*)dd->dof_bs,
                   (void*)dd->leaves, (void*)dd->ranges, (void*)dd->range_map, (void*)dd->geo, (void*)dd->depth,
-                  (void*)dd->n_desc, (void*)dd->dyad, (void*)dd->bs_elem, (void*)dd->bs_pack, (void*)dd->spec_pos})
+                  (void*)dd->n_desc, (void*)dd->dyad, (void*)dd->bs_elem, (void*)dd->bs_pack, (void*)dd->spec_pos,
+                  (void*)dd->dof_sig})
     if (p) (void)hipFree(p);
   delete dd;
 }
