@@ -47,6 +47,8 @@ sys.path.insert(0, ROOT)
 METRIC = "GEP assembly nonzeros/s: mixed-prec vs FP64 GPU at stated rel err; CPU ref"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ENTRY = 20  # col (4) + A (8) + B (8)
+# the fill's Gram library placement (fem2d_timing.fill_lib_lds)
+FILL_PLACEMENT = {0: "global", 1: "lds", 2: "lds_small_blocks"}
 
 
 def build_workload(name: str):
@@ -228,7 +230,7 @@ def other_workload(name: str, steps: int = 10, warmup: int = 3):
     ctx.close()
     return {"value": nnz / (ms * 1e-3), "unit": "nonzeros/s", "ms_per_step": ms, "n_dofs": int(out.n),
             "nnz_upper": nnz, "glq": list(w.glq), "basis": w.basis.__name__, "large_groups": int(t["n_large"]),
-            "fill_chunk": int(t["fill_chunk"]), "fill_lib_in_lds": bool(t["fill_lib_lds"]),
+            "fill_chunk": int(t["fill_chunk"]), "fill_placement": FILL_PLACEMENT[int(t["fill_lib_lds"])],
             "device_ms": {"total": dev_ms, "describe_kernel": float(np.mean(describe)), "fill_kernel": fill_ms},
             "roofline": {"kernel": "k_fill_flat", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -615,7 +617,7 @@ def main():
             "precision": "fp64",
             "large_groups": int(t_last["n_large"]),
             "fill_chunk": int(t_last["fill_chunk"]),
-            "fill_lib_in_lds": bool(t_last["fill_lib_lds"]),
+            "fill_placement": FILL_PLACEMENT[int(t_last["fill_lib_lds"])],
         },
         "element_pairs_per_s": pairs / (ms_per_step * 1e-3),
         "device_ms": {"total": dev_total, "describe_kernel": float(np.mean(count_ms)),

@@ -115,6 +115,17 @@ struct Store<FEM2D_PRECISION_FP32R> {  // 16 B per entry: the FP64 shape
   static constexpr int kLibLds = 12288;
   static constexpr int kChunk = 256;
 };
+// Launch shape of the streaming fill: the precision's own (kSmall 0), or 512
+// threads a block with a 4096-value library copy (kSmall 1), which lets three
+// blocks share a CU when the Domain's library is that small
+template <int P, int kSmall>
+struct FillShape {
+  static constexpr int kBlock = kSmall ? 512 : Store<P>::kBlock;
+  static constexpr int kLibLds = kSmall ? 4096 : Store<P>::kLibLds;
+};
+// placements of the fill's Gram library: global memory, an LDS copy in the
+// precision's shape, an LDS copy in the small shape
+enum FillPlace { kPlaceGlobal = 0, kPlaceLds = 1, kPlaceLdsSmall = 2 };
 constexpr int kFillWavesPerSimd = 6;
 // latency-bound kernels: at most 64 VGPRs, so that 8 wavefronts per SIMD are resident
 constexpr int kLatencyWavesPerSimd = 8;
@@ -1525,15 +1536,16 @@ __device__ __forceinline__ void store_entry(int* __restrict__ col_out, typename 
 // k_fill_large. kACurlCurl / kBCurlCurl: A and B take the CurlCurl values
 // (else L2Inner); P: the stored precision (Store<P>; the arithmetic is FP64
 // in every precision, values are rounded once at the store).
-template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl, int P>
-__global__ void __launch_bounds__(Store<P>::kBlock, kFillWavesPerSimd) k_fill_flat(Work w, LibShape L, FillOut o) {
+template <bool kLibInLds, bool kACurlCurl, bool kBCurlCurl, int P, int kSmall>
+__global__ void __launch_bounds__((FillShape<P, kSmall>::kBlock), kFillWavesPerSimd)
+    k_fill_flat(Work w, LibShape L, FillOut o) {
   using VA = typename Store<P>::A;
   using VB = typename Store<P>::B;
   if (*w.err) return;
   // the source records of the row group being written, per wavefront
-  __shared__ double4 src_lds[Store<P>::kBlock / kWave][kWS * kCoef / 4];
+  __shared__ double4 src_lds[FillShape<P, kSmall>::kBlock / kWave][kWS * kCoef / 4];
   // the block's copy of the Gram library (when it fits)
-  __shared__ double lib_lds[kLibInLds ? Store<P>::kLibLds : 1];
+  __shared__ double lib_lds[kLibInLds ? FillShape<P, kSmall>::kLibLds : 1];
   const long long nnz = w.row_ptr[w.n_dofs];
   if (nnz > o.capacity) return;  // the host grows the buffers and runs the fill again
   if (kLibInLds) {
@@ -1622,42 +1634,66 @@ __global__ void __launch_bounds__(Store<P>::kBlock, kFillWavesPerSimd) k_fill_fl
   }
 }
 
-// one instance per library placement, integral pair and stored precision
-template <bool kLibInLds, int P>
+// one instance per library placement, integral pair, stored precision and shape
+template <bool kLibInLds, int P, int kSmall>
 void launch_fill_lib(bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
                      const FillOut& o) {
-  constexpr int B = Store<P>::kBlock;
-  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false, P><<<grid, B, 0, st>>>(w, L, o);
-  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true, P><<<grid, B, 0, st>>>(w, L, o);
-  else if (a_cc) k_fill_flat<kLibInLds, true, true, P><<<grid, B, 0, st>>>(w, L, o);
-  else k_fill_flat<kLibInLds, false, false, P><<<grid, B, 0, st>>>(w, L, o);
+  constexpr int B = FillShape<P, kSmall>::kBlock;
+  if (a_cc && !b_cc) k_fill_flat<kLibInLds, true, false, P, kSmall><<<grid, B, 0, st>>>(w, L, o);
+  else if (!a_cc && b_cc) k_fill_flat<kLibInLds, false, true, P, kSmall><<<grid, B, 0, st>>>(w, L, o);
+  else if (a_cc) k_fill_flat<kLibInLds, true, true, P, kSmall><<<grid, B, 0, st>>>(w, L, o);
+  else k_fill_flat<kLibInLds, false, false, P, kSmall><<<grid, B, 0, st>>>(w, L, o);
+}
+
+// the small shape exists for the precisions whose own shape is larger
+template <int P>
+constexpr bool kHasSmall = Store<P>::kBlock > 512;
+
+template <int P>
+void launch_fill_prec(int place, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w,
+                      const LibShape& L, const FillOut& o) {
+  if constexpr (kHasSmall<P>) {
+    if (place == kPlaceLdsSmall) return launch_fill_lib<true, P, 1>(a_cc, b_cc, grid, st, w, L, o);
+  }
+  if (place != kPlaceGlobal) launch_fill_lib<true, P, 0>(a_cc, b_cc, grid, st, w, L, o);
+  else launch_fill_lib<false, P, 0>(a_cc, b_cc, grid, st, w, L, o);
+}
+
+void launch_fill(int place, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
+                 const FillOut& o) {
+  switch (o.precision) {
+    case FEM2D_PRECISION_FP32: launch_fill_prec<FEM2D_PRECISION_FP32>(place, a_cc, b_cc, grid, st, w, L, o); break;
+    case FEM2D_PRECISION_FP32R:
+      launch_fill_prec<FEM2D_PRECISION_FP32R>(place, a_cc, b_cc, grid, st, w, L, o);
+      break;
+    default: launch_fill_prec<FEM2D_PRECISION_FP64>(place, a_cc, b_cc, grid, st, w, L, o);
+  }
 }
 
 template <int P>
-void launch_fill_prec(bool lds, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w,
-                      const LibShape& L, const FillOut& o) {
-  if (lds) launch_fill_lib<true, P>(a_cc, b_cc, grid, st, w, L, o);
-  else launch_fill_lib<false, P>(a_cc, b_cc, grid, st, w, L, o);
-}
-
-void launch_fill(bool lds, bool a_cc, bool b_cc, unsigned grid, hipStream_t st, const Work& w, const LibShape& L,
-                 const FillOut& o) {
-  switch (o.precision) {
-    case FEM2D_PRECISION_FP32: launch_fill_prec<FEM2D_PRECISION_FP32>(lds, a_cc, b_cc, grid, st, w, L, o); break;
-    case FEM2D_PRECISION_FP32R:
-      launch_fill_prec<FEM2D_PRECISION_FP32R>(lds, a_cc, b_cc, grid, st, w, L, o);
-      break;
-    default: launch_fill_prec<FEM2D_PRECISION_FP64>(lds, a_cc, b_cc, grid, st, w, L, o);
+long long lib_lds_cap(int place) {
+  if constexpr (kHasSmall<P>) {
+    if (place == kPlaceLdsSmall) return FillShape<P, 1>::kLibLds;
   }
+  return place == kPlaceLds ? Store<P>::kLibLds : 0;
 }
 
-// the library fits the fill's LDS copy
-bool lib_in_lds(int precision, long long lib_n) {
+// the library fits the fill's LDS copy of a placement (kPlaceGlobal always
+// fits; kPlaceLdsSmall only for the precisions that have the small shape)
+bool place_fits(int precision, int place, long long lib_n) {
+  if (place == kPlaceGlobal) return true;
   switch (precision) {
-    case FEM2D_PRECISION_FP32: return lib_n <= Store<FEM2D_PRECISION_FP32>::kLibLds;
-    case FEM2D_PRECISION_FP32R: return lib_n <= Store<FEM2D_PRECISION_FP32R>::kLibLds;
-    default: return lib_n <= Store<FEM2D_PRECISION_FP64>::kLibLds;
+    case FEM2D_PRECISION_FP32: return lib_n <= lib_lds_cap<FEM2D_PRECISION_FP32>(place);
+    case FEM2D_PRECISION_FP32R: return lib_n <= lib_lds_cap<FEM2D_PRECISION_FP32R>(place);
+    default: return lib_n <= lib_lds_cap<FEM2D_PRECISION_FP64>(place);
   }
+}
+
+// the placement an assembly uses: the tuned one, else the LDS copy when the
+// library fits it
+int fill_place(int precision, int tuned, long long lib_n) {
+  if (tuned >= 0 && place_fits(precision, tuned, lib_n)) return tuned;
+  return place_fits(precision, kPlaceLds, lib_n) ? kPlaceLds : kPlaceGlobal;
 }
 
 // ---------------------------------------------------------------------------
@@ -2201,8 +2237,7 @@ struct fem2d_ctx {
   int count_grid = 0;  // resident blocks of the wave kernels on this device
   // resident blocks of the streaming fill per stored precision, with the library in
   // global memory / in LDS
-  int fill_grid[kPrecisions] = {};
-  int fill_grid_lds[kPrecisions] = {};
+  int fill_grid[kPrecisions][3] = {};  // resident blocks of the fill per precision and FillPlace
   bool use_graphs = true;
   bool autotune = true;  // time the fill's chunk sizes on a Domain's first assembly
   int fill_chunk = 0;     // fill chunk size forced for every assembly (0: tuned or default)
@@ -2408,7 +2443,7 @@ struct Assembly {
   long long tiles_nested = 0, tiles_dof = 0, tiles_mem = 0, tiles_grp = 0;
   long long *ts_nested = nullptr, *ts_dof = nullptr, *ts_mem = nullptr, *ts_grp = nullptr;
   bool run_large = true;  // launch the block path (unless the Domain is known to need none)
-  int lib_lds = -1;       // the fill's library copy: -1 in LDS when it fits, 0 global memory, 1 LDS
+  int lib_lds = -1;       // the fill's FillPlace (-1: untuned, the LDS copy when it fits)
 };
 
 // Option and size checks after check_common; *nu, *nv receive the Gauss point counts.
@@ -2633,9 +2668,8 @@ int launch_prep(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, long long* st
 // buffers it writes nothing)
 int launch_stream_fill(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, const FillOut& o) {
   const int pi = o.precision;
-  const bool lds = lib_in_lds(o.precision, a.lib_n) && a.lib_lds != 0;
-  const unsigned grid = unsigned(lds ? ctx->fill_grid_lds[pi] : ctx->fill_grid[pi]);
-  launch_fill(lds, o.a_is_cc != 0, o.b_is_cc != 0, grid, st, a.w, a.L, o);
+  const int place = fill_place(o.precision, a.lib_lds, a.lib_n);
+  launch_fill(place, o.a_is_cc != 0, o.b_is_cc != 0, unsigned(ctx->fill_grid[pi][place]), st, a.w, a.L, o);
   HIP_TRY_AT(hipGetLastError(), "after the fill kernel");
   return FEM2D_OK;
 }
@@ -2653,12 +2687,18 @@ int launch_large_fill(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, const F
 template <int P>
 int fill_grids(fem2d_ctx* ctx) {
   int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, P>,
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<false, true, false, P, 0>,
                                                        Store<P>::kBlock, 0));
-  ctx->fill_grid[P] = std::max(1, per_cu) * ctx->n_cu;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, P>,
+  ctx->fill_grid[P][kPlaceGlobal] = std::max(1, per_cu) * ctx->n_cu;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, P, 0>,
                                                        Store<P>::kBlock, 0));
-  ctx->fill_grid_lds[P] = std::max(1, per_cu) * ctx->n_cu;
+  ctx->fill_grid[P][kPlaceLds] = std::max(1, per_cu) * ctx->n_cu;
+  ctx->fill_grid[P][kPlaceLdsSmall] = ctx->fill_grid[P][kPlaceLds];
+  if constexpr (kHasSmall<P>) {
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_flat<true, true, false, P, 1>,
+                                                         FillShape<P, 1>::kBlock, 0));
+    ctx->fill_grid[P][kPlaceLdsSmall] = std::max(1, per_cu) * ctx->n_cu;
+  }
   return FEM2D_OK;
 }
 
@@ -3113,7 +3153,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   }
   dd->large_state = n_large > 0 ? 1 : 0;
   ctx->timing.fill_chunk = w.chunk;
-  ctx->timing.fill_lib_lds = lib_in_lds(precision, a.lib_n) && a.lib_lds != 0 ? 1 : 0;
+  ctx->timing.fill_lib_lds = fill_place(precision, a.lib_lds, a.lib_n);
   if (ctx->autotune && !ctx->fill_chunk && dd->tuned_chunk[precision] == 0) {
     // the first assembly of this Domain in this precision: time the streaming
     // fill at each candidate chunk size and library placement (best of two
@@ -3123,8 +3163,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     float best_ms = HUGE_VALF;
     const FillOut o = make_fill_out(precision, ai, bi, ctx->out_capacity, static_cast<int*>(ctx->out_col.p),
                                     ctx->out_a.p, ctx->out_b.p, static_cast<unsigned*>(ctx->out_r.p));
-    const bool fits = lib_in_lds(precision, a.lib_n);
-    for (int place = fits ? 1 : 0; place >= 0; --place) {
+    for (int place : {kPlaceLdsSmall, kPlaceLds, kPlaceGlobal}) {
+      if (!place_fits(precision, place, a.lib_n)) continue;
       a.lib_lds = place;
       for (int c : kChunkCandidates) {
         set_chunk(w, c, precision);

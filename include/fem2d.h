@@ -372,7 +372,8 @@ typedef struct fem2d_timing {
   int64_t nnz;
   int64_t n_large;    /* row groups assembled one block per group (large column sets) */
   int64_t fill_chunk; /* CSR entries per work item of the entry fill (default, tuned or forced) */
-  int64_t fill_lib_lds; /* 1 when the entry fill read the Gram library from an LDS copy */
+  int64_t fill_lib_lds; /* where the entry fill read the Gram library: 0 global memory, 1 an LDS copy,
+                           2 an LDS copy in 512-thread blocks (default or tuned) */
 } fem2d_timing;
 int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out);
 
