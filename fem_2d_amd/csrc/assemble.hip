@@ -1553,18 +1553,6 @@ __global__ void __launch_bounds__((FillShape<P, kSmall>::kBlock), kFillWavesPerS
     for (int i = threadIdx.x; i < n_lib; i += blockDim.x) lib_lds[i] = w.lib[i];
     __syncthreads();
   }
-  // row pointers of the streaming-fill groups' rows: row q of group g starts
-  // q T - q (q - 1) / 2 entries into the group's segment (the layout scan
-  // writes those of block-path groups)
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < w.n_dofs;
-       r += (long long)gridDim.x * blockDim.x) {
-    const int g = w.dof_grp[r] - 1;
-    const int T = w.meta_hdr[g].x;
-    if (T > 0) {
-      const long long q = r - w.g_start[g];
-      w.row_ptr[r] = w.seg_start[g] + q * T - q * (q - 1) / 2;
-    }
-  }
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
   const int waves = int((gridDim.x * blockDim.x) / kWave);
@@ -1630,6 +1618,19 @@ __global__ void __launch_bounds__((FillShape<P, kSmall>::kBlock), kFillWavesPerS
       }
       j0 = j1;
       ++g;
+    }
+  }
+  // row pointers of the streaming-fill groups' rows, after the entries (the
+  // fill reads none of them; written last they overlap other wavefronts'
+  // final chunks): row q of group g starts q T - q (q - 1) / 2 entries into
+  // the group's segment (the layout scan writes those of block-path groups)
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < w.n_dofs;
+       r += (long long)gridDim.x * blockDim.x) {
+    const int g = w.dof_grp[r] - 1;
+    const int T = w.meta_hdr[g].x;
+    if (T > 0) {
+      const long long q = r - w.g_start[g];
+      w.row_ptr[r] = w.seg_start[g] + q * T - q * (q - 1) / 2;
     }
   }
 }
