@@ -534,7 +534,16 @@ def main():
 
     w = build_workload(args.workload)
     ctx = F.Context(0)
+    # the Domain's upload (host view to HBM, and the device kernels that derive its per-Elem,
+    # per-spec and per-DoF fields), once per Domain, outside the timed assemblies: timed on a
+    # second upload (the first loads the kernels) and reported beside them
+    warm = F.DeviceDomain(w.domain, ctx)
+    del warm
+    ctx.synchronize()
+    t_up = time.perf_counter()
     dd = F.DeviceDomain(w.domain, ctx)
+    ctx.synchronize()
+    upload_ms = (time.perf_counter() - t_up) * 1e3
 
     def step():
         return dd.assemble(w.glq, w.basis, F.CurlCurl, F.L2Inner)
@@ -620,6 +629,7 @@ def main():
             "fill_placement": FILL_PLACEMENT[int(t_last["fill_lib_lds"])],
         },
         "element_pairs_per_s": pairs / (ms_per_step * 1e-3),
+        "domain_upload_ms": upload_ms,
         "device_ms": {"total": dev_total, "describe_kernel": float(np.mean(count_ms)),
                       "fill_kernel": fill_avg},
         # the whole assembly (every launch of the recorded graph) against the same output bytes
