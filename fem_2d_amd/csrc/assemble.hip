@@ -2717,8 +2717,10 @@ FillOut make_fill_out(int precision, int ai, int bi, long long capacity, int* co
   return o;
 }
 
-// chunk sizes the fill's autotuning times on a Domain's first assembly
+// chunk sizes the fill's autotuning times on a Domain's first assembly, each
+// placement and size timed kTuneReps times (the fastest launch counts)
 constexpr int kChunkCandidates[] = {128, 256, 512, 1024};
+constexpr int kTuneReps = 3;
 
 
 // CSR entries per fill chunk for a stored precision (before any tuning)
@@ -3157,8 +3159,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   ctx->timing.fill_lib_lds = fill_place(precision, a.lib_lds, a.lib_n);
   if (ctx->autotune && !ctx->fill_chunk && dd->tuned_chunk[precision] == 0) {
     // the first assembly of this Domain in this precision: time the streaming
-    // fill at each candidate chunk size and library placement (best of two
-    // launches each, after the layout scan for that size), keep the fastest,
+    // fill at each candidate chunk size and library placement (best of
+    // kTuneReps launches each, after the layout scan for that size), keep the fastest,
     // and assemble again with it
     int best = w.chunk, best_lds = -1;
     float best_ms = HUGE_VALF;
@@ -3171,7 +3173,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
         set_chunk(w, c, precision);
         w.chunk_cap = o.capacity / w.chunk + 1;
         if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
-        for (int rep = 0; rep < 2; ++rep) {
+        for (int rep = 0; rep < kTuneReps; ++rep) {
           if ((rc = launch_prep(ctx, st, a, ctx->d_status, false))) return rc;
           HIP_TRY(hipEventRecord(ctx->ev[6], st));
           if ((rc = launch_stream_fill(ctx, st, a, o))) return rc;
