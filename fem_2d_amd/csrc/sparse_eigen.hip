@@ -18,23 +18,24 @@
 // block i to block i + 1,
 //   S_0 = D_0,  S_i = L_i L_i^T (potrf),  W_i = L_i^-1 U_i (trsm),
 //   S_{i+1} = D_{i+1} - W_i^T W_i (syrk).
-// Once W_i is formed, L_i is replaced by its inverse (trtri): sum_i w_i^2
-// doubles of HBM. Cholesky is backward stable for SPD matrices, so the
-// Schur complements stay SPD even though K' reaches condition numbers ~1e13
-// on high-order bases (the mass matrix's smallest eigenvalues are ~1e-10).
-// Explicit inverses of the S_i lose accuracy with the square of that
-// condition number and broke the elimination down past ~1e5 DoFs;
-// eliminating the indefinite A - target B the same way is not stable either.
-// The inverses of the triangular factors lose only its first power: M
-// below solves K' to ~1e-3 relative on the 4 496-DoF precision-study GEP
-// (1e-4 with triangular solves, at a hundred times the time).
+// Once W_i is formed, L_i is replaced by its inverse (trtri), kept beside its
+// transpose: 2 sum_i w_i^2 doubles of HBM. Cholesky is backward stable for
+// SPD matrices, so the Schur complements stay SPD even though K' reaches
+// condition numbers ~1e13 on high-order bases (the mass matrix's smallest
+// eigenvalues are ~1e-10). Explicit inverses of the S_i lose accuracy with the
+// square of that condition number and broke the elimination down past ~1e5
+// DoFs; eliminating the indefinite A - target B the same way is not stable
+// either. The inverses of the triangular factors lose only its first power: M
+// below solves K' to ~1e-3 relative on the 4 496-DoF precision-study GEP.
 //
 // Solve K' x = b in level order (the preconditioner M = K'^-1):
 //   forward   z_i = S_i^-1 (b_i - U_{i-1}^T z_{i-1})
 //   backward  x_last = z_last,  x_i = z_i - S_i^-1 (U_i x_{i+1})
-// with S_i^-1 v = L_i^-T (L_i^-1 v): four triangular matrix-vector products
-// per block solve (each reads half a block, HBM-bound) and sparse products
-// with the coupling entries, taken from the CSR in place.
+// with S_i^-1 v = L_i^-T (L_i^-1 v): L_i^-1 v as the column dot products of
+// the stored transpose, L_i^-T of the result as those of L_i^-1 itself, one
+// wavefront per column reading it contiguously (HBM-bound; rocBLAS trmv ran
+// these at ~1.4 TB/s), and sparse products with the coupling entries, taken
+// from the CSR in place.
 //
 // Eigenpair: shift-invert inverse iteration in residual-correction form,
 // x <- x - d with (A - target B) d = A x - lambda B x solved by GMRES
@@ -351,8 +352,6 @@ __global__ void k_fill(double* __restrict__ p, long long count, double v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
 }
 
-// largest magnitude of a block's entries into *out (a non-finite entry gives
-// +inf); out starts at 0 (one block per launch, grid-stride)
 // *out = min over j of L[j, j]^2 from the inverse factor's diagonal 1 / L[j, j]
 // (as the bits of a non-negative double, whose integer order is the numeric
 // order; *out starts at all ones)
@@ -363,11 +362,47 @@ __global__ void k_min_pivot(const double* __restrict__ Linv, long long w, unsign
   atomicMin(out, static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
-// the w x w identity (column-major)
-__global__ void k_identity(double* __restrict__ p, long long w) {
+// sum over the wavefront (every lane receives it)
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+// u = l^T for the lower triangle of a column-major w x w block: u[j + i w] =
+// l[i + j w] for i >= j (u's strict lower triangle is left as it is)
+__global__ void k_transpose_lower(const double* __restrict__ l, double* __restrict__ u, long long w) {
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < w * w; i += stride)
-    p[i] = (i % w) == (i / w) ? 1.0 : 0.0;
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < w * w; k += stride) {
+    const long long i = k % w, j = k / w;
+    if (i >= j) u[j + i * w] = l[k];
+  }
+}
+
+// y[c] = sum_r m[r + c w] x[r] (+ beta y[c]) over the rows r <= c of an
+// upper-triangular column-major block (kUpper) or r >= c of a lower one: one
+// wavefront per column, reading it contiguously (four loads in flight a lane)
+template <bool kUpper>
+__global__ void k_tri_cols(const double* __restrict__ m, long long w, const double* __restrict__ x,
+                           double* __restrict__ y, double beta) {
+  const int lane = threadIdx.x % kWave;
+  const long long wave = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave;
+  const long long waves = ((long long)gridDim.x * blockDim.x) / kWave;
+  for (long long c = wave; c < w; c += waves) {
+    const long long r0 = kUpper ? 0 : c, r1 = kUpper ? c + 1 : w;
+    const double* __restrict__ col = m + c * w;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    long long r = r0 + lane;
+    for (; r + 3 * kWave < r1; r += 4 * kWave) {
+      s0 += col[r] * x[r];
+      s1 += col[r + kWave] * x[r + kWave];
+      s2 += col[r + 2 * kWave] * x[r + 2 * kWave];
+      s3 += col[r + 3 * kWave] * x[r + 3 * kWave];
+    }
+    for (; r < r1; r += kWave) s0 += col[r] * x[r];
+    const double s = wave_sum((s0 + s1) + (s2 + s3));
+    if (lane == 0) y[c] = beta == 0.0 ? s : s + beta * y[c];
+  }
 }
 
 // entries of K = A - sigma B into a dense column-major block (leading
@@ -384,11 +419,6 @@ __global__ void k_scatter(long long count, const int* __restrict__ ks, const int
   }
 }
 
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
-  return x;
-}
 
 // y[p] = beta y[p] - sum over the listed entries (k, c) of p: K_k x[c], for
 // the level-order positions p in [p0, p1) (one wavefront per row)
@@ -474,11 +504,13 @@ struct Factor {
   const double* a = nullptr;  // FP64 values of the stored entries
   const double* b = nullptr;
   double* chol = nullptr;     // the inverses of the blocks' Cholesky factors L_i^-1 (lower, column-major), at off[i]
+  double* cholT = nullptr;    // their transposes (upper, column-major), at the same offsets
   std::vector<long long> off;
   // device copies of the plan's lists
   int *d_k, *d_r, *d_c, *u_k, *u_r, *u_c, *ur_k, *ur_c, *ut_k, *ut_c, *perm;
   long long *ur_ptr, *ut_ptr;
   double *y, *z, *t;  // level-order work vectors
+  double* tmp = nullptr;  // one block's intermediate L_i^-1 v
   unsigned long long* pivots = nullptr;  // [nb] smallest squared diagonal entry of each factor (as bits)
   double min_pivot = 0.0;                // smallest over all factors
   int bad_block = -1;                    // first block that is not positive definite
@@ -518,6 +550,8 @@ int factorize(Factor& F, double* wbuf, rocblas_int* d_info) {
     EIG_BLAS(rocsolver_dpotrf(F.h, rocblas_fill_lower, wi, L, wi, d_info + i));
     if (i + 1 == P.nb) {
       EIG_BLAS(rocsolver_dtrtri(F.h, rocblas_fill_lower, rocblas_diagonal_non_unit, wi, L, wi, d_info + P.nb + i));
+      k_transpose_lower<<<grid_for(w * w), kBlock, 0, st>>>(L, F.cholT + F.off[size_t(i)], w);
+      EIG_HIP(hipGetLastError());
       break;
     }
     const long long w2 = w_of(i + 1);
@@ -531,8 +565,11 @@ int factorize(Factor& F, double* wbuf, rocblas_int* d_info) {
     EIG_HIP(hipGetLastError());
     EIG_BLAS(rocblas_dtrsm(F.h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                            rocblas_diagonal_non_unit, wi, wn, &one, L, wi, wbuf, wi));
-    // L_i is needed no more: keep its inverse for the solves
+    // L_i is needed no more: keep its inverse (and that inverse's transpose)
+    // for the solves
     EIG_BLAS(rocsolver_dtrtri(F.h, rocblas_fill_lower, rocblas_diagonal_non_unit, wi, L, wi, d_info + P.nb + i));
+    k_transpose_lower<<<grid_for(w * w), kBlock, 0, st>>>(L, F.cholT + F.off[size_t(i)], w);
+    EIG_HIP(hipGetLastError());
     // S_{i+1} = D_{i+1} - W_i^T W_i (lower triangle, which is all potrf reads)
     if (int rc = load_diag(i + 1)) return rc;
     EIG_BLAS(rocblas_dsyrk(F.h, rocblas_fill_lower, rocblas_operation_transpose, wn, wi, &m_one, wbuf, wi, &one,
@@ -566,14 +603,15 @@ int solve(Factor& F, const double* f, double* out) {
   const Plan& P = *F.P;
   const hipStream_t st = F.st;
   const long long n = P.n;
-  const double one = 1.0;
-  auto chol_solve = [&](int i, double* v) -> int {  // v <- S_i^-1 v = L_i^-T (L_i^-1 v)
-    const rocblas_int wi = rocblas_int(P.bstart[size_t(i) + 1] - P.bstart[size_t(i)]);
-    const double* Li = F.chol + F.off[size_t(i)];
-    EIG_BLAS(rocblas_dtrmv(F.h, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit, wi, Li, wi, v,
-                           1));
-    EIG_BLAS(rocblas_dtrmv(F.h, rocblas_fill_lower, rocblas_operation_transpose, rocblas_diagonal_non_unit, wi, Li,
-                           wi, v, 1));
+  // out <- S_i^-1 in + beta out = L_i^-T (L_i^-1 in) + beta out: L_i^-1 in as
+  // the column dots of its transpose, then L_i^-T of that as the column dots
+  // of L_i^-1 itself (both read their blocks contiguously)
+  auto chol_solve = [&](int i, const double* in, double* out, double beta) -> int {
+    const long long w = P.bstart[size_t(i) + 1] - P.bstart[size_t(i)];
+    const unsigned grid = grid_for(w * kWave);
+    k_tri_cols<true><<<grid, kBlock, 0, st>>>(F.cholT + F.off[size_t(i)], w, in, F.tmp, 0.0);
+    k_tri_cols<false><<<grid, kBlock, 0, st>>>(F.chol + F.off[size_t(i)], w, F.tmp, out, beta);
+    EIG_HIP(hipGetLastError());
     return FEM2D_OK;
   };
   k_permute<<<grid_for(n), kBlock, 0, st>>>(n, F.perm, f, F.y, 0);
@@ -581,8 +619,7 @@ int solve(Factor& F, const double* f, double* out) {
   for (int i = 0; i < P.nb; ++i) {
     const long long p0 = P.bstart[size_t(i)], p1 = P.bstart[size_t(i) + 1];
     // z_i = S_i^-1 y_i
-    EIG_HIP(hipMemcpyAsync(F.z + p0, F.y + p0, size_t(p1 - p0) * sizeof(double), hipMemcpyDeviceToDevice, st));
-    if (int rc = chol_solve(i, F.z + p0)) return rc;
+    if (int rc = chol_solve(i, F.y + p0, F.z + p0, 0.0)) return rc;
     if (i + 1 < P.nb) {  // y_{i+1} -= U_i^T z_i
       const long long q1 = P.bstart[size_t(i) + 2];
       k_coupling<<<row_grid(q1 - p1), kBlock, 0, st>>>(p1, q1, F.ut_ptr, F.ut_k, F.ut_c, F.a, F.b, F.sigma, F.z, F.y,
@@ -592,13 +629,11 @@ int solve(Factor& F, const double* f, double* out) {
   }
   for (int i = P.nb - 2; i >= 0; --i) {
     const long long p0 = P.bstart[size_t(i)], p1 = P.bstart[size_t(i) + 1];
-    const rocblas_int wi = rocblas_int(p1 - p0);
     // t_i = -U_i x_{i+1}; x_i = z_i + S_i^-1 t_i
     k_coupling<<<row_grid(p1 - p0), kBlock, 0, st>>>(p0, p1, F.ur_ptr, F.ur_k, F.ur_c, F.a, F.b, F.sigma, F.z, F.t,
                                                     0.0);
     EIG_HIP(hipGetLastError());
-    if (int rc = chol_solve(i, F.t + p0)) return rc;
-    EIG_BLAS(rocblas_daxpy(F.h, wi, &one, F.t + p0, 1, F.z + p0, 1));
+    if (int rc = chol_solve(i, F.t + p0, F.z + p0, 1.0)) return rc;
   }
   k_permute<<<grid_for(n), kBlock, 0, st>>>(n, F.perm, F.z, out, 1);
   EIG_HIP(hipGetLastError());
@@ -779,12 +814,12 @@ int sparse_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem
   if (int rc = build_plan(n, rp, col, P)) return rc;
   stats->blocks = P.nb;
   stats->max_block = int(P.max_w);
-  stats->factor_bytes = 8.0 * P.sum_w2;
+  stats->factor_bytes = 16.0 * P.sum_w2;  // the inverse factors and their transposes
   stats->symbolic_ms = ms_since(t_sym);
   // memory: the factors, one coupling-size block, values, lists, vectors
   size_t free_b = 0, total_b = 0;
   EIG_HIP(hipMemGetInfo(&free_b, &total_b));
-  const double need = 8.0 * P.sum_w2 + 8.0 * double(P.max_coupling) + 16.0 * double(nnz) +
+  const double need = 16.0 * P.sum_w2 + 8.0 * double(P.max_coupling) + 16.0 * double(nnz) +
                       4.0 * double(P.d_k.size() * 3 + P.u_k.size() * 7 + P.lo_k.size() * 2) + 120.0 * double(n);
   if (need > 0.92 * double(free_b)) {
     set_error("the block factorisation of this GEP needs " + std::to_string(need / 1e9) + " GB of device memory, " +
@@ -831,6 +866,8 @@ int sparse_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem
   double *wbuf, *x, *bx, *ax, *y, *ay, *by, *r, *dy;
   rocblas_int* d_info;
   EIG_HIP(m.alloc(&F.chol, size_t(F.off[size_t(P.nb)])));
+  EIG_HIP(m.alloc(&F.cholT, size_t(F.off[size_t(P.nb)])));
+  EIG_HIP(m.alloc(&F.tmp, size_t(std::max(1LL, P.max_w))));
   EIG_HIP(m.alloc(&wbuf, size_t(std::max(1LL, P.max_coupling))));
   EIG_HIP(m.alloc(&d_info, 2 * size_t(P.nb)));
   EIG_HIP(m.alloc(&F.pivots, size_t(P.nb)));

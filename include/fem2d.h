@@ -298,8 +298,8 @@ int fem2d_gep_device_solve_dense(fem2d_ctx* ctx, const fem2d_gep_device* gep, do
  * preconditions GMRES solves with A - target B inside a shift-invert inverse
  * iteration that runs against the sparse matrices in place. It converges when the relative residual
  * |A x - lambda B x| / max(|A x|, |target| |B x|) reaches 1e-12 (1e-10 once the
- * Rayleigh quotient stops moving). Device memory: 8 bytes times the sum of the
- * squared block widths (about 21 GB for the 445 380-DoF benchmark GEP).
+ * Rayleigh quotient stops moving). Device memory: 16 bytes times the sum of the
+ * squared block widths (about 41 GB for the 445 380-DoF benchmark GEP).
  * Errors: FEM2D_ERR_PROBLEM_TOO_LARGE (the blocks do not fit the free device
  * memory), FEM2D_ERR_FAILED_TO_INVERT_B (A + tau B is not positive definite),
  * FEM2D_ERR_SPURIOUSLY_CONVERGED (no DoFs), FEM2D_ERR_NOT_CONVERGED. */
@@ -310,7 +310,7 @@ typedef struct fem2d_eigen_stats {
   int64_t n;                 /* DoFs */
   int32_t blocks;            /* blocks of the level ordering */
   int32_t max_block;         /* widest block */
-  double factor_bytes;       /* device bytes of the kept block Cholesky factors */
+  double factor_bytes;       /* device bytes of the kept inverse Cholesky factors and their transposes */
   int32_t iterations;        /* outer (shift-invert) iterations */
   int32_t inner_iterations;  /* preconditioned GMRES steps over all outer iterations */
   double residual;           /* final relative residual */
