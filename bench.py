@@ -258,8 +258,8 @@ def sparse_eigensolve(w):
     ms = (time.perf_counter() - t0) * 1e3
     st = ctx.last_eigen_stats()
     out = {"solver": "block-tridiagonal level-set ordering; block Cholesky of A + tau B (rocSOLVER potrf/trtri, "
-                     "rocBLAS trsm/syrk/trmv) preconditioning GMRES inside a residual-correction shift-invert "
-                     "iteration; FP64",
+                     "rocBLAS trsm/syrk; applied by column-dot kernels over the inverse factors and their "
+                     "transposes) preconditioning GMRES inside a residual-correction shift-invert iteration; FP64",
            w.name: {"n": int(dev.n), "ms": ms, "eigenvalue": pair.value,
                     "rel_err_vs_te10": abs(pair.value - np.pi ** 2) / np.pi ** 2,
                     "residual": st["residual"], "blocks": st["blocks"], "max_block": st["max_block"],
