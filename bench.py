@@ -314,27 +314,35 @@ def host_boundary(w, nnz: int, calls: int = 3):
 
 
 def domain_build(w, calls: int = 3):
-    """Host Domain construction from the workload's Mesh (Domain::from_mesh, domain.rs:69-159):
-    the data-parallel builder against the sequential one (the reference's construction order);
-    both produce the same Domain. Mean over `calls` builds, after one untimed build of each."""
+    """Domain construction from the workload's Mesh (Domain::from_mesh, domain.rs:69-159), three
+    ways, all giving the same Domain: on the GPU straight into HBM (fem2d_device_domain_from_mesh:
+    host flattening of the Mesh, one upload, the kernels, the Domain's derived device fields), and
+    on the host data-parallel and sequentially (the reference's construction order; neither
+    includes the later upload of the view, ``domain_upload_ms``). Mean over `calls` builds, after
+    one untimed build of each; each build includes freeing the previous one."""
     import fem_2d_amd as F
 
     mesh = w.domain.mesh
+    ctx = F.Context(0)
     out = {}
-    for name, build in (("parallel", lambda: F.Domain.from_mesh(mesh, F.ContinuityCondition.HCurl)),
+    for name, build in (("device", lambda: F.DeviceDomain.from_mesh(mesh, F.ContinuityCondition.HCurl, ctx=ctx)),
+                        ("parallel", lambda: F.Domain.from_mesh(mesh, F.ContinuityCondition.HCurl)),
                         ("serial", lambda: F.Domain.from_mesh_serial(mesh, F.ContinuityCondition.HCurl))):
         d = build()
-        n = d.num_dofs
+        n = d.n_dofs if name == "device" else d.num_dofs
         del d
         t0 = time.perf_counter()
         for _ in range(calls):
             d = build()
             del d
         out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / calls
+    ctx.close()
     out["n_dofs"] = n
     out["threads"] = min(16, os.cpu_count() or 1)
     out["speedup"] = out["serial_ms"] / out["parallel_ms"]
+    out["device_speedup_vs_serial"] = out["serial_ms"] / out["device_ms"]
     out["dofs_per_s"] = n / (out["parallel_ms"] * 1e-3)
+    out["device_dofs_per_s"] = n / (out["device_ms"] * 1e-3)
     return out
 
 

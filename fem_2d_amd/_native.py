@@ -165,6 +165,9 @@ SIGNATURES = {
         C.c_int, [vp, P(DomainView), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepCsr)]),
     "fem2d_gep_csr_free": (None, [P(GepCsr)]),
     "fem2d_device_domain_upload": (C.c_int, [vp, P(DomainView), P(vp)]),
+    "fem2d_device_domain_from_mesh": (C.c_int, [vp, vp, i32, P(vp)]),
+    "fem2d_device_domain_info": (C.c_int, [vp, P(i32), P(i64), P(i64)]),
+    "fem2d_device_domain_download_view": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "fem2d_device_domain_free": (None, [vp]),
     "fem2d_assemble_device": (
         C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(GepDevice)]),

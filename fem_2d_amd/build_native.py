@@ -25,7 +25,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 HOST_SOURCES = ["host/mesh.cpp", "host/domain.cpp", "host/capi_host.cpp"]
-HIP_SOURCES = ["assemble.hip", "petsc.hip", "eigen.hip", "sparse_eigen.hip"]
+HIP_SOURCES = ["assemble.hip", "domain_build.hip", "petsc.hip", "eigen.hip", "sparse_eigen.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra"]
 HIPFLAGS = [

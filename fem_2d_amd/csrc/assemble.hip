@@ -53,6 +53,7 @@
 #include "host/capi_internal.hpp"
 #include "host/parallel.hpp"
 #include "fp32r.hpp"
+#include "domain_build.hpp"
 #include "pinned_staging.hpp"
 
 namespace {
@@ -2218,6 +2219,8 @@ struct fem2d_device_domain {
   unsigned* bs_pack = nullptr;
   unsigned* spec_pos = nullptr;
   int4* dof_sig = nullptr;
+  // one allocation holding every array above (Domains built on the device), else null
+  void* arena = nullptr;
 };
 
 struct fem2d_ctx {
@@ -2494,19 +2497,11 @@ DevView dev_view(const fem2d_device_domain* dd) {
   return v;
 }
 
-// k_domain_prep into the device Domain's derived arrays (allocated here); the
-// view's relation checks surface as the call's status
-int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
+// k_domain_prep and k_dof_sig into the device Domain's derived arrays (already
+// allocated), with the error word at d_err; no synchronisation
+int domain_prep_launch(fem2d_ctx* ctx, fem2d_device_domain* dd, int* d_err) {
   const int ne = dd->n_elems;
-  const size_t E1 = size_t(std::max(ne, 1)), B1 = size_t(std::max(dd->n_bs, 1LL));
-  if (int rc = alloc_device(E1, &dd->geo)) return rc;
-  if (int rc = alloc_device(E1, &dd->depth)) return rc;
-  if (int rc = alloc_device(E1, &dd->n_desc)) return rc;
-  if (int rc = alloc_device(E1, &dd->dyad)) return rc;
-  if (int rc = alloc_device(B1, &dd->bs_elem)) return rc;
-  if (int rc = alloc_device(B1, &dd->bs_pack)) return rc;
-  if (int rc = alloc_device(B1, &dd->spec_pos)) return rc;
-  if (int rc = alloc_device(size_t(std::max(dd->n_dofs, 1LL)), &dd->dof_sig)) return rc;
+  const size_t E1 = size_t(std::max(ne, 1));
   hipStream_t st = ctx->stream;
   Work w{};
   w.geo = dd->geo;
@@ -2517,11 +2512,7 @@ int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
   w.bs_pack = dd->bs_pack;
   w.spec_pos = dd->spec_pos;
   w.dof_sig = dd->dof_sig;
-  if (int rc = alloc_device(2, &w.err)) return rc;
-  struct FreeErr {
-    int* p;
-    ~FreeErr() { (void)hipFree(p); }
-  } free_err{w.err};
+  w.err = d_err;
   HIP_TRY(hipMemsetAsync(dd->depth, 0, E1 * sizeof(int), st));
   HIP_TRY(hipMemsetAsync(dd->n_desc, 0, E1 * sizeof(int), st));
   HIP_TRY(hipMemsetAsync(w.err, 0, sizeof(int), st));
@@ -2533,9 +2524,31 @@ int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
   // the DoF signatures read the specs' owners written above
   k_dof_sig<<<grid_for(dd->n_dofs, kBlock), kBlock, 0, st>>>(dev_view(dd), w);
   HIP_TRY_AT(hipGetLastError(), "after the Domain preparation kernels");
+  return FEM2D_OK;
+}
+
+// the device Domain's derived arrays, allocated here, then k_domain_prep; the
+// view's relation checks surface as the call's status
+int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
+  const size_t E1 = size_t(std::max(dd->n_elems, 1)), B1 = size_t(std::max(dd->n_bs, 1LL));
+  if (int rc = alloc_device(E1, &dd->geo)) return rc;
+  if (int rc = alloc_device(E1, &dd->depth)) return rc;
+  if (int rc = alloc_device(E1, &dd->n_desc)) return rc;
+  if (int rc = alloc_device(E1, &dd->dyad)) return rc;
+  if (int rc = alloc_device(B1, &dd->bs_elem)) return rc;
+  if (int rc = alloc_device(B1, &dd->bs_pack)) return rc;
+  if (int rc = alloc_device(B1, &dd->spec_pos)) return rc;
+  if (int rc = alloc_device(size_t(std::max(dd->n_dofs, 1LL)), &dd->dof_sig)) return rc;
+  int* d_err = nullptr;
+  if (int rc = alloc_device(2, &d_err)) return rc;
+  struct FreeErr {
+    int* p;
+    ~FreeErr() { (void)hipFree(p); }
+  } free_err{d_err};
+  if (int rc = domain_prep_launch(ctx, dd, d_err)) return rc;
   int err = 0;
-  HIP_TRY(hipMemcpyAsync(&err, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   return err ? err_to_status(err) : FEM2D_OK;
 }
 
@@ -2722,7 +2735,6 @@ FillOut make_fill_out(int precision, int ai, int bi, long long capacity, int* co
 constexpr int kChunkCandidates[] = {128, 256, 512, 1024};
 constexpr int kTuneReps = 3;
 
-
 // CSR entries per fill chunk for a stored precision (before any tuning)
 int chunk_of(int precision) {
   switch (precision) {
@@ -2734,6 +2746,101 @@ int chunk_of(int precision) {
 
 // w.chunk for a chunk size (0: the stored precision's default)
 void set_chunk(Work& w, int chunk, int precision) { w.chunk = chunk > 0 ? chunk : chunk_of(precision); }
+
+// Host metadata of a device Domain that depends on the Elem tree only: the
+// sub-ranges of the Gram library (the position of each Elem inside each of its
+// ancestors, per axis: h_refinement.rs:247-279 halves the range per HRefLoc,
+// plus the whole range), their direct map for shallow Domains, the size of the
+// nested lists and the leaves.
+struct TreeMeta {
+  long long nested_cap = 0;     // sum over Elems of 1 + 2 * depth
+  std::vector<long long> keys;  // range keys, ascending per axis, x axis then y
+  int nr_u = 0, nr_v = 0;
+  std::vector<int> range_map;   // empty unless every key is at most kDenseRangeDepth deep
+  int range_map_v = 0;
+  std::vector<int> leaves;      // Elems without children, ascending
+};
+
+void tree_meta(int ne, const int* parent, const unsigned char* loc, TreeMeta& m) {
+  // keys up to kBitDepth halvings deep are collected in a bitmap indexed by
+  // 2^d - 1 + i, whose ascending order is the keys' order; deeper ones are sorted
+  constexpr int kBitDepth = 16;
+  std::vector<unsigned long long> bits[2];
+  std::vector<long long> deep[2];
+  for (int ax = 0; ax < 2; ++ax) bits[ax].assign(((size_t(2) << kBitDepth) + 63) / 64, 0);
+  auto add = [&](int ax, int d, int i) {
+    if (d <= kBitDepth) {
+      const size_t b = (size_t(1) << d) - 1 + size_t(i);
+      bits[ax][b >> 6] |= 1ull << (b & 63);
+    } else {
+      deep[ax].push_back(range_key(d, i));
+    }
+  };
+  add(0, 0, 0);
+  add(1, 0, 0);
+  m.nested_cap = 0;
+  for (int e = 0; e < ne; ++e) {
+    unsigned char locs[kMaxDepth + 1];
+    int steps = 0;
+    for (int x = e; x >= 0 && parent[x] >= 0 && parent[x] < ne && steps <= kMaxDepth; x = parent[x], ++steps)
+      locs[steps] = loc[x];
+    m.nested_cap += 1 + 2LL * steps;
+    if (steps > kMaxDepth) continue;  // rejected by the device checks
+    // Elem e inside its ancestor t levels up: the steps locs[t-1] (topmost) .. locs[0]
+    for (int t = 1; t <= steps; ++t) {
+      int du = 0, iu = 0, dv = 0, iv = 0;
+      for (int k = t - 1; k >= 0; --k) {
+        const int l = locs[k];
+        if (l <= FEM2D_LOC_E) {
+          iu = 2 * iu + (l == FEM2D_LOC_SE || l == FEM2D_LOC_NE || l == FEM2D_LOC_E ? 1 : 0);
+          ++du;
+        }
+        if (l <= FEM2D_LOC_NE || l >= FEM2D_LOC_S) {
+          iv = 2 * iv + (l == FEM2D_LOC_NW || l == FEM2D_LOC_NE || l == FEM2D_LOC_N ? 1 : 0);
+          ++dv;
+        }
+      }
+      add(0, du, iu);
+      add(1, dv, iv);
+    }
+  }
+  std::vector<long long> keys[2];
+  for (int ax = 0; ax < 2; ++ax) {
+    for (size_t w = 0; w < bits[ax].size(); ++w)
+      for (unsigned long long v = bits[ax][w]; v; v &= v - 1) {
+        const size_t b = w * 64 + size_t(__builtin_ctzll(v));
+        const int d = 63 - __builtin_clzll((unsigned long long)(b + 1));
+        keys[ax].push_back(range_key(d, int(b + 1 - (size_t(1) << d))));
+      }
+    std::sort(deep[ax].begin(), deep[ax].end());
+    deep[ax].erase(std::unique(deep[ax].begin(), deep[ax].end()), deep[ax].end());
+    keys[ax].insert(keys[ax].end(), deep[ax].begin(), deep[ax].end());
+  }
+  m.nr_u = int(std::min<size_t>(keys[0].size(), size_t(1) << 30));
+  m.nr_v = int(std::min<size_t>(keys[1].size(), size_t(1) << 30));
+  // shallow Domains (every sub-range at most kDenseRangeDepth halvings deep):
+  // a direct map by dyadic id, one load instead of a search per lookup
+  m.range_map.clear();
+  m.range_map_v = 0;
+  if (int(keys[0].back() >> 32) <= kDenseRangeDepth && int(keys[1].back() >> 32) <= kDenseRangeDepth) {
+    const int su = (1 << (int(keys[0].back() >> 32) + 1)) - 1;
+    const int sv = (1 << (kDenseRangeDepth + 1)) - 1;
+    m.range_map.assign(size_t(su + sv), -1);
+    for (size_t k = 0; k < keys[0].size(); ++k)
+      m.range_map[size_t((1 << int(keys[0][k] >> 32)) - 1 + int(keys[0][k] & 0xffffffffLL))] = int(k);
+    for (size_t k = 0; k < keys[1].size(); ++k)
+      m.range_map[size_t(su + (1 << int(keys[1][k] >> 32)) - 1 + int(keys[1][k] & 0xffffffffLL))] = int(k);
+    m.range_map_v = su;
+  }
+  m.keys = std::move(keys[0]);
+  m.keys.insert(m.keys.end(), keys[1].begin(), keys[1].end());
+  std::vector<char> has_child(size_t(ne), 0);
+  for (int e = 0; e < ne; ++e)
+    if (parent[e] >= 0 && parent[e] < ne) has_child[size_t(parent[e])] = 1;
+  m.leaves.clear();
+  for (int e = 0; e < ne; ++e)
+    if (!has_child[size_t(e)]) m.leaves.push_back(e);
+}
 
 }  // namespace
 
@@ -2901,59 +3008,18 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
   dd->n_elems = ne;
   dd->n_dofs = v->n_dofs;
   dd->n_bs = nbs;
-  // host metadata: maximum orders (Mesh::max_expansion_orders, mesh.rs:626-630),
-  // and the sub-ranges of the Gram library: the position of each Elem inside
-  // each of its ancestors, per axis (h_refinement.rs:247-279 halves the range
-  // per HRefLoc), plus the whole range
-  std::vector<long long> keys_u(1, range_key(0, 0)), keys_v(1, range_key(0, 0));
+  // host metadata: maximum orders (Mesh::max_expansion_orders, mesh.rs:626-630)
+  // and the Elem tree's
   for (int e = 0; e < ne; ++e) {
     dd->max_i = std::max<int>(dd->max_i, v->orders[2 * e]);
     dd->max_j = std::max<int>(dd->max_j, v->orders[2 * e + 1]);
-    unsigned char locs[kMaxDepth + 1];
-    int steps = 0;
-    for (int x = e; x >= 0 && v->parent[x] >= 0 && v->parent[x] < ne && steps <= kMaxDepth;
-         x = v->parent[x], ++steps)
-      locs[steps] = v->loc[x];
-    dd->nested_cap += 1 + 2LL * steps;
-    if (steps > kMaxDepth) continue;  // rejected by the device checks
-    // Elem e inside its ancestor t levels up: the steps locs[t-1] (topmost) .. locs[0]
-    for (int t = 1; t <= steps; ++t) {
-      int du = 0, iu = 0, dv = 0, iv = 0;
-      for (int k = t - 1; k >= 0; --k) {
-        const int l = locs[k];
-        if (l <= FEM2D_LOC_E) {
-          iu = 2 * iu + (l == FEM2D_LOC_SE || l == FEM2D_LOC_NE || l == FEM2D_LOC_E ? 1 : 0);
-          ++du;
-        }
-        if (l <= FEM2D_LOC_NE || l >= FEM2D_LOC_S) {
-          iv = 2 * iv + (l == FEM2D_LOC_NW || l == FEM2D_LOC_NE || l == FEM2D_LOC_N ? 1 : 0);
-          ++dv;
-        }
-      }
-      keys_u.push_back(range_key(du, iu));
-      keys_v.push_back(range_key(dv, iv));
-    }
   }
-  for (auto* k : {&keys_u, &keys_v}) {
-    std::sort(k->begin(), k->end());
-    k->erase(std::unique(k->begin(), k->end()), k->end());
-  }
-  dd->nr_u = int(std::min<size_t>(keys_u.size(), size_t(1) << 30));
-  dd->nr_v = int(std::min<size_t>(keys_v.size(), size_t(1) << 30));
-  // shallow Domains (every sub-range at most kDenseRangeDepth halvings deep):
-  // a direct map by dyadic id, one load instead of a search per lookup
-  std::vector<int> range_map;
-  if (int(keys_u.back() >> 32) <= kDenseRangeDepth && int(keys_v.back() >> 32) <= kDenseRangeDepth) {
-    const int su = (1 << (int(keys_u.back() >> 32) + 1)) - 1;
-    const int sv = (1 << (kDenseRangeDepth + 1)) - 1;
-    range_map.assign(size_t(su + sv), -1);
-    for (size_t k = 0; k < keys_u.size(); ++k)
-      range_map[size_t((1 << int(keys_u[k] >> 32)) - 1 + int(keys_u[k] & 0xffffffffLL))] = int(k);
-    for (size_t k = 0; k < keys_v.size(); ++k)
-      range_map[size_t(su + (1 << int(keys_v[k] >> 32)) - 1 + int(keys_v[k] & 0xffffffffLL))] = int(k);
-    dd->range_map_v = su;
-  }
-  keys_u.insert(keys_u.end(), keys_v.begin(), keys_v.end());
+  TreeMeta tm;
+  tree_meta(ne, v->parent, v->loc, tm);
+  dd->nested_cap = tm.nested_cap;
+  dd->nr_u = tm.nr_u;
+  dd->nr_v = tm.nr_v;
+  dd->range_map_v = tm.range_map_v;
   {
     // row groups: runs of DoFs whose basis specs lie on the same Elems (spec
     // lists are visited in Elem order, so the first Elem is the smaller one)
@@ -2978,25 +3044,13 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
       (rc = upload(v->bs_dir, size_t(nbs), &dd->bs_dir)) || (rc = upload(v->bs_i, size_t(nbs), &dd->bs_i)) ||
       (rc = upload(v->bs_j, size_t(nbs), &dd->bs_j)) || (rc = upload(v->bs_dof, size_t(nbs), &dd->bs_dof)) ||
       (rc = upload(v->dof_bs, size_t(2 * v->n_dofs), &dd->dof_bs)) ||
-      (rc = upload(keys_u.data(), keys_u.size(), &dd->ranges)) ||
-      (!range_map.empty() && (rc = upload(range_map.data(), range_map.size(), &dd->range_map))) ||
-      (rc = domain_prep(ctx, dd))) {
+      (rc = upload(tm.keys.data(), tm.keys.size(), &dd->ranges)) ||
+      (!tm.range_map.empty() && (rc = upload(tm.range_map.data(), tm.range_map.size(), &dd->range_map))) ||
+      (rc = domain_prep(ctx, dd)) || (rc = upload(tm.leaves.data(), tm.leaves.size(), &dd->leaves))) {
     fem2d_device_domain_free(dd);
     return rc;
   }
-  {
-    std::vector<char> has_child(size_t(ne), 0);
-    for (int e = 0; e < ne; ++e)
-      if (v->parent[e] >= 0 && v->parent[e] < ne) has_child[size_t(v->parent[e])] = 1;
-    std::vector<int> leaves;
-    for (int e = 0; e < ne; ++e)
-      if (!has_child[size_t(e)]) leaves.push_back(e);
-    dd->n_leaves = (long long)leaves.size();
-    if ((rc = upload(leaves.data(), leaves.size(), &dd->leaves))) {
-      fem2d_device_domain_free(dd);
-      return rc;
-    }
-  }
+  dd->n_leaves = (long long)tm.leaves.size();
   *out = dd;
   return FEM2D_OK;
 }
@@ -3004,6 +3058,11 @@ int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* v, fem2d
 void fem2d_device_domain_free(fem2d_device_domain* dd) {
   if (!dd) return;
   (void)hipSetDevice(dd->device);
+  if (dd->arena) {
+    (void)hipFree(dd->arena);
+    delete dd;
+    return;
+  }
   for (void* p : {(void*)dd->parent, (void*)dd->loc, (void*)dd->box, (void*)dd->mat, (void*)dd->bs_off,
                   (void*)dd->bs_dir, (void*)dd->bs_i, (void*)dd->bs_j, (void*)dd->bs_dof, (void*)dd->dof_bs,
                   (void*)dd->leaves, (void*)dd->ranges, (void*)dd->range_map, (void*)dd->geo, (void*)dd->depth,
@@ -3593,4 +3652,221 @@ int fem2d_galerkin_sample_gep_hcurl(fem2d_ctx* ctx, const fem2d_domain_view* dom
   return rc;
 }
 
+int fem2d_device_domain_info(const fem2d_device_domain* dd, int32_t* n_elems, int64_t* n_dofs, int64_t* n_bs) {
+  if (!dd) {
+    set_error("null device domain", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  if (n_elems) *n_elems = dd->n_elems;
+  if (n_dofs) *n_dofs = dd->n_dofs;
+  if (n_bs) *n_bs = dd->n_bs;
+  return FEM2D_OK;
+}
+
+int fem2d_device_domain_download_view(fem2d_ctx* ctx, const fem2d_device_domain* dd, int32_t* parent, uint8_t* loc,
+                                      double* element_box, double* materials, int64_t* bs_offsets, uint8_t* bs_dir,
+                                      uint8_t* bs_i, uint8_t* bs_j, int32_t* bs_dof, int32_t* dof_bs) {
+  if (!ctx || !dd) {
+    set_error("null argument", 0);
+    return FEM2D_ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(dd->device));
+  const size_t ne = size_t(dd->n_elems), nbs = size_t(dd->n_bs), nd = size_t(dd->n_dofs);
+  struct Part {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  } parts[] = {{parent, dd->parent, ne * sizeof(int32_t)},
+               {loc, dd->loc, ne},
+               {element_box, dd->box, 4 * ne * sizeof(double)},
+               {materials, dd->mat, 2 * ne * sizeof(double)},
+               {bs_offsets, dd->bs_off, (ne + 1) * sizeof(int64_t)},
+               {bs_dir, dd->bs_dir, nbs},
+               {bs_i, dd->bs_i, nbs},
+               {bs_j, dd->bs_j, nbs},
+               {bs_dof, dd->bs_dof, nbs * sizeof(int32_t)},
+               {dof_bs, dd->dof_bs, 2 * nd * sizeof(int32_t)}};
+  for (const Part& q : parts)
+    if (q.dst && q.bytes) HIP_TRY(hipMemcpyAsync(q.dst, q.src, q.bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return FEM2D_OK;
+}
+
 }  // extern "C"
+
+namespace {
+
+// Sub-allocations of one device allocation, each 256-byte aligned and at
+// least one element long
+struct ArenaPlan {
+  size_t bytes = 0;
+  template <class T>
+  size_t take(size_t count) {
+    const size_t off = bytes;
+    bytes += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+    return off;
+  }
+};
+
+}  // namespace
+
+// Domain::from_mesh on the device (the kernels of domain_build.hip). One
+// allocation holds the Domain and the build's work arrays; the flattened Mesh
+// and the Elem tree's metadata go up in one copy; the counts' totals come back
+// once to size the lists, and the status once at the end.
+int fem2d::capi::device_domain_from_flat(fem2d_ctx* ctx, const MeshFlat& m, int32_t cc, fem2d_device_domain** out) {
+  namespace B = fem2d::build;
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int ne = m.n_elems, ned = m.n_edges;
+  const size_t E = size_t(ne), G = size_t(ned);
+  const size_t cap = size_t(std::max<int64_t>(m.uv_specs, 1));
+  TreeMeta tm;
+  tree_meta(ne, m.parent.data(), m.loc.data(), tm);
+
+  // the host-written part first (one copy), then the device-written part
+  ArenaPlan plan;
+  const size_t o_status = plan.take<B::Status>(1);
+  const size_t o_parent = plan.take<int>(E), o_loc = plan.take<unsigned char>(E);
+  const size_t o_box = plan.take<double>(4 * E), o_mat = plan.take<double>(2 * E);
+  const size_t o_edges = plan.take<int4>(E), o_meta = plan.take<unsigned>(E);
+  const size_t o_side = plan.take<int2>(G), o_eparent = plan.take<int>(G);
+  const size_t o_child = plan.take<int2>(G), o_open = plan.take<unsigned char>(G);
+  const size_t o_ranges = plan.take<long long>(tm.keys.size());
+  const size_t o_rmap = plan.take<int>(tm.range_map.size());
+  const size_t o_leaves = plan.take<int>(tm.leaves.size());
+  const size_t host_bytes = plan.bytes;
+  const size_t o_bs_off = plan.take<long long>(E + 1), o_elem_base = plan.take<long long>(E + 1);
+  const size_t o_edge_base = plan.take<long long>(G + 1);
+  const size_t o_tiles = plan.take<long long>(size_t(B::scan_tiles(ne, ned)));
+  const size_t o_act = plan.take<int2>(G), o_npairs = plan.take<int>(G);
+  const size_t o_nelem = plan.take<int>(E), o_len = plan.take<int>(E);
+  const size_t o_dir = plan.take<unsigned char>(cap), o_bi = plan.take<unsigned char>(cap);
+  const size_t o_bj = plan.take<unsigned char>(cap), o_bdof = plan.take<int>(cap);
+  const size_t o_dof_bs = plan.take<int>(2 * cap);
+  const size_t o_geo = plan.take<ElemGeo>(E), o_depth = plan.take<int>(E), o_ndesc = plan.take<int>(E);
+  const size_t o_dyad = plan.take<int4>(E);
+  const size_t o_bs_elem = plan.take<int>(cap), o_bs_pack = plan.take<unsigned>(cap);
+  const size_t o_spec_pos = plan.take<unsigned>(cap), o_dof_sig = plan.take<int4>(cap);
+
+  std::vector<unsigned char> stage(host_bytes, 0);
+  auto put = [&](size_t off, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(stage.data() + off, src, bytes);
+  };
+  put(o_parent, m.parent.data(), E * sizeof(int));
+  put(o_loc, m.loc.data(), E);
+  put(o_box, m.box.data(), 4 * E * sizeof(double));
+  put(o_mat, m.mat.data(), 2 * E * sizeof(double));
+  put(o_edges, m.edges.data(), 4 * E * sizeof(int));
+  put(o_meta, m.meta.data(), E * sizeof(unsigned));
+  put(o_side, m.side.data(), 2 * G * sizeof(int));
+  put(o_eparent, m.edge_parent.data(), G * sizeof(int));
+  put(o_child, m.child.data(), 2 * G * sizeof(int));
+  put(o_open, m.open_base.data(), G);
+  put(o_ranges, tm.keys.data(), tm.keys.size() * sizeof(long long));
+  put(o_rmap, tm.range_map.data(), tm.range_map.size() * sizeof(int));
+  put(o_leaves, tm.leaves.data(), tm.leaves.size() * sizeof(int));
+
+  auto* dd = new fem2d_device_domain();
+  struct Guard {
+    fem2d_device_domain*& p;
+    ~Guard() { fem2d_device_domain_free(p); }
+  } guard{dd};
+  dd->serial = ++g_domain_serial;
+  dd->device = ctx->device;
+  dd->cc = cc;
+  dd->n_elems = ne;
+  dd->max_i = m.max_i;
+  dd->max_j = m.max_j;
+  dd->nested_cap = tm.nested_cap;
+  dd->nr_u = tm.nr_u;
+  dd->nr_v = tm.nr_v;
+  dd->range_map_v = tm.range_map_v;
+  dd->n_leaves = (long long)tm.leaves.size();
+  HIP_TRY(hipMalloc(&dd->arena, plan.bytes));
+  unsigned char* base = static_cast<unsigned char*>(dd->arena);
+  auto at = [&](size_t off) { return static_cast<void*>(base + off); };
+  dd->parent = static_cast<int*>(at(o_parent));
+  dd->loc = static_cast<unsigned char*>(at(o_loc));
+  dd->box = static_cast<double*>(at(o_box));
+  dd->mat = static_cast<double*>(at(o_mat));
+  dd->ranges = static_cast<long long*>(at(o_ranges));
+  dd->range_map = tm.range_map.empty() ? nullptr : static_cast<int*>(at(o_rmap));
+  dd->leaves = static_cast<int*>(at(o_leaves));
+  dd->bs_off = static_cast<long long*>(at(o_bs_off));
+  dd->bs_dir = static_cast<unsigned char*>(at(o_dir));
+  dd->bs_i = static_cast<unsigned char*>(at(o_bi));
+  dd->bs_j = static_cast<unsigned char*>(at(o_bj));
+  dd->bs_dof = static_cast<int*>(at(o_bdof));
+  dd->dof_bs = static_cast<int*>(at(o_dof_bs));
+  dd->geo = static_cast<ElemGeo*>(at(o_geo));
+  dd->depth = static_cast<int*>(at(o_depth));
+  dd->n_desc = static_cast<int*>(at(o_ndesc));
+  dd->dyad = static_cast<int4*>(at(o_dyad));
+  dd->bs_elem = static_cast<int*>(at(o_bs_elem));
+  dd->bs_pack = static_cast<unsigned*>(at(o_bs_pack));
+  dd->spec_pos = static_cast<unsigned*>(at(o_spec_pos));
+  dd->dof_sig = static_cast<int4*>(at(o_dof_sig));
+  auto* status = static_cast<B::Status*>(at(o_status));
+
+  B::Args a;
+  a.n_elems = ne;
+  a.n_edges = ned;
+  a.spec_cap = (long long)cap;
+  a.elem_edges = static_cast<const int4*>(at(o_edges));
+  a.elem_meta = static_cast<const unsigned*>(at(o_meta));
+  a.side = static_cast<const int2*>(at(o_side));
+  a.edge_parent = static_cast<const int*>(at(o_eparent));
+  a.child = static_cast<const int2*>(at(o_child));
+  a.open_base = static_cast<const unsigned char*>(at(o_open));
+  a.act = static_cast<int2*>(at(o_act));
+  a.n_pairs = static_cast<int*>(at(o_npairs));
+  a.n_elem = static_cast<int*>(at(o_nelem));
+  a.list_len = static_cast<int*>(at(o_len));
+  a.elem_base = static_cast<long long*>(at(o_elem_base));
+  a.edge_base = static_cast<long long*>(at(o_edge_base));
+  a.tile_sums = static_cast<long long*>(at(o_tiles));
+  a.status = status;
+  a.bs_off = dd->bs_off;
+  a.bs_dir = dd->bs_dir;
+  a.bs_i = dd->bs_i;
+  a.bs_j = dd->bs_j;
+  a.bs_dof = dd->bs_dof;
+  a.dof_bs = dd->dof_bs;
+
+  hipStream_t st = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(dd->arena, stage.data(), host_bytes, hipMemcpyHostToDevice, st));
+  B::Status h{};
+  if (ne > 0) {
+    HIP_TRY_AT(B::launch_counts(a, st), "launching the Domain counts");
+    HIP_TRY(hipMemcpyAsync(&h, status, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (h.flags & (B::kFlagNoPair | B::kFlagInconsistent | B::kFlagTooDeep)) return kActivationFailed;
+    if (h.elem_dofs + h.edge_dofs > 0x7fffffffLL) {
+      set_error("too many DoFs for 32-bit ids", 0);
+      return FEM2D_ERR_PANIC;
+    }
+    if (h.specs > (long long)cap || h.elem_dofs + h.edge_dofs > h.specs) {
+      set_error("device Domain build: list sizes beyond their bound", 0);
+      return FEM2D_ERR_HIP;
+    }
+    HIP_TRY_AT(B::launch_lists(a, st), "launching the Domain lists");
+  } else {
+    HIP_TRY(hipMemsetAsync(dd->bs_off, 0, sizeof(long long), st));
+  }
+  dd->n_dofs = h.elem_dofs + h.edge_dofs;
+  dd->n_bs = h.specs;
+  if (int rc = domain_prep_launch(ctx, dd, &status->err)) return rc;
+  HIP_TRY_AT(B::launch_group_count(dd->dof_bs, dd->bs_elem, dd->n_dofs, dd->n_bs, status, st), "launching the group count");
+  HIP_TRY(hipMemcpyAsync(&h, status, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (h.flags & B::kFlagCapacity) {
+    set_error("device Domain build: a list entry beyond the arrays' capacity", 0);
+    return FEM2D_ERR_HIP;
+  }
+  if (h.err) return err_to_status(h.err);
+  dd->n_groups = (long long)h.groups;
+  *out = dd;
+  dd = nullptr;  // owned by the caller now
+  return FEM2D_OK;
+}

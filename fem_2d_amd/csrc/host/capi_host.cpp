@@ -1,5 +1,6 @@
 // C ABI of the host input builder (Mesh, Domain), error reporting and
 // Gauss-Legendre nodes. Declarations and reference citations: include/fem2d.h.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -113,6 +114,94 @@ static HRef make_href(int32_t kind, int32_t ext) {
     case FEM2D_HREF_V: return HRef::v(ext);
     default: throw MeshPanic("unknown h-refinement kind");
   }
+}
+
+// The Mesh as the flat arrays of the device Domain builder (every index checked
+// here, so the builder's kernels need not). The Edge side arrays hold the Elem
+// Edge::set_activation would pick on each side (edge.rs:203-214: the last entry
+// of the side's h-level map).
+static void flatten_mesh(const Mesh& mesh, int threads, capi::MeshFlat& f) {
+  const size_t ne = mesh.elems.size(), ned = mesh.edges.size();
+  if (ne > size_t(INT32_MAX) || ned > size_t(INT32_MAX)) throw MeshPanic("too many Elems or Edges for 32-bit ids");
+  f.n_elems = int32_t(ne);
+  f.n_edges = int32_t(ned);
+  f.parent.resize(ne);
+  f.loc.resize(ne);
+  f.box.resize(4 * ne);
+  f.mat.resize(2 * ne);
+  f.edges.resize(4 * ne);
+  f.meta.resize(ne);
+  f.side.resize(2 * ned);
+  f.edge_parent.resize(ned);
+  f.child.resize(2 * ned);
+  f.open_base.resize(ned);
+  std::atomic<int64_t> uv{0}, all{0};
+  std::atomic<int> max_i{0}, max_j{0};
+  parallel_ranges(ne, threads, 1024, [&](size_t b, size_t e) {
+    int64_t uv_part = 0, all_part = 0;
+    int mi = 0, mj = 0;
+    for (size_t x = b; x < e; ++x) {
+      const Elem& el = mesh.elems[x];
+      if (el.element < 0 || size_t(el.element) >= mesh.elements.size())
+        throw MeshPanic("Elem " + std::to_string(x) + " refers to a non-existent Element");
+      const Element& elt = mesh.elements[size_t(el.element)];
+      const int p = el.parent_id();
+      if (p < -1 || p >= int(ne)) throw MeshPanic("Elem " + std::to_string(x) + " has a non-existent parent");
+      f.parent[x] = p;
+      f.loc[x] = el.ancestors.empty() ? 0 : uint8_t(el.ancestors.back().second);
+      f.box[4 * x + 0] = elt.points[0].x;
+      f.box[4 * x + 1] = elt.points[0].y;
+      f.box[4 * x + 2] = elt.points[3].x;
+      f.box[4 * x + 3] = elt.points[3].y;
+      f.mat[2 * x + 0] = elt.materials.eps_re;
+      f.mat[2 * x + 1] = elt.materials.mu_re;
+      for (int k = 0; k < 4; ++k) {
+        const int g = el.edges[size_t(k)];
+        if (g < 0 || g >= int(ned)) throw MeshPanic("Elem " + std::to_string(x) + " refers to a non-existent Edge");
+        for (int q = 0; q < k; ++q)
+          if (el.edges[size_t(q)] == g)
+            throw MeshPanic("Elem " + std::to_string(x) + " lists Edge " + std::to_string(g) + " twice");
+        f.edges[4 * x + size_t(k)] = g;
+      }
+      const int ni = el.poly_orders.ni, nj = el.poly_orders.nj;
+      if (ni < 1 || nj < 1) throw MeshPanic("Elem " + std::to_string(x) + " has an expansion order below 1");
+      f.meta[x] = uint32_t(ni) | uint32_t(nj) << 8 | uint32_t(el.has_children() ? 0 : 1) << 16;
+      uv_part += int64_t(ni) * (nj + 1) + int64_t(ni + 1) * nj;
+      all_part += int64_t(ni) * (nj + 1) + int64_t(ni + 1) * nj + int64_t(ni + 1) * (nj + 1);
+      mi = std::max(mi, ni);
+      mj = std::max(mj, nj);
+    }
+    uv += uv_part;
+    all += all_part;
+    for (int cur = max_i.load(); mi > cur && !max_i.compare_exchange_weak(cur, mi);) {
+    }
+    for (int cur = max_j.load(); mj > cur && !max_j.compare_exchange_weak(cur, mj);) {
+    }
+  });
+  parallel_ranges(ned, threads, 2048, [&](size_t b, size_t e) {
+    for (size_t g = b; g < e; ++g) {
+      const Edge& ed = mesh.edges[g];
+      for (int s = 0; s < 2; ++s) {
+        const int x = ed.elems[size_t(s)].empty() ? -1 : ed.elems[size_t(s)].rbegin()->second;
+        if (x < -1 || x >= int(ne)) throw MeshPanic("Edge " + std::to_string(g) + " refers to a non-existent Elem");
+        f.side[2 * g + size_t(s)] = x;
+      }
+      if (ed.parent < -1 || ed.parent >= int(ned))
+        throw MeshPanic("Edge " + std::to_string(g) + " has a non-existent parent");
+      f.edge_parent[g] = ed.parent;
+      for (int s = 0; s < 2; ++s) {
+        const int c = ed.has_children() ? ed.children[size_t(s)] : -1;
+        if (ed.has_children() && (c < 0 || c >= int(ned)))
+          throw MeshPanic("Edge " + std::to_string(g) + " has a non-existent child");
+        f.child[2 * g + size_t(s)] = c;
+      }
+      f.open_base[g] = uint8_t(ed.parent < 0 && !ed.boundary);
+    }
+  });
+  f.uv_specs = uv.load();
+  f.all_specs = all.load();
+  f.max_i = max_i.load();
+  f.max_j = max_j.load();
 }
 
 static int write_ids(const std::vector<int>& v, int32_t* out, int64_t cap, int64_t* n) {
@@ -514,6 +603,32 @@ int fem2d_domain_dof_info(const fem2d_domain* d, int64_t dof, int32_t* n_bs, int
     if (elem_idx4) elem_idx4[k] = f.basis_specs[k].elem_idx;
   }
   return FEM2D_OK;
+}
+
+int fem2d_device_domain_from_mesh(fem2d_ctx* ctx, const fem2d_mesh* m, int32_t cc, fem2d_device_domain** out) {
+  FEM2D_REQUIRE(ctx && m && out, "null argument");
+  *out = nullptr;
+  FEM2D_REQUIRE(cc >= 0 && cc <= 2, "unknown continuity condition");
+  FEM2D_GUARD({
+    const Mesh& mesh = m->m;
+    // Domain::from_mesh sets the edge activation before it generates the
+    // first BasisSpec, so an activation panic comes before this one
+    if (cc != FEM2D_CC_HCURL && !mesh.elems.empty()) {
+      mesh.clone(1).set_edge_activation(1);
+      throw MeshPanic("not implemented: only H(Curl) BasisSpecs are supported");
+    }
+    capi::MeshFlat f;
+    flatten_mesh(mesh, default_threads(), f);
+    int rc = capi::device_domain_from_flat(ctx, f, cc, out);
+    if (rc == capi::kActivationFailed) {
+      // the device found an Edge without an active pair where the reference
+      // panics; the sequential activation names the Edge it meets first
+      mesh.clone(1).set_edge_activation(1);
+      set_error("edge activation failed on the device, but not in the sequential activation", 0);
+      rc = FEM2D_ERR_PANIC;
+    }
+    if (rc) return rc;
+  })
 }
 
 // ---------------------------------------------------------------------------

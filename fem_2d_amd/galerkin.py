@@ -13,7 +13,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native as N
-from .domain import Domain
+from .domain import ContinuityCondition, Domain
 from .errors import check
 
 MIN_GLQ_ORDER = 4  # galerkin.rs:13
@@ -250,15 +250,44 @@ def galerkin_sample_gep_hcurl(domain: Domain, glq_grid_dim: Optional[Sequence[in
 
 
 class DeviceDomain:
-    """A Domain uploaded to HBM once, for repeated device-resident assembly."""
+    """A Domain held in HBM for repeated device-resident assembly: uploaded from a host Domain,
+    or built on the GPU from a Mesh (:meth:`from_mesh`)."""
 
-    def __init__(self, domain: Domain, ctx: Optional[Context] = None):
+    def __init__(self, domain: Optional[Domain], ctx: Optional[Context] = None, _handle=None):
         self.ctx = ctx or default_context()
         self._lib = N.load()
+        if _handle is None:
+            h = C.c_void_p()
+            check(self._lib.fem2d_device_domain_upload(self.ctx._h, C.byref(domain.view()), C.byref(h)))
+            _handle = h
+        self._h = _handle
+        n_elems, n_dofs, n_bs = C.c_int32(), C.c_int64(), C.c_int64()
+        check(self._lib.fem2d_device_domain_info(self._h, C.byref(n_elems), C.byref(n_dofs), C.byref(n_bs)))
+        self.n_elems, self.n_dofs, self.n_bs = n_elems.value, n_dofs.value, n_bs.value
+
+    @classmethod
+    def from_mesh(cls, mesh, cc: ContinuityCondition = ContinuityCondition.HCurl,
+                  ctx: Optional[Context] = None) -> "DeviceDomain":
+        """Domain::from_mesh (domain.rs:69-159) built on the GPU straight into HBM: the same device
+        Domain as uploading ``Domain.from_mesh(mesh, cc)``, bit for bit."""
+        ctx = ctx or default_context()
+        lib = N.load()
         h = C.c_void_p()
-        check(self._lib.fem2d_device_domain_upload(self.ctx._h, C.byref(domain.view()), C.byref(h)))
-        self._h = h
-        self.n_dofs = domain.num_dofs
+        check(lib.fem2d_device_domain_from_mesh(ctx._h, mesh._h, int(cc), C.byref(h)))
+        return cls(None, ctx, _handle=h)
+
+    def download_view(self) -> dict:
+        """The device Domain's view arrays (fem2d_domain_view without orders) as numpy arrays."""
+        ne, nbs, nd = self.n_elems, self.n_bs, self.n_dofs
+        out = {"parent": np.empty(ne, np.int32), "loc": np.empty(ne, np.uint8),
+               "element_box": np.empty(4 * ne, np.float64), "materials": np.empty(2 * ne, np.float64),
+               "bs_offsets": np.empty(ne + 1, np.int64), "bs_dir": np.empty(nbs, np.uint8),
+               "bs_i": np.empty(nbs, np.uint8), "bs_j": np.empty(nbs, np.uint8),
+               "bs_dof": np.empty(nbs, np.int32), "dof_bs": np.empty(2 * nd, np.int32)}
+        ptr = [out[k].ctypes.data_as(C.c_void_p) for k in ("parent", "loc", "element_box", "materials", "bs_offsets",
+                                                            "bs_dir", "bs_i", "bs_j", "bs_dof", "dof_bs")]
+        check(self._lib.fem2d_device_domain_download_view(self.ctx._h, self._h, *ptr))
+        return out
 
     def __del__(self):
         try:

@@ -181,6 +181,8 @@ void fem2d_gep_csr_free(fem2d_gep_csr* gep);
  * ------------------------------------------------------------------------- */
 
 typedef struct fem2d_device_domain fem2d_device_domain;
+/* Mesh (mesh.rs) of the host input builder below */
+typedef struct fem2d_mesh fem2d_mesh;
 
 /* Device pointers owned by the context; valid until the next assembly on that
  * context or its destruction. fem2d_assemble_device stamps the handle with the
@@ -205,7 +207,23 @@ typedef struct fem2d_gep_device {
 } fem2d_gep_device;
 
 int fem2d_device_domain_upload(fem2d_ctx* ctx, const fem2d_domain_view* domain, fem2d_device_domain** out);
+/* Domain::from_mesh (domain.rs:69-159) built on the GPU, straight into a device
+ * Domain: the Mesh is flattened on the host and copied up once, then the edge
+ * activation (Mesh::set_edge_activation, mesh.rs:1180-1215), the DoF numbering
+ * and the per-Elem BasisSpec lists are computed by kernels. The result is the
+ * device Domain fem2d_device_domain_upload makes of the view of
+ * fem2d_domain_from_mesh(m, cc): the same arrays, bit for bit. Panics of the
+ * reference's construction (no active Edge pair, a non-H(curl) condition on a
+ * non-empty Mesh) return FEM2D_ERR_PANIC with its message. */
+int fem2d_device_domain_from_mesh(fem2d_ctx* ctx, const fem2d_mesh* m, int32_t cc, fem2d_device_domain** out);
 void fem2d_device_domain_free(fem2d_device_domain* dd);
+/* sizes of a device Domain (any pointer may be NULL) */
+int fem2d_device_domain_info(const fem2d_device_domain* dd, int32_t* n_elems, int64_t* n_dofs, int64_t* n_bs);
+/* copies a device Domain's view arrays to host arrays sized as fem2d_domain_view
+ * describes them (orders are not kept on the device); NULL arrays are skipped */
+int fem2d_device_domain_download_view(fem2d_ctx* ctx, const fem2d_device_domain* dd, int32_t* parent, uint8_t* loc,
+                                      double* element_box, double* materials, int64_t* bs_offsets, uint8_t* bs_dir,
+                                      uint8_t* bs_i, uint8_t* bs_j, int32_t* bs_dof, int32_t* dof_bs);
 int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int basis, int a_integral, int b_integral,
                           int glq_u, int glq_v, int precision, fem2d_gep_device* out);
 int fem2d_gep_device_download(fem2d_ctx* ctx, const fem2d_gep_device* src, fem2d_gep_csr* out);
@@ -389,7 +407,6 @@ int fem2d_last_error_detail(void);
  * Host input builder: Mesh (mesh.rs) and Domain (domain.rs)
  * ------------------------------------------------------------------------- */
 
-typedef struct fem2d_mesh fem2d_mesh;
 typedef struct fem2d_domain fem2d_domain;
 
 typedef struct fem2d_elem_info {
