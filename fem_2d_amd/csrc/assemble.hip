@@ -2233,6 +2233,8 @@ struct fem2d_ctx {
   DevBuf field_scratch;  // grid points and status word of the field evaluation
   long long out_capacity = 0;
   long long* h_status = nullptr;  // mapped pinned status block: nnz, err, n_large, n_groups
+  void* h_stage = nullptr;        // pinned upload staging of the device Domain builder (grow-only)
+  size_t h_stage_bytes = 0;
   long long* d_status = nullptr;  // its device address
   GlqArgs glq{};
   int glq_n[2] = {0, 0};  // point counts glq holds
@@ -2498,8 +2500,9 @@ DevView dev_view(const fem2d_device_domain* dd) {
 }
 
 // k_domain_prep and k_dof_sig into the device Domain's derived arrays (already
-// allocated), with the error word at d_err; no synchronisation
-int domain_prep_launch(fem2d_ctx* ctx, fem2d_device_domain* dd, int* d_err) {
+// allocated), with the error word at d_err; no synchronisation. With zeroed the
+// caller has already cleared depth, n_desc and the error word.
+int domain_prep_launch(fem2d_ctx* ctx, fem2d_device_domain* dd, int* d_err, bool zeroed) {
   const int ne = dd->n_elems;
   const size_t E1 = size_t(std::max(ne, 1));
   hipStream_t st = ctx->stream;
@@ -2513,9 +2516,11 @@ int domain_prep_launch(fem2d_ctx* ctx, fem2d_device_domain* dd, int* d_err) {
   w.spec_pos = dd->spec_pos;
   w.dof_sig = dd->dof_sig;
   w.err = d_err;
-  HIP_TRY(hipMemsetAsync(dd->depth, 0, E1 * sizeof(int), st));
-  HIP_TRY(hipMemsetAsync(dd->n_desc, 0, E1 * sizeof(int), st));
-  HIP_TRY(hipMemsetAsync(w.err, 0, sizeof(int), st));
+  if (!zeroed) {
+    HIP_TRY(hipMemsetAsync(dd->depth, 0, E1 * sizeof(int), st));
+    HIP_TRY(hipMemsetAsync(dd->n_desc, 0, E1 * sizeof(int), st));
+    HIP_TRY(hipMemsetAsync(w.err, 0, sizeof(int), st));
+  }
   const unsigned prep_blocks = grid_for(ne, kBlock);
   const unsigned spec_blocks =
       unsigned(std::min<long long>(std::max<long long>(1, (ne + kWavesPerBlock - 1) / kWavesPerBlock), 1 << 16));
@@ -2545,7 +2550,7 @@ int domain_prep(fem2d_ctx* ctx, fem2d_device_domain* dd) {
     int* p;
     ~FreeErr() { (void)hipFree(p); }
   } free_err{d_err};
-  if (int rc = domain_prep_launch(ctx, dd, d_err)) return rc;
+  if (int rc = domain_prep_launch(ctx, dd, d_err, false)) return rc;
   int err = 0;
   HIP_TRY(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2901,6 +2906,7 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->h_status) (void)hipHostFree(ctx->h_status);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   fem2d::capi::release_solver(ctx->solver);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -3735,6 +3741,8 @@ int fem2d::capi::device_domain_from_flat(fem2d_ctx* ctx, const MeshFlat& m, int3
   const size_t o_ranges = plan.take<long long>(tm.keys.size());
   const size_t o_rmap = plan.take<int>(tm.range_map.size());
   const size_t o_leaves = plan.take<int>(tm.leaves.size());
+  // the preparation kernels accumulate into these two: they go up as zeros
+  const size_t o_depth = plan.take<int>(E), o_ndesc = plan.take<int>(E);
   const size_t host_bytes = plan.bytes;
   const size_t o_bs_off = plan.take<long long>(E + 1), o_elem_base = plan.take<long long>(E + 1);
   const size_t o_edge_base = plan.take<long long>(G + 1);
@@ -3744,14 +3752,25 @@ int fem2d::capi::device_domain_from_flat(fem2d_ctx* ctx, const MeshFlat& m, int3
   const size_t o_dir = plan.take<unsigned char>(cap), o_bi = plan.take<unsigned char>(cap);
   const size_t o_bj = plan.take<unsigned char>(cap), o_bdof = plan.take<int>(cap);
   const size_t o_dof_bs = plan.take<int>(2 * cap);
-  const size_t o_geo = plan.take<ElemGeo>(E), o_depth = plan.take<int>(E), o_ndesc = plan.take<int>(E);
+  const size_t o_geo = plan.take<ElemGeo>(E);
   const size_t o_dyad = plan.take<int4>(E);
   const size_t o_bs_elem = plan.take<int>(cap), o_bs_pack = plan.take<unsigned>(cap);
   const size_t o_spec_pos = plan.take<unsigned>(cap), o_dof_sig = plan.take<int4>(cap);
 
-  std::vector<unsigned char> stage(host_bytes, 0);
+  // the context's pinned staging buffer (grow-only; free again once this call
+  // has synchronised), so the upload is one DMA transfer
+  if (ctx->h_stage_bytes < host_bytes) {
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+    ctx->h_stage = nullptr;
+    ctx->h_stage_bytes = 0;
+    HIP_TRY(hipHostMalloc(&ctx->h_stage, host_bytes + host_bytes / 4, hipHostMallocDefault));
+    ctx->h_stage_bytes = host_bytes + host_bytes / 4;
+  }
+  unsigned char* stage = static_cast<unsigned char*>(ctx->h_stage);
+  std::memset(stage + o_status, 0, sizeof(B::Status));
+  std::memset(stage + o_depth, 0, host_bytes - o_depth);  // depth and n_desc
   auto put = [&](size_t off, const void* src, size_t bytes) {
-    if (bytes) std::memcpy(stage.data() + off, src, bytes);
+    if (bytes) std::memcpy(stage + off, src, bytes);
   };
   put(o_parent, m.parent.data(), E * sizeof(int));
   put(o_loc, m.loc.data(), E);
@@ -3835,7 +3854,7 @@ int fem2d::capi::device_domain_from_flat(fem2d_ctx* ctx, const MeshFlat& m, int3
   a.dof_bs = dd->dof_bs;
 
   hipStream_t st = ctx->stream;
-  HIP_TRY(hipMemcpyAsync(dd->arena, stage.data(), host_bytes, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(dd->arena, stage, host_bytes, hipMemcpyHostToDevice, st));
   B::Status h{};
   if (ne > 0) {
     HIP_TRY_AT(B::launch_counts(a, st), "launching the Domain counts");
@@ -3856,7 +3875,7 @@ int fem2d::capi::device_domain_from_flat(fem2d_ctx* ctx, const MeshFlat& m, int3
   }
   dd->n_dofs = h.elem_dofs + h.edge_dofs;
   dd->n_bs = h.specs;
-  if (int rc = domain_prep_launch(ctx, dd, &status->err)) return rc;
+  if (int rc = domain_prep_launch(ctx, dd, &status->err, true)) return rc;
   HIP_TRY_AT(B::launch_group_count(dd->dof_bs, dd->bs_elem, dd->n_dofs, dd->n_bs, status, st), "launching the group count");
   HIP_TRY(hipMemcpyAsync(&h, status, sizeof(h), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));

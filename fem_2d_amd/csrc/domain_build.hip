@@ -20,6 +20,8 @@
 // per Elem, three prefix sums, and one pass writing every Elem's list.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "domain_build.hpp"
 
 namespace fem2d {
@@ -31,6 +33,7 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kItems = 8;                    // per thread in the prefix sums
 constexpr int kTile = kBlock * kItems;       // items per prefix-sum tile
+constexpr long long kGroupCountBlocks = 512;  // blocks of the row-group count (one atomic each)
 
 __device__ __forceinline__ bool has_pair(int2 s) { return s.x >= 0 && s.y >= 0; }
 
@@ -354,20 +357,29 @@ __device__ __forceinline__ int2 dof_elems(const int* dof_bs, const int* bs_elem,
   return make_int2(s0 >= 0 && s0 < n_bs ? bs_elem[s0] : -1, s1 >= 0 && s1 < n_bs ? bs_elem[s1] : -1);
 }
 
+// grid-stride over the DoFs; one atomic per block
 __global__ void __launch_bounds__(kBlock) k_group_count(const int* dof_bs, const int* bs_elem, long long n_dofs,
                                                         long long n_bs, Status* status) {
-  const long long d = (long long)blockIdx.x * kBlock + threadIdx.x;
-  bool starts = false;
-  if (d < n_dofs) {
+  __shared__ int wave_sum[kWavesPerBlock];
+  int count = 0;
+  for (long long d = (long long)blockIdx.x * kBlock + threadIdx.x; d < n_dofs; d += (long long)gridDim.x * kBlock) {
     if (d == 0) {
-      starts = true;
+      ++count;
     } else {
       const int2 c = dof_elems(dof_bs, bs_elem, n_bs, d), p = dof_elems(dof_bs, bs_elem, n_bs, d - 1);
-      starts = c.x != p.x || c.y != p.y;
+      count += c.x != p.x || c.y != p.y ? 1 : 0;
     }
   }
-  const unsigned long long mask = __ballot(starts);
-  if (int(threadIdx.x) % kWave == 0 && mask) atomicAdd(&status->groups, (unsigned long long)__popcll(mask));
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) count += __shfl_xor(count, off, kWave);
+  if (int(threadIdx.x) % kWave == 0) wave_sum[threadIdx.x / kWave] = count;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) total += wave_sum[i];
+    if (total) atomicAdd(&status->groups, (unsigned long long)total);
+  }
 }
 
 unsigned grid(long long n, long long per_block) { return unsigned((n + per_block - 1) / per_block); }
@@ -396,7 +408,9 @@ hipError_t launch_lists(const Args& a, hipStream_t st) {
 
 hipError_t launch_group_count(const int* dof_bs, const int* bs_elem, long long n_dofs, long long n_bs, Status* status,
                               hipStream_t st) {
-  if (n_dofs > 0) k_group_count<<<grid(n_dofs, kBlock), kBlock, 0, st>>>(dof_bs, bs_elem, n_dofs, n_bs, status);
+  if (n_dofs > 0)
+    k_group_count<<<unsigned(std::min<long long>(grid(n_dofs, kBlock), kGroupCountBlocks)), kBlock, 0, st>>>(
+        dof_bs, bs_elem, n_dofs, n_bs, status);
   return hipGetLastError();
 }
 

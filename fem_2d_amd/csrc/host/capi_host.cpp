@@ -119,7 +119,7 @@ static HRef make_href(int32_t kind, int32_t ext) {
 // The Mesh as the flat arrays of the device Domain builder (every index checked
 // here, so the builder's kernels need not). The Edge side arrays hold the Elem
 // Edge::set_activation would pick on each side (edge.rs:203-214: the last entry
-// of the side's h-level map).
+// of the side's h-level map, Edge::top).
 static void flatten_mesh(const Mesh& mesh, int threads, capi::MeshFlat& f) {
   const size_t ne = mesh.elems.size(), ned = mesh.edges.size();
   if (ne > size_t(INT32_MAX) || ned > size_t(INT32_MAX)) throw MeshPanic("too many Elems or Edges for 32-bit ids");
@@ -137,7 +137,8 @@ static void flatten_mesh(const Mesh& mesh, int threads, capi::MeshFlat& f) {
   f.open_base.resize(ned);
   std::atomic<int64_t> uv{0}, all{0};
   std::atomic<int> max_i{0}, max_j{0};
-  parallel_ranges(ne, threads, 1024, [&](size_t b, size_t e) {
+  // Elems [0, ne) and Edges [ne, ne + ned) in one pass of contiguous ranges
+  auto elem_range = [&](size_t b, size_t e) {
     int64_t uv_part = 0, all_part = 0;
     int mi = 0, mj = 0;
     for (size_t x = b; x < e; ++x) {
@@ -177,12 +178,12 @@ static void flatten_mesh(const Mesh& mesh, int threads, capi::MeshFlat& f) {
     }
     for (int cur = max_j.load(); mj > cur && !max_j.compare_exchange_weak(cur, mj);) {
     }
-  });
-  parallel_ranges(ned, threads, 2048, [&](size_t b, size_t e) {
+  };
+  auto edge_range = [&](size_t b, size_t e) {
     for (size_t g = b; g < e; ++g) {
       const Edge& ed = mesh.edges[g];
       for (int s = 0; s < 2; ++s) {
-        const int x = ed.elems[size_t(s)].empty() ? -1 : ed.elems[size_t(s)].rbegin()->second;
+        const int x = ed.top[size_t(s)];
         if (x < -1 || x >= int(ne)) throw MeshPanic("Edge " + std::to_string(g) + " refers to a non-existent Elem");
         f.side[2 * g + size_t(s)] = x;
       }
@@ -197,6 +198,10 @@ static void flatten_mesh(const Mesh& mesh, int threads, capi::MeshFlat& f) {
       }
       f.open_base[g] = uint8_t(ed.parent < 0 && !ed.boundary);
     }
+  };
+  parallel_ranges(ne + ned, threads, 4096, [&](size_t b, size_t e) {
+    if (b < ne) elem_range(b, std::min(e, ne));
+    if (e > ne) edge_range(std::max(b, ne) - ne, e - ne);
   });
   f.uv_specs = uv.load();
   f.all_specs = all.load();

@@ -248,12 +248,16 @@ void Edge::connect_elem(const Elem& elem) {
             "; cannot connect to Elem " + std::to_string(elem.id));
   } else {
     elems[side][address] = elem.id;
+    if (top[side] < 0 || address > top_key[side]) {
+      top[side] = elem.id;
+      top_key[side] = address;
+    }
   }
 }
 
 bool Edge::set_activation() {
-  if (!elems[0].empty() && !elems[1].empty()) {
-    active = {elems[0].rbegin()->second, elems[1].rbegin()->second};
+  if (top[0] >= 0 && top[1] >= 0) {
+    active = top;
     has_active = true;
     return true;
   }

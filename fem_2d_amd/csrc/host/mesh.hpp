@@ -194,6 +194,10 @@ struct Edge {
   std::array<int, 2> children{{-1, -1}};
   int parent = -1;
   std::array<std::map<uint16_t, int>, 2> elems;  // side -> (h-level key -> elem id)
+  // per side, the elem of the highest h-level key (the map's last entry), -1
+  // for none; kept by connect_elem, the maps' only writer
+  std::array<int, 2> top{{-1, -1}};
+  std::array<uint16_t, 2> top_key{{0, 0}};
   bool has_active = false;
   std::array<int, 2> active{{-1, -1}};
   int child_node = -1;
