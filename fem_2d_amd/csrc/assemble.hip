@@ -2226,6 +2226,10 @@ struct fem2d_device_domain {
 struct fem2d_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the block-path fill's branch of a recorded assembly (beside the streaming
+  // fill), forked from and joined back into `stream` by the two events
+  hipStream_t side_stream = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<DevBuf> bufs;  // grow-only work buffers, by slot
   DevBuf out_col, out_a, out_b;
@@ -2867,7 +2871,9 @@ int fem2d_ctx_create(int device, fem2d_ctx** out) {
   auto* ctx = new fem2d_ctx();
   ctx->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+  for (hipEvent_t* e : {&ctx->fork_ev, &ctx->join_ev}) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_status), 64, hipHostMallocMapped));
   HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->d_status), ctx->h_status, 0));
   hipDeviceProp_t prop;
@@ -2905,9 +2911,12 @@ void fem2d_ctx_destroy(fem2d_ctx* ctx) {
     if (g) (void)hipGraphExecDestroy(g);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ctx->fork_ev, ctx->join_ev})
+    if (e) (void)hipEventDestroy(e);
   if (ctx->h_status) (void)hipHostFree(ctx->h_status);
   if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   fem2d::capi::release_solver(ctx->solver);
+  if (ctx->side_stream) (void)hipStreamDestroy(ctx->side_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -3156,8 +3165,18 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
         ctx->graph_key = GraphKey{};
         if ((rc = record(0, [&] {
                int prc = launch_prep(ctx, st, a, ctx->d_status, false);
-               if (prc == FEM2D_OK && fill) prc = launch_stream_fill(ctx, st, a, o);
-               if (prc == FEM2D_OK && tail) prc = launch_large_fill(ctx, st, a, o);
+               if (prc != FEM2D_OK) return prc;
+               // the block-path fill writes segments of its own, from row
+               // pointers the layout scan wrote: a branch beside the streaming
+               // fill (a branch costs the graph ~20 us, so only this one)
+               if (tail) {
+                 HIP_TRY(hipEventRecord(ctx->fork_ev, st));
+                 HIP_TRY(hipStreamWaitEvent(ctx->side_stream, ctx->fork_ev, 0));
+                 if ((prc = launch_large_fill(ctx, ctx->side_stream, a, o))) return prc;
+                 HIP_TRY(hipEventRecord(ctx->join_ev, ctx->side_stream));
+               }
+               if (fill && (prc = launch_stream_fill(ctx, st, a, o))) return prc;
+               if (tail) HIP_TRY(hipStreamWaitEvent(st, ctx->join_ev, 0));
                return prc;
              })))
           return rc;
