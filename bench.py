@@ -313,13 +313,13 @@ def host_boundary(w, nnz: int, calls: int = 3):
             "entry_point": "fem2d_galerkin_sample_gep_hcurl (host view in, host CSR out)"}
 
 
-def domain_build(w, calls: int = 3):
+def domain_build(w, calls: int = 7):
     """Domain construction from the workload's Mesh (Domain::from_mesh, domain.rs:69-159), three
     ways, all giving the same Domain: on the GPU straight into HBM (fem2d_device_domain_from_mesh:
     host flattening of the Mesh, one upload, the kernels, the Domain's derived device fields), and
     on the host data-parallel and sequentially (the reference's construction order; neither
-    includes the later upload of the view, ``domain_upload_ms``). Mean over `calls` builds, after
-    one untimed build of each; each build includes freeing the previous one."""
+    includes the later upload of the view, ``domain_upload_ms``). Median over `calls` builds, after
+    one untimed build of each; each timed build includes freeing the previous one."""
     import fem_2d_amd as F
 
     mesh = w.domain.mesh
@@ -330,12 +330,14 @@ def domain_build(w, calls: int = 3):
                         ("serial", lambda: F.Domain.from_mesh_serial(mesh, F.ContinuityCondition.HCurl))):
         d = build()
         n = d.n_dofs if name == "device" else d.num_dofs
-        del d
-        t0 = time.perf_counter()
+        ts = []
         for _ in range(calls):
-            d = build()
+            t0 = time.perf_counter()
             del d
-        out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / calls
+            d = build()
+            ts.append(time.perf_counter() - t0)
+        del d
+        out[name + "_ms"] = float(sorted(ts)[len(ts) // 2]) * 1e3
     ctx.close()
     out["n_dofs"] = n
     out["threads"] = min(16, os.cpu_count() or 1)
