@@ -69,3 +69,18 @@ def test_error_messages():
     with pytest.raises(F.MeshAccessError):
         F.Mesh.unit().elem(5)
     assert "non-existent" in N.last_error()
+
+
+def test_device_domain_builder_argument_errors_without_a_device():
+    """fem2d_device_domain_from_mesh rejects null handles and unknown continuity conditions
+    before it touches the (absent) device; fem2d_device_domain_info rejects a null Domain."""
+    lib = N.load()
+    m = F.Mesh.unit()
+    out = C.c_void_p(1)
+    assert lib.fem2d_device_domain_from_mesh(None, m._h, 0, C.byref(out)) == N.ERR_INVALID_ARG
+    assert lib.fem2d_device_domain_from_mesh(C.c_void_p(1), None, 0, C.byref(out)) == N.ERR_INVALID_ARG
+    assert lib.fem2d_device_domain_from_mesh(C.c_void_p(1), m._h, 0, None) == N.ERR_INVALID_ARG
+    assert lib.fem2d_device_domain_from_mesh(C.c_void_p(1), m._h, 3, C.byref(out)) == N.ERR_INVALID_ARG
+    assert out.value is None  # cleared before the condition is checked
+    n = C.c_int64()
+    assert lib.fem2d_device_domain_info(None, None, C.byref(n), None) == N.ERR_INVALID_ARG
