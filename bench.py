@@ -411,8 +411,10 @@ def eigensolve(calls: int = 3):
     import cases
 
     ctx = F.Context(0)
-    out = {"solver": "dense symmetric-definite: shift-invert inverse iteration on rocSOLVER dgetrf/dgetrs with one "
-                     "refinement step (dsygvd when inconclusive); nalgebra: dpotrf+dpotri+dgemm+dsyevd; FP64"}
+    out = {"solver": "dense symmetric-definite: shift-invert inverse iteration on the rocSOLVER dgetrf factors of the "
+                     "diagonally scaled A - target B, solved with their inverses (rocBLAS trsm, up to 2048 DoFs) or "
+                     "dgetrs, refined against the unfactored matrix (dsygvd when inconclusive); nalgebra: "
+                     "dpotrf+dpotri+dgemm+dsyevd; FP64"}
 
     def timed(fn):
         fn()
@@ -427,6 +429,10 @@ def eigensolve(calls: int = 3):
         pair, ms = timed(lambda: F.solve_gep_dense(dev, 10.0, ctx=ctx))
         out[name] = {"n": int(dev.n), "ms": ms, "eigenvalue": pair.value,
                      "rel_err_vs_te10": abs(pair.value - np.pi ** 2) / np.pi ** 2}
+        # slepc_solve_gep: the dense solve up to SLEPC_DENSE_LIMIT DoFs, the sparse one beyond
+        pair, ms = timed(lambda: F.slepc_solve_gep(dev, 10.0, ctx=ctx))
+        out[name]["slepc_solve_gep"] = {"ms": ms, "path": "dense" if dev.n <= F.linalg.SLEPC_DENSE_LIMIT
+                                        else "sparse", "rel_err_vs_te10": abs(pair.value - np.pi ** 2) / np.pi ** 2}
         if name == "waveguide_1096":
             from oracle import eigen as OE
 

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -224,6 +225,78 @@ int nalgebra_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, f
   return pick(hD, C, n, target, st, out);
 }
 
+// X = I (column-major n x n)
+__global__ void k_identity(long long n, double* __restrict__ X) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n * n; i += stride)
+    X[i] = i % n == i / n ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// The two substitutions of an LU solve K y = b (K = P L U, getrf's row
+// interchanges applied to b first) as products with the inverse factors,
+// stored transposed so that each output entry is one wavefront's dot product
+// with a contiguous column: t = L^-1 P^T b with column c of X1 = (L^-1)^T
+// (rows 0..c), then y = U^-1 t with column c of X2 = (U^-1)^T (rows c..n-1).
+// src_of[r] is the entry of b that lands in row r after the interchanges.
+template <bool kLower>
+__global__ void k_lu_apply(long long n, const double* __restrict__ X, const int* __restrict__ src_of,
+                           const double* __restrict__ d, const double* __restrict__ x, double* __restrict__ y) {
+  const int lane = threadIdx.x % kWave;
+  const long long wave = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave;
+  const long long waves = ((long long)gridDim.x * blockDim.x) / kWave;
+  for (long long c = wave; c < n; c += waves) {
+    const long long r0 = kLower ? 0 : c, r1 = kLower ? c + 1 : n;
+    const double* __restrict__ col = X + c * n;
+    double s0 = 0.0, s1 = 0.0;
+    long long r = r0 + lane;
+    // the lower pass gathers the interchanged, scaled right-hand side d[i] b[i]
+    auto in = [&](long long q) {
+      if (!kLower) return x[q];
+      const int i = src_of[q];
+      return d[i] * x[i];
+    };
+    for (; r + kWave < r1; r += 2 * kWave) {
+      s0 += col[r] * in(r);
+      s1 += col[r + kWave] * in(r + kWave);
+    }
+    for (; r < r1; r += kWave) s0 += col[r] * in(r);
+    const double s = wave_sum(s0 + s1);
+    if (lane == 0) y[c] = kLower ? s : d[c] * s;  // the upper pass scales its result back
+  }
+}
+
+// d[i] = 1 / sqrt(|K(i, i)|) (1 where that is not a positive finite number)
+__global__ void k_diag_scale(long long n, const double* __restrict__ K, double* __restrict__ d) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double v = 1.0 / std::sqrt(std::fabs(K[i + i * n]));
+    d[i] = v > 0.0 && v < HUGE_VAL ? v : 1.0;
+  }
+}
+
+// S = D K D, D = diag(d)
+__global__ void k_scale_sym(long long n, const double* __restrict__ K, const double* __restrict__ d,
+                            double* __restrict__ S) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n * n; idx += stride) {
+    const long long i = idx % n, j = idx / n;
+    S[idx] = d[i] * K[idx] * d[j];
+  }
+}
+
+// y[i] = d[i] x[i]
+__global__ void k_scale_vec(long long n, const double* __restrict__ d, const double* __restrict__ x,
+                            double* __restrict__ y) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) y[i] = d[i] * x[i];
+}
+
 // K = A - shift B, elementwise over the dense matrices
 __global__ void k_shift(long long count, const double* __restrict__ A, const double* __restrict__ B, double shift,
                         double* __restrict__ K) {
@@ -237,6 +310,9 @@ __global__ void k_shift(long long count, const double* __restrict__ A, const dou
 // dominant one of K^-1 B), lambda = x^T A x. Returns 1 when the iteration is
 // inconclusive (K singular, no convergence, or an eigenvalue so small that the
 // spurious-mode test needs the whole spectrum), leaving A and B unchanged.
+// The solves with K use its inverse LU factors (two trsm against the identity,
+// once), each solve two bandwidth-bound passes over them (k_lu_apply); every
+// solve is followed by one step of iterative refinement against K itself.
 int inverse_iteration(rocblas_handle h, hipStream_t st, long long n, const double* A, const double* B, double* K,
                       double target, DevMem& m, double* value, std::vector<double>* vec) {
   const rocblas_int ni = rocblas_int(n);
@@ -252,11 +328,77 @@ int inverse_iteration(rocblas_handle h, hipStream_t st, long long n, const doubl
   EIG_HIP(m.alloc(&ax, size_t(n)));
   k_shift<<<grid_for(n * n), kBlock, 0, st>>>(n * n, A, B, target, K0);
   EIG_HIP(hipGetLastError());
-  EIG_HIP(hipMemcpyAsync(K, K0, size_t(n * n) * sizeof(double), hipMemcpyDeviceToDevice, st));
+  // the factored matrix is D K D, D the inverse square roots of K's diagonal:
+  // the hierarchical shape functions' scales differ by orders of magnitude
+  double* dsc;
+  EIG_HIP(m.alloc(&dsc, size_t(n)));
+  k_diag_scale<<<grid_for(n), kBlock, 0, st>>>(n, K0, dsc);
+  k_scale_sym<<<grid_for(n * n), kBlock, 0, st>>>(n, K0, dsc, K);
+  EIG_HIP(hipGetLastError());
   EIG_BLAS(rocsolver_dgetrf(h, ni, ni, K, ni, ipiv, d_info));
   rocblas_int info = 0;
   if (int rc = read_info(d_info, st, &info)) return rc;
   if (info != 0) return 1;  // the target is an eigenvalue to working precision
+  // the row interchanges as a gather map, and the transposed inverse factors
+  // (up to kInverseMax DoFs: beyond, the two trsm cost more than the
+  // substitutions they replace)
+  constexpr long long kInverseMax = 2048;
+  bool exact = n > kInverseMax;
+  double *X1 = nullptr, *X2 = nullptr;
+  int* src_of = nullptr;
+  if (!exact) {
+    EIG_HIP(m.alloc(&X1, size_t(n * n)));
+    EIG_HIP(m.alloc(&X2, size_t(n * n)));
+    EIG_HIP(m.alloc(&src_of, size_t(n)));
+  }
+  if (!exact) {
+    std::vector<rocblas_int> piv(static_cast<size_t>(n));
+    EIG_HIP(hipMemcpyAsync(piv.data(), ipiv, size_t(n) * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+    EIG_HIP(hipStreamSynchronize(st));
+    std::vector<int> src(static_cast<size_t>(n));
+    for (long long i = 0; i < n; ++i) src[size_t(i)] = int(i);
+    for (long long i = 0; i < n; ++i) {  // getrs applies the interchanges in order (1-based pivots)
+      const long long p = piv[size_t(i)] - 1;
+      if (p < 0 || p >= n) {
+        set_error("getrf returned a pivot outside the matrix", 0);
+        return FEM2D_ERR_HIP;
+      }
+      std::swap(src[size_t(i)], src[size_t(p)]);
+    }
+    EIG_HIP(hipMemcpyAsync(src_of, src.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice, st));
+    EIG_HIP(hipStreamSynchronize(st));  // src is read before it goes out of scope
+  }
+  if (!exact) {
+    k_identity<<<grid_for(n * n), kBlock, 0, st>>>(n, X1);
+    EIG_HIP(hipGetLastError());
+    k_identity<<<grid_for(n * n), kBlock, 0, st>>>(n, X2);
+    EIG_HIP(hipGetLastError());
+    const double one = 1.0;
+    // L^T X1 = I and U^T X2 = I: X1 = (L^-1)^T, X2 = (U^-1)^T
+    EIG_BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                           rocblas_diagonal_unit, ni, ni, &one, K, ni, X1, ni));
+    EIG_BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                           rocblas_diagonal_non_unit, ni, ni, &one, K, ni, X2, ni));
+  }
+  double* t;
+  EIG_HIP(m.alloc(&t, size_t(n)));
+  const unsigned apply_grid = unsigned(std::max<long long>(1, std::min<long long>((n + 3) / 4, 1 << 16)));
+  // y = K^-1 b = D (D K D)^-1 D b: the inverse factors, or (exact) getrs's
+  // substitutions when those prove too inaccurate for this K
+  auto lu_solve = [&](const double* b, double* y) -> int {
+    if (!exact) {
+      k_lu_apply<true><<<apply_grid, kBlock, 0, st>>>(n, X1, src_of, dsc, b, t);
+      k_lu_apply<false><<<apply_grid, kBlock, 0, st>>>(n, X2, nullptr, dsc, t, y);
+      EIG_HIP(hipGetLastError());
+      return FEM2D_OK;
+    }
+    k_scale_vec<<<grid_for(n), kBlock, 0, st>>>(n, dsc, b, t);
+    EIG_HIP(hipGetLastError());
+    EIG_BLAS(rocsolver_dgetrs(h, rocblas_operation_none, ni, 1, K, ni, ipiv, t, ni));
+    k_scale_vec<<<grid_for(n), kBlock, 0, st>>>(n, dsc, t, y);
+    EIG_HIP(hipGetLastError());
+    return FEM2D_OK;
+  };
   const std::vector<double> x0 = start_vector(n);
   EIG_HIP(hipMemcpyAsync(x, x0.data(), size_t(n) * sizeof(double), hipMemcpyHostToDevice, st));
   EIG_HIP(hipStreamSynchronize(st));  // x0 is read before any return below
@@ -265,19 +407,41 @@ int inverse_iteration(rocblas_handle h, hipStream_t st, long long n, const doubl
   // converged: a relative residual |A x - lambda B x| / |A x| of kTol, or of
   // kTolStall once the Rayleigh quotient no longer moves (the residual's
   // rounding floor grows with the conditioning of A and B)
-  constexpr int kMaxIter = 100;
-  constexpr double kTol = 1e-12, kTolStall = 1e-10;
+  constexpr int kMaxIter = 100, kRefine = 6;
+  constexpr double kTol = 1e-12, kTolStall = 1e-10, kSolveTol = 1e-8;
   double lam_prev = HUGE_VAL;
   for (int it = 0; it < kMaxIter; ++it) {
-    EIG_BLAS(rocblas_dcopy(h, ni, bx, 1, y, 1));
-    EIG_BLAS(rocsolver_dgetrs(h, rocblas_operation_none, ni, 1, K, ni, ipiv, y, ni));  // y = K^-1 B x
-    // one step of iterative refinement against the unfactored K: the
-    // triangular solves alone leave errors of the size of K's conditioning
+    // y = K^-1 B x, then iterative refinement against the unfactored K while
+    // the residual B x - K y keeps shrinking (at most kRefine steps): the
+    // products with the inverse factors leave errors of the size of their
+    // conditioning. A solve the refinement leaves inaccurate (the first
+    // iteration shows it) switches every later solve to getrs.
     const double m_one = -1.0;
-    EIG_BLAS(rocblas_dcopy(h, ni, bx, 1, r, 1));
-    EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &m_one, K0, ni, y, 1, &one, r, 1));  // B x - K y
-    EIG_BLAS(rocsolver_dgetrs(h, rocblas_operation_none, ni, 1, K, ni, ipiv, r, ni));
-    EIG_BLAS(rocblas_daxpy(h, ni, &one, r, 1, y, 1));
+    double nb = 0.0;
+    if (!exact) EIG_BLAS(rocblas_dnrm2(h, ni, bx, 1, &nb));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if (int rc = lu_solve(bx, y)) return rc;
+      double nres = HUGE_VAL, nr_prev = HUGE_VAL;
+      // getrs: one step and no residual norms, its substitutions being backward stable
+      const int steps = exact ? 1 : kRefine;
+      for (int k = 0; k <= steps; ++k) {
+        if (exact && k == steps) break;
+        EIG_BLAS(rocblas_dcopy(h, ni, bx, 1, r, 1));
+        EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &m_one, K0, ni, y, 1, &one, r, 1));  // B x - K y
+        if (!exact) {
+          EIG_BLAS(rocblas_dnrm2(h, ni, r, 1, &nres));
+          if (k == steps || !(nres > 1e-15 * nb) || !(nres < 0.5 * nr_prev)) break;
+          nr_prev = nres;
+        }
+        if (int rc = lu_solve(r, ax)) return rc;  // the correction (ax is free until below)
+        EIG_BLAS(rocblas_daxpy(h, ni, &one, ax, 1, y, 1));
+      }
+      if (exact || it > 0 || nres <= kSolveTol * nb) break;
+      if (verbose())
+        std::fprintf(stderr, "[fem2d dense] n=%lld inverse factors leave a residual %.3e: getrs from here on\n", n,
+                     nres / nb);
+      exact = true;
+    }
     EIG_BLAS(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, B, ni, y, 1, &zero, bx, 1));
     double s = 0.0;
     EIG_BLAS(rocblas_ddot(h, ni, y, 1, bx, 1, &s));
@@ -297,6 +461,8 @@ int inverse_iteration(rocblas_handle h, hipStream_t st, long long n, const doubl
     if (!std::isfinite(lam) || !std::isfinite(nr)) return 1;
     const bool stalled = std::fabs(lam - lam_prev) <= 1e-14 * std::fabs(lam);
     lam_prev = lam;
+    if (verbose())
+      std::fprintf(stderr, "[fem2d dense] n=%lld it=%d lambda=%.17g residual=%.3e\n", n, it, lam, nr / na);
     if (nr <= kTol * na || (stalled && nr <= kTolStall * na)) {
       if (std::fabs(lam) < kSpurious) return 1;
       vec->resize(static_cast<size_t>(n));

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -21,6 +23,15 @@ using fem2d::capi::set_error;
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+
+// progress lines on stderr when FEM2D_EIGEN_VERBOSE is set (to anything but 0)
+inline bool verbose() {
+  static const bool v = [] {
+    const char* e = std::getenv("FEM2D_EIGEN_VERBOSE");
+    return e && *e && std::strcmp(e, "0") != 0;
+  }();
+  return v;
+}
 
 inline unsigned grid_for(long long count) {
   return unsigned(std::max<long long>(1, std::min<long long>((count + kBlock - 1) / kBlock, 1 << 16)));

@@ -755,15 +755,6 @@ int inner_solve(Factor& F, const SymMats& S, double target, rocblas_handle h, co
   return FEM2D_OK;
 }
 
-// progress lines on stderr when FEM2D_EIGEN_VERBOSE is set (to anything but 0)
-bool verbose() {
-  static const bool v = [] {
-    const char* e = std::getenv("FEM2D_EIGEN_VERBOSE");
-    return e && *e && std::strcmp(e, "0") != 0;
-  }();
-  return v;
-}
-
 int sparse_solve(fem2d_ctx* ctx, const fem2d_gep_device* gep, double target, fem2d_eigenpair* out) {
   if (int rc = check_device_args(ctx, gep, out)) return rc;
   const long long n = gep->n, nnz = gep->nnz;

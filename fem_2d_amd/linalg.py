@@ -80,8 +80,10 @@ def solve_gep_sparse(gep: Union[GEP, N.GepDevice], target_eigenvalue: float,
     return _solve("fem2d_solve_gep_sparse", "fem2d_gep_device_solve_sparse", gep, target_eigenvalue, ctx)
 
 
-# the dense solve is used up to this many DoFs by slepc_solve_gep, the sparse one beyond
-SLEPC_DENSE_LIMIT = 4096
+# the dense solve is used up to this many DoFs by slepc_solve_gep, the sparse one beyond (at 4496
+# DoFs the dense solve takes 106-154 ms, its LU panels and substitutions latency-bound, the sparse
+# one 36 ms; at 1096 both take ~15 ms)
+SLEPC_DENSE_LIMIT = 2048
 
 
 def slepc_solve_gep(gep: Union[GEP, N.GepDevice], target_eigenvalue: float,
