@@ -1026,14 +1026,16 @@ size_t gram_table_bytes(const LibShape& L) {
 // own, more than most of these parts).
 // ---------------------------------------------------------------------------
 
-// counters and spec slots cleared
-__global__ void __launch_bounds__(kBlock) k_init(Work w, int n_elems, long long n_dofs) {
-  init_body(w, n_elems, n_dofs, blockIdx.x * (long long)kBlock + threadIdx.x);
-}
-
 // The Gram library of one assembly (a function of the Domain's dyadic
-// sub-ranges, the shape-function space and the Gauss points)
-__global__ void __launch_bounds__(kBlock) k_gram(Work w, LibShape L, GlqArgs glq, int use_table) {
+// sub-ranges, the shape-function space and the Gauss points) in the first
+// gram_blocks blocks, the clearing of the counters in the rest: the
+// two are independent, and one launch saves the second's ramp
+__global__ void __launch_bounds__(kBlock) k_gram(Work w, LibShape L, GlqArgs glq, int use_table, unsigned gram_blocks,
+                                                 int n_elems, long long n_dofs) {
+  if (blockIdx.x >= gram_blocks) {
+    init_body(w, n_elems, n_dofs, (blockIdx.x - gram_blocks) * (long long)kBlock + threadIdx.x);
+    return;
+  }
   extern __shared__ double gram_tab[];
   double* tab = use_table ? gram_tab : nullptr;
   // one instance per shape-function space (no evaluation of the other's recurrence)
@@ -2655,10 +2657,11 @@ int launch_prep(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, long long* st
   const long long nd = v.n_dofs, nbs = v.n_bs;
   const int B = 256;
   auto scan_grid = [&](long long tiles) { return unsigned(std::max(1LL, std::min(tiles, 8LL * ctx->n_cu))); };
-  k_init<<<grid_for(ne, B), B, 0, st>>>(w, ne, nd);
   const size_t gram_lds = gram_table_bytes(a.L);
-  k_gram<<<grid_for(a.lib_n, B), B, gram_lds, st>>>(w, a.L, ctx->glq, gram_lds > 0 ? 1 : 0);
-  HIP_TRY_AT(hipGetLastError(), "after the Gram library kernel");
+  const unsigned gram_blocks = grid_for(a.lib_n, B);
+  k_gram<<<gram_blocks + grid_for(ne, B), B, gram_lds, st>>>(w, a.L, ctx->glq, gram_lds > 0 ? 1 : 0, gram_blocks, ne,
+                                                             nd);
+  HIP_TRY_AT(hipGetLastError(), "after the Gram library and clearing kernel");
   // nested-list offsets and row groups side by side
   const unsigned gn = scan_grid(a.tiles_nested), gd = scan_grid(a.tiles_dof), gm = scan_grid(a.tiles_mem);
   k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, a.ts_nested, gn, nd, DofGroupScan{v, w},
@@ -3349,7 +3352,7 @@ int fem2d_assemble_device_noalloc(fem2d_ctx* ctx, const fem2d_device_domain* dd,
   hipStream_t st = ctx->stream;
   ensure_glq(ctx, nu, nv);
   const FillOut o = make_fill_out(precision, ai, bi, capacity, col, a_val, b_val, res16);
-  // kernel by kernel on the context's stream: k_init clears every counter
+  // kernel by kernel on the context's stream: k_gram's clearing blocks reset every counter
   // and flag the kernels accumulate into, so the array's earlier contents do
   // not matter
   if ((rc = launch_prep(ctx, st, a, ctx->d_status, false))) return rc;
