@@ -1026,14 +1026,23 @@ size_t gram_table_bytes(const LibShape& L) {
 // own, more than most of these parts).
 // ---------------------------------------------------------------------------
 
-// The Gram library of one assembly (a function of the Domain's dyadic
-// sub-ranges, the shape-function space and the Gauss points) in the first
-// gram_blocks blocks, the clearing of the counters in the rest: the
-// two are independent, and one launch saves the second's ramp
-__global__ void __launch_bounds__(kBlock) k_gram(Work w, LibShape L, GlqArgs glq, int use_table, unsigned gram_blocks,
-                                                 int n_elems, long long n_dofs) {
+// The first launch of an assembly, by block ranges of three independent
+// parts: the Gram library (a function of the Domain's dyadic sub-ranges, the
+// shape-function space and the Gauss points; gram_blocks blocks), the clearing
+// of the counters (init_blocks), and the first pass of the nested-list and
+// row-group prefix sums (gn and gd blocks, from the Domain alone). One launch
+// saves the others' ramps.
+__global__ void __launch_bounds__(kBlock) k_gram(DevView v, Work w, LibShape L, GlqArgs glq, int use_table,
+                                                 unsigned gram_blocks, unsigned init_blocks, unsigned gn,
+                                                 long long* ts_nested, long long* ts_dof) {
+  if (blockIdx.x >= gram_blocks + init_blocks) {
+    const unsigned b = blockIdx.x - gram_blocks - init_blocks;
+    if (b < gn) scan_reduce_body(v.n_elems + 1, NestedScan{w, v.n_elems}, ts_nested, b, gn);
+    else scan_reduce_body(v.n_dofs, DofGroupScan{v, w}, ts_dof, b - gn, gridDim.x - gram_blocks - init_blocks - gn);
+    return;
+  }
   if (blockIdx.x >= gram_blocks) {
-    init_body(w, n_elems, n_dofs, (blockIdx.x - gram_blocks) * (long long)kBlock + threadIdx.x);
+    init_body(w, v.n_elems, v.n_dofs, (blockIdx.x - gram_blocks) * (long long)kBlock + threadIdx.x);
     return;
   }
   extern __shared__ double gram_tab[];
@@ -1065,16 +1074,8 @@ __global__ void __launch_bounds__(kBlock) k_domain_prep(DevView v, Work w, unsig
   }
 }
 
-// two independent prefix sums in one launch (first pass, then second pass)
-template <class OpA, class OpB>
-__global__ void __launch_bounds__(kBlock) k_scan_reduce2(long long na, OpA a, long long* ts_a, unsigned nba,
-                                                         long long nb, OpB b, long long* ts_b) {
-  if (blockIdx.x < nba)
-    scan_reduce_body(na, a, ts_a, blockIdx.x, nba);
-  else
-    scan_reduce_body(nb, b, ts_b, blockIdx.x - nba, gridDim.x - nba);
-}
-
+// second pass of two independent prefix sums in one launch (the first pass
+// shares k_gram's launch)
 template <class OpA, class OpB>
 __global__ void __launch_bounds__(kBlock) k_scan_apply2(long long na, OpA a, const long long* ts_a, unsigned nba,
                                                         long long nb, OpB b, const long long* ts_b) {
@@ -2657,15 +2658,16 @@ int launch_prep(fem2d_ctx* ctx, hipStream_t st, const Assembly& a, long long* st
   const long long nd = v.n_dofs, nbs = v.n_bs;
   const int B = 256;
   auto scan_grid = [&](long long tiles) { return unsigned(std::max(1LL, std::min(tiles, 8LL * ctx->n_cu))); };
+  // the Gram library, the counters' clearing and the first pass of the
+  // nested-list offsets and the row groups, side by side
   const size_t gram_lds = gram_table_bytes(a.L);
-  const unsigned gram_blocks = grid_for(a.lib_n, B);
-  k_gram<<<gram_blocks + grid_for(ne, B), B, gram_lds, st>>>(w, a.L, ctx->glq, gram_lds > 0 ? 1 : 0, gram_blocks, ne,
-                                                             nd);
-  HIP_TRY_AT(hipGetLastError(), "after the Gram library and clearing kernel");
-  // nested-list offsets and row groups side by side
+  const unsigned gram_blocks = grid_for(a.lib_n, B), init_blocks = grid_for(ne, B);
   const unsigned gn = scan_grid(a.tiles_nested), gd = scan_grid(a.tiles_dof), gm = scan_grid(a.tiles_mem);
-  k_scan_reduce2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, a.ts_nested, gn, nd, DofGroupScan{v, w},
-                                             a.ts_dof);
+  k_gram<<<gram_blocks + init_blocks + gn + gd, B, gram_lds, st>>>(v, w, a.L, ctx->glq, gram_lds > 0 ? 1 : 0,
+                                                                   gram_blocks, init_blocks, gn, a.ts_nested,
+                                                                   a.ts_dof);
+  HIP_TRY_AT(hipGetLastError(), "after the first preparation kernel");
+  // second pass of both
   k_scan_apply2<<<gn + gd, kBlock, 0, st>>>(ne + 1, NestedScan{w, ne}, a.ts_nested, gn, nd, DofGroupScan{v, w},
                                             a.ts_dof);
   // nested lists beside the first pass of the membership numbering
