@@ -3244,6 +3244,7 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     return fem2d_assemble_device(ctx, dd, basis, ai, bi, glq_u, glq_v, precision, out);
   }
   dd->large_state = n_large > 0 ? 1 : 0;
+  a.run_large = n_large > 0;  // the tuning trials below skip the block path's count when it has no group
   ctx->timing.fill_chunk = w.chunk;
   ctx->timing.fill_lib_lds = fill_place(precision, a.lib_lds, a.lib_n);
   if (ctx->autotune && !ctx->fill_chunk && dd->tuned_chunk[precision] == 0) {
