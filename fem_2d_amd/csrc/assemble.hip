@@ -3885,7 +3885,13 @@ int fem2d::capi::device_domain_from_flat(fem2d_ctx* ctx, const MeshFlat& m, int3
     HIP_TRY_AT(B::launch_counts(a, st), "launching the Domain counts");
     HIP_TRY(hipMemcpyAsync(&h, status, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (h.flags & (B::kFlagNoPair | B::kFlagInconsistent | B::kFlagTooDeep)) return kActivationFailed;
+    if (h.flags & B::kFlagTooDeep) {
+      set_error("an Edge lies more than " + std::to_string(B::kMaxEdgeDepth) +
+                    " refinements below its base Edge (the device Domain build's limit)",
+                0);
+      return FEM2D_ERR_UNSUPPORTED;
+    }
+    if (h.flags & (B::kFlagNoPair | B::kFlagInconsistent)) return kActivationFailed;
     if (h.elem_dofs + h.edge_dofs > 0x7fffffffLL) {
       set_error("too many DoFs for 32-bit ids", 0);
       return FEM2D_ERR_PANIC;
