@@ -624,6 +624,10 @@ int fem2d_device_domain_from_mesh(fem2d_ctx* ctx, const fem2d_mesh* m, int32_t c
     }
     capi::MeshFlat f;
     flatten_mesh(mesh, default_threads(), f);
+    if (f.all_specs > INT32_MAX) {  // Domain::from_mesh's id limit, after its activation
+      mesh.clone(1).set_edge_activation(1);
+      throw MeshPanic("too many BasisSpecs for 32-bit ids");
+    }
     int rc = capi::device_domain_from_flat(ctx, f, cc, out);
     if (rc == capi::kActivationFailed) {
       // the device found an Edge without an active pair where the reference
