@@ -3256,14 +3256,16 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     float best_ms = HUGE_VALF;
     const FillOut o = make_fill_out(precision, ai, bi, ctx->out_capacity, static_cast<int*>(ctx->out_col.p),
                                     ctx->out_a.p, ctx->out_b.p, static_cast<unsigned*>(ctx->out_r.p));
-    for (int place : {kPlaceLdsSmall, kPlaceLds, kPlaceGlobal}) {
-      if (!place_fits(precision, place, a.lib_n)) continue;
-      a.lib_lds = place;
-      for (int c : kChunkCandidates) {
-        set_chunk(w, c, precision);
-        w.chunk_cap = o.capacity / w.chunk + 1;
-        if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
-        for (int rep = 0; rep < kTuneReps; ++rep) {
+    // the repetitions are the outer loop, so that a drift of the GPU's clocks
+    // over the trials favours no candidate
+    for (int rep = 0; rep < kTuneReps; ++rep)
+      for (int place : {kPlaceLdsSmall, kPlaceLds, kPlaceGlobal}) {
+        if (!place_fits(precision, place, a.lib_n)) continue;
+        a.lib_lds = place;
+        for (int c : kChunkCandidates) {
+          set_chunk(w, c, precision);
+          w.chunk_cap = o.capacity / w.chunk + 1;
+          if ((rc = slot(ctx, kChunkSlot, size_t(w.chunk_cap), &w.chunk_first))) return rc;
           if ((rc = launch_prep(ctx, st, a, ctx->d_status, false))) return rc;
           HIP_TRY(hipEventRecord(ctx->ev[6], st));
           if ((rc = launch_stream_fill(ctx, st, a, o))) return rc;
@@ -3279,7 +3281,6 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
           }
         }
       }
-    }
     dd->tuned_chunk[precision] = best;
     dd->tuned_lds[precision] = best_lds;
     return fem2d_assemble_device(ctx, dd, basis, ai, bi, glq_u, glq_v, precision, out);
