@@ -305,6 +305,22 @@ class DeviceDomain:
                                               int(precision), C.byref(out)))
         return out
 
+    def assembler(self, glq_grid_dim=None, BSpace=HierPoly, AI=CurlCurl, BI=L2Inner, precision: int = FP64):
+        """A callable running :meth:`assemble` with these arguments, converted once: each call
+        is one foreign call into the library and returns the same GepDevice object, refilled."""
+        gu, gv = _glq(glq_grid_dim)
+        out = N.GepDevice()
+        fn = self._lib.fem2d_assemble_device
+        args = (self.ctx._h, self._h, BSpace.code, AI.code, BI.code, gu, gv, int(precision), C.byref(out))
+
+        def run() -> N.GepDevice:
+            rc = fn(*args)
+            if rc:
+                check(rc)
+            return out
+
+        return run
+
     def print_to_petsc_binary_files(self, dev: N.GepDevice, dir: str, prefix: str) -> None:
         """PETSc AIJ binary files of a device-resident GEP (see GEP.print_to_petsc_binary_files)."""
         check(self._lib.fem2d_gep_device_print_to_petsc_binary_files(self.ctx._h, C.byref(dev), str(dir).encode(),
