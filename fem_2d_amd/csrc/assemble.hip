@@ -2246,6 +2246,7 @@ struct fem2d_ctx {
   GlqArgs glq{};
   int glq_n[2] = {0, 0};  // point counts glq holds
   fem2d_timing timing{};
+  int timing_events = 0;  // event times of the latest assembly not yet read: 1 graph, 2 kernel by kernel
   int n_cu = 0;
   int count_grid = 0;  // resident blocks of the wave kernels on this device
   // resident blocks of the streaming fill per stored precision, with the library in
@@ -2983,6 +2984,26 @@ int fem2d_ctx_last_timing(fem2d_ctx* ctx, fem2d_timing* out) {
     set_error("null argument", 0);
     return FEM2D_ERR_INVALID_ARG;
   }
+  if (ctx->timing_events) {  // the latest assembly's events (it has synchronised)
+    float t01 = 0, t12 = 0, t23 = 0, t03 = 0, tc = 0, tf = 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
+    if (ctx->timing_events == 2) {  // per-stage events exist in kernel-by-kernel mode only
+      (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
+      (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
+      (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
+      (void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
+      (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
+    }
+    (void)hipGetLastError();  // a timing query failure is not an assembly failure
+    ctx->timing.prep_ms = t01;
+    ctx->timing.symbolic_ms = t12;
+    ctx->timing.fill_ms = t23;
+    ctx->timing.total_ms = t03;
+    ctx->timing.count_kernel_ms = tc;
+    ctx->timing.fill_kernel_ms = tf;
+    ctx->timing_events = 0;
+  }
   *out = ctx->timing;
   return FEM2D_OK;
 }
@@ -3103,8 +3124,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
   if (int rc = check_assembly(dd, glq_u, glq_v, &nu, &nv)) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   // whatever happens below, the output buffers no longer hold the previous
-  // assembly's GEP: its handles become stale
+  // assembly's GEP: its handles become stale; nor do the events its times
   ctx->out_generation = 0;
+  ctx->timing_events = 0;
   const long long nd = dd->n_dofs;
 
   // work buffers (grow-only, reused across calls)
@@ -3219,22 +3241,8 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
     ctx->out_capacity = (long long)cap;
   }
 
-  float t01 = 0, t12 = 0, t23 = 0, t03 = 0, tc = 0, tf = 0;
-  (void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
-  if (fine_timing) {  // per-stage events exist in kernel-by-kernel mode only
-    (void)hipEventElapsedTime(&tf, ctx->ev[6], ctx->ev[7]);
-    (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
-    (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
-    (void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
-    (void)hipEventElapsedTime(&tc, ctx->ev[4], ctx->ev[5]);
-  }
-  (void)hipGetLastError();  // a timing query failure is not an assembly failure
-  ctx->timing.prep_ms = t01;
-  ctx->timing.symbolic_ms = t12;
-  ctx->timing.fill_ms = t23;
-  ctx->timing.total_ms = t03;
-  ctx->timing.count_kernel_ms = tc;
-  ctx->timing.fill_kernel_ms = tf;
+  // the event times are read when the caller asks for them (fem2d_ctx_last_timing)
+  ctx->timing_events = fine_timing ? 2 : 1;
   ctx->timing.nnz = nnz;
   ctx->timing.n_groups = ctx->h_status[3];
   ctx->timing.n_large = n_large;
