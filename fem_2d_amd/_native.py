@@ -217,6 +217,7 @@ SIGNATURES = {
     "fem2d_mesh_global_h_refinement": (C.c_int, [vp, i32, i32]),
     "fem2d_mesh_h_refine_elems": (C.c_int, [vp, P(i32), i64, i32, i32]),
     "fem2d_mesh_execute_h_refinements": (C.c_int, [vp, P(i32), P(i32), P(i32), i64]),
+    "fem2d_mesh_execute_h_refinements_threads": (C.c_int, [vp, P(i32), P(i32), P(i32), i64, i32]),
     "fem2d_mesh_global_p_refinement": (C.c_int, [vp, i32, i32]),
     "fem2d_mesh_p_refine_elems": (C.c_int, [vp, P(i32), i64, i32, i32]),
     "fem2d_mesh_execute_p_refinements": (C.c_int, [vp, P(i32), P(i32), P(i32), i64]),

@@ -423,14 +423,20 @@ int fem2d_mesh_h_refine_elems(fem2d_mesh* m, const int32_t* ids, int64_t n, int3
   FEM2D_GUARD(m->m.h_refine_elems(std::vector<int>(ids, ids + n), make_href(kind, ext)))
 }
 
-int fem2d_mesh_execute_h_refinements(fem2d_mesh* m, const int32_t* ids, const int32_t* kinds, const int32_t* exts,
-                                     int64_t n) {
+int fem2d_mesh_execute_h_refinements_threads(fem2d_mesh* m, const int32_t* ids, const int32_t* kinds,
+                                             const int32_t* exts, int64_t n, int32_t n_threads) {
   FEM2D_REQUIRE(m && n >= 0 && (n == 0 || (ids && kinds && exts)), "invalid argument");
+  FEM2D_REQUIRE(n_threads >= 0, "negative thread count");
   FEM2D_GUARD({
     std::vector<std::pair<int, HRef>> refs;
     for (int64_t k = 0; k < n; ++k) refs.push_back({ids[k], make_href(kinds[k], exts[k])});
-    m->m.execute_h_refinements(refs);
+    m->m.execute_h_refinements(refs, n_threads);
   })
+}
+
+int fem2d_mesh_execute_h_refinements(fem2d_mesh* m, const int32_t* ids, const int32_t* kinds, const int32_t* exts,
+                                     int64_t n) {
+  return fem2d_mesh_execute_h_refinements_threads(m, ids, kinds, exts, n, 0);
 }
 
 int fem2d_mesh_global_p_refinement(fem2d_mesh* m, int32_t di, int32_t dj) {

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 
 #include "json.hpp"
@@ -628,7 +629,52 @@ void Mesh::h_refine_elems(const std::vector<int>& ids, const HRef& r) {
   execute_h_refinements(refs);
 }
 
-void Mesh::execute_h_refinements(const std::vector<std::pair<int, HRef>>& refinements) {
+// How one refinement creates its Nodes and Edges: appended as they are made
+// (the reference's sequential loop), or at the ids planned for the whole batch
+struct Mesh::RefineOps {
+  virtual ~RefineOps() = default;
+  virtual int center_node(const Point& p) = 0;
+  // the child Edges of an Edge and the Node between them (split now if needed)
+  virtual std::pair<std::array<int, 2>, int> split(int edge_id) = 0;
+  virtual int inner_edge(std::array<int, 2> node_ids, int parent_elem) = 0;
+};
+
+struct Mesh::AppendOps : Mesh::RefineOps {
+  Mesh& m;
+  explicit AppendOps(Mesh& mesh) : m(mesh) {}
+  int center_node(const Point& p) override {
+    Node n;
+    n.id = int(m.nodes.size());
+    n.coords = p;
+    n.boundary = false;
+    m.nodes.push_back(n);
+    return n.id;
+  }
+  std::pair<std::array<int, 2>, int> split(int edge_id) override { return m.h_refine_edge_if_needed(edge_id); }
+  int inner_edge(std::array<int, 2> node_ids, int parent_elem) override {
+    return m.new_edge_between_nodes(node_ids, parent_elem);
+  }
+};
+
+std::vector<Mesh::ElemUninit> Mesh::uninit_children(int elem_id, const HRef& r, int first_id) {
+  std::vector<ElemUninit> ne;
+  const int n = r.num_children();
+  for (int idx = 0; idx < n; ++idx) {
+    ElemUninit u;
+    u.id = first_id + idx;
+    u.element = elems[elem_id].element;
+    u.ancestors = elems[elem_id].ancestors;
+    u.ancestors.push_back({elem_id, r.loc(idx)});
+    u.h_levels = elems[elem_id].h_levels.refined(r);
+    u.poly_orders = elems[elem_id].poly_orders;
+    ne.push_back(u);
+    elems[elem_id].children.push_back(u.id);
+  }
+  return ne;
+}
+
+void Mesh::execute_h_refinements(const std::vector<std::pair<int, HRef>>& refinements, int threads) {
+  if (threads <= 0) threads = default_threads();
   std::map<int, HRef> refmap;
   for (const auto& [elem_id, h] : refinements) {
     if (elem_id < 0 || elem_id >= int(elems.size()))
@@ -646,54 +692,238 @@ void Mesh::execute_h_refinements(const std::vector<std::pair<int, HRef>>& refine
   }
 
   std::vector<std::pair<int, HRef>> extensions;
-  int elem_id_tracker = int(elems.size());
-
-  for (const auto& [elem_id, r] : refmap) {
-    // Elem::h_refine (elem.rs:128-155)
-    if (elems[elem_id].has_children())
-      throw HRefError(HRefErrorKind::ElemHasChildren, elem_id,
-                      "Elem " + std::to_string(elem_id) + " already has children; Cannot h-refine!");
-    std::vector<ElemUninit> ne;
-    const int n = r.num_children();
-    for (int idx = 0; idx < n; ++idx) {
-      ElemUninit u;
-      u.id = elem_id_tracker + idx;
-      u.element = elems[elem_id].element;
-      u.ancestors = elems[elem_id].ancestors;
-      u.ancestors.push_back({elem_id, r.loc(idx)});
-      u.h_levels = elems[elem_id].h_levels.refined(r);
-      u.poly_orders = elems[elem_id].poly_orders;
-      ne.push_back(u);
-      elems[elem_id].children.push_back(u.id);
+  const std::vector<std::pair<int, HRef>> batch(refmap.begin(), refmap.end());
+  if (threads == 1 || batch.size() < kParallelRefinements ||
+      !execute_h_refinements_planned(batch, threads, extensions)) {
+    int elem_id_tracker = int(elems.size());
+    AppendOps ops(*this);
+    for (const auto& [elem_id, r] : batch) {
+      // Elem::h_refine (elem.rs:128-155)
+      if (elems[elem_id].has_children())
+        throw HRefError(HRefErrorKind::ElemHasChildren, elem_id,
+                        "Elem " + std::to_string(elem_id) + " already has children; Cannot h-refine!");
+      std::vector<ElemUninit> ne = uninit_children(elem_id, r, elem_id_tracker);
+      elem_id_tracker += r.num_children();
+      if (r.kind == HRef::Kind::T) {
+        execute_t_refinement(ne, elem_id, ops);
+      } else if (r.kind == HRef::Kind::U) {
+        execute_u_refinement(ne, elem_id, ops);
+        if (r.ext >= 0) extensions.push_back({ne[size_t(r.ext)].id, HRef::v()});
+      } else {
+        execute_v_refinement(ne, elem_id, ops);
+        if (r.ext >= 0) extensions.push_back({ne[size_t(r.ext)].id, HRef::u()});
+      }
+      for (auto& e : upgrade_uninit_elems(ne)) elems.push_back(std::move(e));
     }
-    elem_id_tracker += n;
-
-    std::vector<Elem> created;
-    if (r.kind == HRef::Kind::T) {
-      created = execute_t_refinement(ne, elem_id);
-    } else if (r.kind == HRef::Kind::U) {
-      created = execute_u_refinement(ne, elem_id);
-      if (r.ext >= 0) extensions.push_back({created[r.ext].id, HRef::v()});
-    } else {
-      created = execute_v_refinement(ne, elem_id);
-      if (r.ext >= 0) extensions.push_back({created[r.ext].id, HRef::u()});
-    }
-    for (auto& e : created) elems.push_back(std::move(e));
   }
 
-  if (!extensions.empty()) execute_h_refinements(extensions);
-  set_edge_activation();
+  if (!extensions.empty()) execute_h_refinements(extensions, threads);
+  set_edge_activation(threads);
 }
 
-std::vector<Elem> Mesh::execute_t_refinement(std::vector<ElemUninit>& ne, int parent) {
+// The outer Edge slots a refinement splits (if not yet split), in the order the
+// sequential loop visits them
+static int refine_slots(HRef::Kind kind, int (&slot)[4]) {
+  if (kind == HRef::Kind::T) {
+    for (int k = 0; k < 4; ++k) slot[k] = k;
+    return 4;
+  }
+  if (kind == HRef::Kind::U) {
+    slot[0] = 0;
+    slot[1] = 1;
+  } else {
+    slot[0] = 2;
+    slot[1] = 3;
+  }
+  return 2;
+}
+
+// Ids of everything a batch creates, assigned in the sequential loop's order
+// from per-refinement counts: refinement k creates its children, then (T) a
+// centre Node, and for each outer Edge slot in order the split of that Edge
+// (two Edges and a Node) when k is the batch's first refinement to meet the
+// Edge unsplit, and (T) the inner Edge of the slot; U and V create their one
+// inner Edge after both slots.
+struct Mesh::RefinePlan {
+  std::vector<int> first_child, center, inner;  // inner: 4 per refinement (-1 unused)
+  std::vector<int> split_c0, split_c1, split_node;  // per Edge split by the batch, else -1
+  std::vector<int> split_owner;                     // the refinement that splits it, else -1
+  int new_elems = 0, new_nodes = 0, new_edges = 0;
+};
+
+struct Mesh::PlannedOps : Mesh::RefineOps {
+  Mesh& m;
+  const RefinePlan& plan;
+  int k, used = 0;
+  PlannedOps(Mesh& mesh, const RefinePlan& p, int refinement) : m(mesh), plan(p), k(refinement) {}
+  int center_node(const Point& p) override {
+    Node n;
+    n.id = plan.center[size_t(k)];
+    n.coords = p;
+    n.boundary = false;
+    m.nodes[size_t(n.id)] = n;
+    return n.id;
+  }
+  std::pair<std::array<int, 2>, int> split(int edge_id) override {
+    const size_t e = size_t(edge_id);
+    if (plan.split_owner[e] >= 0) return {{plan.split_c0[e], plan.split_c1[e]}, plan.split_node[e]};
+    return {m.edges[e].children, m.edges[e].child_node};  // split before the batch
+  }
+  int inner_edge(std::array<int, 2> node_ids, int parent_elem) override {
+    const int id = plan.inner[size_t(4 * k + used++)];
+    const Node& n0 = m.nodes[size_t(node_ids[0])];
+    const Node& n1 = m.nodes[size_t(node_ids[1])];
+    const int ord = m.elements[size_t(m.elems[size_t(parent_elem)].element)].order_points(n0.coords, n1.coords);
+    m.edges[size_t(id)] = ord < 0 ? make_edge(id, n0, n1, false) : make_edge(id, n1, n0, false);
+    return id;
+  }
+};
+
+bool Mesh::execute_h_refinements_planned(const std::vector<std::pair<int, HRef>>& batch, int threads,
+                                         std::vector<std::pair<int, HRef>>& extensions) {
+  const size_t K = batch.size(), n_edges0 = edges.size();
+  RefinePlan P;
+  P.first_child.resize(K);
+  P.center.assign(K, -1);
+  P.inner.assign(4 * K, -1);
+  P.split_c0.assign(n_edges0, -1);
+  P.split_c1.assign(n_edges0, -1);
+  P.split_node.assign(n_edges0, -1);
+  P.split_owner.assign(n_edges0, -1);
+  // the plan, in the batch's order (integer work only), with the checks that
+  // would make the sequential loop throw: then nothing is changed here
+  int elem_next = int(elems.size()), node_next = int(nodes.size()), edge_next = int(n_edges0);
+  for (size_t k = 0; k < K; ++k) {
+    const int elem_id = batch[k].first;
+    const HRef& r = batch[k].second;
+    if (elems[size_t(elem_id)].has_children()) return false;
+    P.first_child[k] = elem_next;
+    elem_next += r.num_children();
+    const bool t = r.kind == HRef::Kind::T;
+    auto pp = elem_points(elem_id);
+    const Point centre = Point::between(*pp[0], *pp[3]);
+    if (t) P.center[k] = node_next++;
+    int slot[4];
+    const int n_slots = refine_slots(r.kind, slot);
+    Point shared[4];
+    for (int s = 0; s < n_slots; ++s) {
+      const int e = elems[size_t(elem_id)].edges[size_t(slot[s])];
+      const Edge& ed = edges[size_t(e)];
+      if (ed.has_children()) {
+        shared[s] = nodes[size_t(ed.child_node)].coords;
+      } else {
+        if (P.split_owner[size_t(e)] < 0) {
+          if (ed.length / 2.0 < MIN_EDGE_LENGTH) return false;
+          P.split_owner[size_t(e)] = int(k);
+          P.split_c0[size_t(e)] = edge_next;
+          P.split_c1[size_t(e)] = edge_next + 1;
+          P.split_node[size_t(e)] = node_next++;
+          edge_next += 2;
+        }
+        shared[s] = Point::between(nodes[size_t(ed.nodes[0])].coords, nodes[size_t(ed.nodes[1])].coords);
+      }
+      if (t) {
+        P.inner[4 * k + size_t(s)] = edge_next++;
+        if (elements[size_t(elems[size_t(elem_id)].element)].order_points(shared[s], centre) == 0) return false;
+      }
+    }
+    if (!t) {
+      P.inner[4 * k] = edge_next++;
+      if (elements[size_t(elems[size_t(elem_id)].element)].order_points(shared[0], shared[1]) == 0) return false;
+    }
+  }
+  P.new_elems = elem_next - int(elems.size());
+  P.new_nodes = node_next - int(nodes.size());
+  P.new_edges = edge_next - int(n_edges0);
+
+  // storage for everything new, then three parallel passes: the splits (their
+  // Edges and Nodes, which other refinements of the batch share), the
+  // refinements themselves, the Edges' and Nodes' Elem maps
+  const size_t elems0 = elems.size();
+  elems.resize(elems0 + size_t(P.new_elems));
+  nodes.resize(nodes.size() + size_t(P.new_nodes));
+  edges.resize(n_edges0 + size_t(P.new_edges));
+  parallel_ranges(K, threads, 64, [&](size_t b, size_t e) {
+    for (size_t k = b; k < e; ++k) {
+      const int elem_id = batch[k].first;
+      int slot[4];
+      const int n_slots = refine_slots(batch[k].second.kind, slot);
+      for (int s = 0; s < n_slots; ++s) {
+        const int eid = elems[size_t(elem_id)].edges[size_t(slot[s])];
+        if (P.split_owner[size_t(eid)] != int(k)) continue;
+        // Edge::h_refine (edge.rs:126-170), at the planned ids
+        Edge& parent = edges[size_t(eid)];
+        const int new0 = P.split_c0[size_t(eid)], new1 = P.split_c1[size_t(eid)], node = P.split_node[size_t(eid)];
+        parent.has_children_ = true;
+        parent.children = {new0, new1};
+        parent.child_node = node;
+        Edge c0, c1;
+        c0.id = new0;
+        c0.nodes = {parent.nodes[0], node};
+        c1.id = new1;
+        c1.nodes = {node, parent.nodes[1]};
+        for (Edge* c : {&c0, &c1}) {
+          c->boundary = parent.boundary;
+          c->dir = parent.dir;
+          c->length = parent.length / 2.0;
+          c->parent = eid;
+        }
+        edges[size_t(new0)] = c0;
+        edges[size_t(new1)] = c1;
+        Node n;
+        n.id = node;
+        n.coords = Point::between(nodes[size_t(parent.nodes[0])].coords, nodes[size_t(parent.nodes[1])].coords);
+        n.boundary = parent.boundary;
+        nodes[size_t(node)] = n;
+      }
+    }
+  });
+  std::vector<std::vector<std::pair<int, HRef>>> ext(K);
+  parallel_ranges(K, threads, 64, [&](size_t b, size_t e) {
+    for (size_t k = b; k < e; ++k) {
+      const int elem_id = batch[k].first;
+      const HRef& r = batch[k].second;
+      std::vector<ElemUninit> ne = uninit_children(elem_id, r, P.first_child[k]);
+      PlannedOps ops(*this, P, int(k));
+      if (r.kind == HRef::Kind::T) {
+        execute_t_refinement(ne, elem_id, ops);
+      } else if (r.kind == HRef::Kind::U) {
+        execute_u_refinement(ne, elem_id, ops);
+        if (r.ext >= 0) ext[k].push_back({ne[size_t(r.ext)].id, HRef::v()});
+      } else {
+        execute_v_refinement(ne, elem_id, ops);
+        if (r.ext >= 0) ext[k].push_back({ne[size_t(r.ext)].id, HRef::u()});
+      }
+      for (const ElemUninit& u : ne) elems[size_t(u.id)] = u.into_elem();
+    }
+  });
+  for (auto& v : ext) extensions.insert(extensions.end(), v.begin(), v.end());
+  // an Elem's connection to its Edges and Nodes (maps keyed by h-level, one
+  // entry per Elem: the order of insertion does not change them); striped locks
+  constexpr size_t kStripes = 1024;
+  std::vector<std::mutex> edge_lock(kStripes), node_lock(kStripes);
+  parallel_ranges(size_t(P.new_elems), threads, 256, [&](size_t b, size_t e) {
+    for (size_t x = elems0 + b; x < elems0 + e; ++x) {
+      const Elem& el = elems[x];
+      for (int k = 0; k < 4; ++k) {
+        const size_t eid = size_t(el.edges[size_t(k)]);
+        std::lock_guard<std::mutex> g(edge_lock[eid % kStripes]);
+        edges[eid].connect_elem(el);
+      }
+      for (int k = 0; k < 4; ++k) {
+        const size_t nid = size_t(el.nodes[size_t(k)]);
+        std::lock_guard<std::mutex> g(node_lock[nid % kStripes]);
+        nodes[nid].connect_elem(el);
+      }
+    }
+  });
+  return true;
+}
+
+void Mesh::execute_t_refinement(std::vector<ElemUninit>& ne, int parent, RefineOps& ops) {
   if (ne.size() != 4) panic("T refinement needs 4 children");
   auto pp = elem_points(parent);
-  const int center_id = int(nodes.size());
-  Node center;
-  center.id = center_id;
-  center.coords = Point::between(*pp[0], *pp[3]);
-  center.boundary = false;
-  nodes.push_back(center);
+  const int center_id = ops.center_node(Point::between(*pp[0], *pp[3]));
 
   for (int idx = 0; idx < 4; ++idx) {
     ne[idx].set_node(3 - idx, center_id);
@@ -708,24 +938,23 @@ std::vector<Elem> Mesh::execute_t_refinement(std::vector<ElemUninit>& ne, int pa
       {3, 1, 3, 3, 1, 1, 0},
   };
   for (const auto& t : kT) {
-    auto [child_edges, shared] = h_refine_edge_if_needed(elems[parent].edges[t[0]]);
+    auto [child_edges, shared] = ops.split(elems[parent].edges[t[0]]);
     ne[t[1]].set_edge(t[0], child_edges[0]);
     ne[t[2]].set_edge(t[0], child_edges[1]);
     ne[t[1]].set_node(t[3], shared);
     ne[t[2]].set_node(t[4], shared);
-    int new_edge = new_edge_between_nodes({shared, center_id}, parent);
+    int new_edge = ops.inner_edge({shared, center_id}, parent);
     ne[t[1]].set_edge(t[5], new_edge);
     ne[t[2]].set_edge(t[6], new_edge);
   }
-  return upgrade_uninit_elems(ne);
 }
 
-std::vector<Elem> Mesh::execute_u_refinement(std::vector<ElemUninit>& ne, int parent) {
+void Mesh::execute_u_refinement(std::vector<ElemUninit>& ne, int parent, RefineOps& ops) {
   if (ne.size() != 2) panic("U refinement needs 2 children");
   std::array<int, 2> outer = {0, 0};
   static const int kU[2][5] = {{0, 1, 0, 0, 1}, {1, 3, 2, 2, 3}};
   for (const auto& t : kU) {
-    auto [child_edges, shared] = h_refine_edge_if_needed(elems[parent].edges[t[0]]);
+    auto [child_edges, shared] = ops.split(elems[parent].edges[t[0]]);
     outer[t[0]] = shared;
     ne[0].set_edge(t[0], child_edges[0]);
     ne[1].set_edge(t[0], child_edges[1]);
@@ -734,20 +963,19 @@ std::vector<Elem> Mesh::execute_u_refinement(std::vector<ElemUninit>& ne, int pa
     ne[0].set_node(t[3], elems[parent].nodes[t[3]]);
     ne[1].set_node(t[4], elems[parent].nodes[t[4]]);
   }
-  int new_edge = new_edge_between_nodes(outer, parent);
+  int new_edge = ops.inner_edge(outer, parent);
   ne[0].set_edge(3, new_edge);
   ne[1].set_edge(2, new_edge);
   ne[0].set_edge(2, elems[parent].edges[2]);
   ne[1].set_edge(3, elems[parent].edges[3]);
-  return upgrade_uninit_elems(ne);
 }
 
-std::vector<Elem> Mesh::execute_v_refinement(std::vector<ElemUninit>& ne, int parent) {
+void Mesh::execute_v_refinement(std::vector<ElemUninit>& ne, int parent, RefineOps& ops) {
   if (ne.size() != 2) panic("V refinement needs 2 children");
   std::array<int, 2> outer = {0, 0};
   static const int kV[2][5] = {{2, 2, 0, 0, 2}, {3, 3, 1, 1, 3}};
   for (const auto& t : kV) {
-    auto [child_edges, shared] = h_refine_edge_if_needed(elems[parent].edges[t[0]]);
+    auto [child_edges, shared] = ops.split(elems[parent].edges[t[0]]);
     outer[t[0] - 2] = shared;
     ne[0].set_edge(t[0], child_edges[0]);
     ne[1].set_edge(t[0], child_edges[1]);
@@ -756,12 +984,11 @@ std::vector<Elem> Mesh::execute_v_refinement(std::vector<ElemUninit>& ne, int pa
     ne[0].set_node(t[3], elems[parent].nodes[t[3]]);
     ne[1].set_node(t[4], elems[parent].nodes[t[4]]);
   }
-  int new_edge = new_edge_between_nodes(outer, parent);
+  int new_edge = ops.inner_edge(outer, parent);
   ne[0].set_edge(1, new_edge);
   ne[1].set_edge(0, new_edge);
   ne[0].set_edge(0, elems[parent].edges[0]);
   ne[1].set_edge(1, elems[parent].edges[1]);
-  return upgrade_uninit_elems(ne);
 }
 
 std::pair<std::array<int, 2>, int> Mesh::h_refine_edge_if_needed(int edge_id) {

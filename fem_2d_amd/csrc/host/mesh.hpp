@@ -249,7 +249,12 @@ class Mesh {
   // h-refinement (mesh.rs:713-914)
   void global_h_refinement(const HRef& r);
   void h_refine_elems(const std::vector<int>& ids, const HRef& r);
-  void execute_h_refinements(const std::vector<std::pair<int, HRef>>& refinements);
+  // The batch of refinements, with the ids the reference's sequential loop
+  // gives every new Elem, Node and Edge. Batches of at least
+  // kParallelRefinements Elems run data-parallel on up to `threads` threads
+  // (0: the machine's cores, at most 16; 1: the sequential loop).
+  void execute_h_refinements(const std::vector<std::pair<int, HRef>>& refinements, int threads = 0);
+  static constexpr size_t kParallelRefinements = 256;
 
   // p-refinement and expansion orders (mesh.rs:1265-1665)
   void global_p_refinement(const PRef& r);
@@ -267,12 +272,24 @@ class Mesh {
 
  private:
   struct ElemUninit;
-  std::vector<Elem> execute_t_refinement(std::vector<ElemUninit>& ne, int parent);
-  std::vector<Elem> execute_u_refinement(std::vector<ElemUninit>& ne, int parent);
-  std::vector<Elem> execute_v_refinement(std::vector<ElemUninit>& ne, int parent);
+  struct RefineOps;
+  struct AppendOps;
+  struct PlannedOps;
+  struct RefinePlan;
+  // the children of one refinement: their Nodes and Edges set through ops,
+  // which creates what is new
+  void execute_t_refinement(std::vector<ElemUninit>& ne, int parent, RefineOps& ops);
+  void execute_u_refinement(std::vector<ElemUninit>& ne, int parent, RefineOps& ops);
+  void execute_v_refinement(std::vector<ElemUninit>& ne, int parent, RefineOps& ops);
   std::pair<std::array<int, 2>, int> h_refine_edge_if_needed(int edge_id);
   int new_edge_between_nodes(std::array<int, 2> node_ids, int parent_elem);
   std::vector<Elem> upgrade_uninit_elems(std::vector<ElemUninit>& ne);
+  // the children (uninitialised) of refinement r of parent elem_id, first id first_id
+  std::vector<ElemUninit> uninit_children(int elem_id, const HRef& r, int first_id);
+  // the batch run data-parallel; false (nothing changed) when a check of the
+  // sequential loop would fail, which then runs instead and reports it
+  bool execute_h_refinements_planned(const std::vector<std::pair<int, HRef>>& batch, int threads,
+                                     std::vector<std::pair<int, HRef>>& extensions);
   bool rec_set_edge_activation(int edge_id);
 };
 

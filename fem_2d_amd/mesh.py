@@ -396,12 +396,15 @@ class Mesh:
         ids = _i32(elems)
         check(N.load().fem2d_mesh_h_refine_elems(self._h, ids, len(elems), refinement.kind, refinement.ext))
 
-    def execute_h_refinements(self, refinements: Iterable[Tuple[int, HRef]]) -> None:
+    def execute_h_refinements(self, refinements: Iterable[Tuple[int, HRef]], n_threads: int = 0) -> None:
+        """Mesh::execute_h_refinements (mesh.rs:844-914). Batches of at least 256 Elems run on
+        ``n_threads`` host threads (0: the machine's cores, at most 16; 1: sequentially), with the
+        ids of the sequential loop."""
         refs = list(refinements)
         ids = _i32([r[0] for r in refs])
         kinds = _i32([r[1].kind for r in refs])
         exts = _i32([r[1].ext for r in refs])
-        check(N.load().fem2d_mesh_execute_h_refinements(self._h, ids, kinds, exts, len(refs)))
+        check(N.load().fem2d_mesh_execute_h_refinements_threads(self._h, ids, kinds, exts, len(refs), int(n_threads)))
 
     def h_refine_with_filter(self, filt: Callable[[Elem], Optional[HRef]]) -> None:
         """mesh.rs:789-808: the filter sees every h-refineable Elem in id order."""

@@ -469,6 +469,11 @@ int fem2d_mesh_global_h_refinement(fem2d_mesh* m, int32_t kind, int32_t ext);
 int fem2d_mesh_h_refine_elems(fem2d_mesh* m, const int32_t* ids, int64_t n, int32_t kind, int32_t ext);
 int fem2d_mesh_execute_h_refinements(fem2d_mesh* m, const int32_t* ids, const int32_t* kinds, const int32_t* exts,
                                      int64_t n);
+/* the same batch on n_threads host threads (0: the machine's cores, at most 16;
+   1: the reference's sequential loop, mesh.rs:844-914); batches of at least 256
+   Elems run data-parallel with the sequential loop's ids, the same Mesh */
+int fem2d_mesh_execute_h_refinements_threads(fem2d_mesh* m, const int32_t* ids, const int32_t* kinds,
+                                             const int32_t* exts, int64_t n, int32_t n_threads);
 int fem2d_mesh_global_p_refinement(fem2d_mesh* m, int32_t di, int32_t dj);
 int fem2d_mesh_p_refine_elems(fem2d_mesh* m, const int32_t* ids, int64_t n, int32_t di, int32_t dj);
 int fem2d_mesh_execute_p_refinements(fem2d_mesh* m, const int32_t* ids, const int32_t* dis, const int32_t* djs,
