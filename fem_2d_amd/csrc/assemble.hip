@@ -42,6 +42,8 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -2750,6 +2752,15 @@ FillOut make_fill_out(int precision, int ai, int bi, long long capacity, int* co
 constexpr int kChunkCandidates[] = {128, 256, 512, 1024};
 constexpr int kTuneReps = 3;
 
+// FEM2D_TUNE_VERBOSE=1 prints every tuning trial's time to stderr
+bool tune_verbose() {
+  static const bool on = [] {
+    const char* e = std::getenv("FEM2D_TUNE_VERBOSE");
+    return e != nullptr && e[0] != '\0' && e[0] != '0';
+  }();
+  return on;
+}
+
 // CSR entries per fill chunk for a stored precision (before any tuning)
 int chunk_of(int precision) {
   switch (precision) {
@@ -3282,6 +3293,9 @@ int fem2d_assemble_device(fem2d_ctx* ctx, const fem2d_device_domain* dd, int bas
           float ms = 0.0f;
           HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
           if (ctx->h_status[1]) return err_to_status(ctx->h_status[1]);
+          if (tune_verbose())
+            std::fprintf(stderr, "[fem2d tune] precision %d placement %d chunk %d: %.4f ms\n", precision, place, c,
+                         double(ms));
           if (ms < best_ms) {
             best_ms = ms;
             best = c;
