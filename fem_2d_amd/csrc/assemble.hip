@@ -465,6 +465,20 @@ __device__ __forceinline__ void st32(T* __restrict__ base, unsigned idx, T v) {
   __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(T))));
 }
 
+// A wave-uniform load of data no kernel in flight writes, through the scalar
+// cache: with a uniform index it compiles to a scalar load, which the lgkm
+// counter tracks, so it does not wait for the wavefront's earlier vector stores
+// the way a vector load does (the vector memory counter retires in issue order)
+template <class T>
+__device__ __forceinline__ T ld_const(const T* base, long long idx) {
+  using ConstT = const __attribute__((address_space(4))) T;
+  return ((ConstT*)base)[idx];  // only a C-style cast converts to the constant address space
+}
+__device__ __forceinline__ int4 ld_const(const int4* base, long long idx) {
+  const int* p = reinterpret_cast<const int*>(base + idx);
+  return make_int4(ld_const(p, 0), ld_const(p, 1), ld_const(p, 2), ld_const(p, 3));
+}
+
 template <class T>
 __device__ __forceinline__ T ld32(const T* __restrict__ base, unsigned idx) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * unsigned(sizeof(T)));
@@ -1113,13 +1127,15 @@ __global__ void __launch_bounds__(kBlock) k_nested_members(DevView v, Work w, un
 // The sources of row group g: the Elems nested with its first Elem (k = 0)
 // and with its second Elem (k = 1), read from the nested lists. Returns the
 // count; thread t < count of the calling group (stride nthreads) stores source t.
+// g is uniform over the caller's wavefront: the group's Elems and list bounds
+// come through scalar loads.
 __device__ __forceinline__ int gather_sources(const Work& w, int g, int cap, int* xs, int* info, int t,
                                               int nthreads) {
-  const int e1 = w.g_e1[g], e2 = w.g_e2[g];
+  const int e1 = ld_const(w.g_e1, g), e2 = ld_const(w.g_e2, g);
   if (e1 < 0) return 0;  // a DoF without basis specs (flagged as an invalid view)
-  const int a1 = w.nested_ptr[e1], n1 = w.nested_ptr[e1 + 1] - a1;
-  const int a2 = e2 >= 0 ? w.nested_ptr[e2] : 0;
-  const int n2 = e2 >= 0 ? w.nested_ptr[e2 + 1] - a2 : 0;
+  const int a1 = ld_const(w.nested_ptr, e1), n1 = ld_const(w.nested_ptr, e1 + 1) - a1;
+  const int a2 = e2 >= 0 ? ld_const(w.nested_ptr, e2) : 0;
+  const int n2 = e2 >= 0 ? ld_const(w.nested_ptr, e2 + 1) - a2 : 0;
   const int n = n1 + n2;
   if (n > cap) return n;
   for (int s = t; s < n; s += nthreads) {
@@ -1136,7 +1152,6 @@ __device__ __forceinline__ int gather_sources(const Work& w, int g, int cap, int
 // ---------------------------------------------------------------------------
 
 // a wave-uniform value loaded through the vector path, moved to a scalar register
-__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ long long uniform(long long x) {
   const unsigned long long u = (unsigned long long)x;
   return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(int(u >> 32)) << 32) |
@@ -1302,7 +1317,7 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
   const int n_dist = running;
   // prefix sums of the distinct group sizes; locate the group itself
   int carry = 0;
-  const int gs_self = w.g_start[g];
+  const int gs_self = ld_const(w.g_start, g);
   for (int base = 0; base < n_dist; base += kWave) {
     const int t = base + lane;
     const int sz = t < n_dist ? S.cum[t] : 0;
@@ -1323,7 +1338,7 @@ __device__ int wave_build(const DevView& v, const Work& w, int g, WaveShared& S,
     if (lane == 0) set_err(w.err, kErrView);
     return 2;
   }
-  if (S.total - S.u_self > kWC || w.g_start[g + 1] - gs_self > kWRows) return 1;
+  if (S.total - S.u_self > kWC || ld_const(w.g_start, g + 1) - gs_self > kWRows) return 1;
   return 0;
 }
 
@@ -1336,7 +1351,7 @@ __global__ void __launch_bounds__(kBlock, kLatencyWavesPerSimd) k_describe_wave(
   if (*w.err) return;  // an invalid view was detected: nothing below may trust it
   __shared__ WaveShared shared[kWavesPerBlock];
   const int lane = threadIdx.x % kWave;
-  const int wid = threadIdx.x / kWave;
+  const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x / kWave));
   WaveShared& S = shared[wid];
   const int n_groups = n_groups_of(w);
   // static assignment: a shared work counter would serialise on device-scope
@@ -1349,7 +1364,7 @@ __global__ void __launch_bounds__(kBlock, kLatencyWavesPerSimd) k_describe_wave(
     }
     const long long gs = (long long)g * kWS;
     const int st = wave_build(v, w, g, S, lane, w.meta_coef + gs * kCoef);
-    const int r0 = w.g_start[g], r1 = w.g_start[g + 1];
+    const int r0 = ld_const(w.g_start, g), r1 = ld_const(w.g_start, g + 1);
     const int nrows = r1 - r0;
     if (st == 1) {
       if (lane == 0) {
@@ -1561,11 +1576,14 @@ __global__ void __launch_bounds__((FillShape<P, kSmall>::kBlock), kFillWavesPerS
   }
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(int((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
-  const int waves = int((gridDim.x * blockDim.x) / kWave);
+  const int waves = __builtin_amdgcn_readfirstlane(int(gridDim.x) * (FillShape<P, kSmall>::kBlock / kWave));
   const long long chunk = w.chunk;  // the layout scan's chunk size (default or tuned per Domain)
   const long long n_chunks = min((nnz + chunk - 1) / chunk, w.chunk_cap);
   const double* __restrict__ lib = kLibInLds ? lib_lds : w.lib;
   const unsigned tab = unsigned(L.tab);
+  // per-group values are wave-uniform: they come through scalar loads and stay
+  // in scalar registers (with the record bases). Loading each chunk's first row
+  // group one chunk ahead was measured 2-3 % slower.
   for (long long ch = wave; ch < n_chunks; ch += waves) {
     const long long pbase = ch * chunk;
     const int len = int(min(chunk, nnz - pbase));
@@ -1574,17 +1592,12 @@ __global__ void __launch_bounds__((FillShape<P, kSmall>::kBlock), kFillWavesPerS
     VA* __restrict__ a_out = static_cast<VA*>(o.aval) + pb;
     VB* __restrict__ b_out = static_cast<VB*>(o.bval) + pb;
     unsigned* __restrict__ r_out = Store<P>::kRes ? o.res + pb : nullptr;
-    // per-group values are wave-uniform: keep them (and the record bases) in
-    // scalar registers
-    int g = uniform(w.chunk_first[ch]);
+    int g = ld_const(w.chunk_first, ch);
     int j0 = 0;
     while (j0 < len) {
-      const long long S = uniform((long long)w.seg_start[g]), S1 = uniform((long long)w.seg_start[g + 1]);
+      const long long S = ld_const(w.seg_start, g), S1 = ld_const(w.seg_start, g + 1);
       const int j1 = int(min((long long)len, S1 - pbase));
-      int4 hdr = w.meta_hdr[g];
-      hdr.x = uniform(hdr.x);
-      hdr.y = uniform(hdr.y);
-      hdr.z = uniform(hdr.z);
+      const int4 hdr = ld_const(w.meta_hdr, g);
       if (hdr.x > 0) {
         const int T = hdr.x, nrows = hdr.y;
         const int off = int(pbase - S);  // segment position of chunk entry 0
