@@ -1725,7 +1725,8 @@ struct GroupShared {
   double cB[kMaxSrc][2];
   int src_x[kMaxSrc];
   int src_info[kMaxSrc];
-  int raw_cnt[kMaxSrc + 1];
+  int raw_cnt[kMaxSrc + 1];  // first raw candidate of each source
+  int raw_m0[kMaxSrc];       // first membership of each source
   unsigned long long key[kMaxRaw];
   short raw_src[kMaxRaw];
   int raw_off[kMaxRaw];
@@ -1770,38 +1771,47 @@ __device__ bool build_group(const DevView& v, const Work& w, int g, GroupShared&
     return false;
   }
   __syncthreads();
+  // membership counts of the sources -> their first raw candidates (one source
+  // per thread: kMaxSrc <= kBlock)
+  static_assert(kMaxSrc <= kBlock, "one source per thread");
+  int cnt = 0, m0 = 0;
+  if (tid < n) {
+    const int X = S.src_x[tid];
+    m0 = w.mem_idx[v.bs_off[X]];
+    cnt = w.mem_idx[v.bs_off[X + 1]] - m0;
+  }
+  int n_raw = 0;
+  const int first_raw = block_excl_scan(cnt, S.wave_sum, &n_raw);
+  if (n_raw > kMaxRaw) {  // block-uniform
+    if (tid == 0) set_err(w.err, kErrSize);
+    return false;
+  }
+  if (tid < n) {
+    S.raw_cnt[tid] = first_raw;
+    S.raw_m0[tid] = m0;
+  }
   if (tid == 0) {
-    S.fail = 0;
+    S.raw_cnt[n] = n_raw;
     S.n_src = n;
-    int tot = 0;
-    for (int s = 0; s < S.n_src; ++s) {
-      S.raw_cnt[s] = tot;
-      const int X = S.src_x[s];
-      tot += w.mem_idx[v.bs_off[X + 1]] - w.mem_idx[v.bs_off[X]];
-    }
-    S.raw_cnt[S.n_src] = tot;
-    if (tot > kMaxRaw) S.fail = 1;
-    S.n_raw = tot;
+    S.n_raw = n_raw;
   }
   __syncthreads();
-  {
-    const int fail = S.fail;
-    __syncthreads();  // every thread has read the flag before it can be reset
-    if (fail) {
-      if (tid == 0) set_err(w.err, kErrSize);
-      return false;
+  const int n_src = n;
+  // raw candidates (one membership of one source each), one per thread; the
+  // source of item r is the last one whose first item is at most r
+  for (int r = tid; r < n_raw; r += blockDim.x) {
+    int a = 0, b = n_src;  // raw_cnt[a] <= r < raw_cnt[b]
+    while (b - a > 1) {
+      const int mid = (a + b) >> 1;
+      if (S.raw_cnt[mid] <= r) a = mid; else b = mid;
     }
+    const int m = S.raw_m0[a] + (r - S.raw_cnt[a]);
+    S.key[r] = ((unsigned long long)(unsigned)w.mem_group[m] << 20) | (unsigned long long)r;
+    S.raw_src[r] = short(a);
+    S.raw_off[r] = w.mem_bsoff[m];
   }
-  const int n_src = S.n_src, n_raw = S.n_raw;
   for (int s = tid; s < n_src; s += blockDim.x) {
     const int X = S.src_x[s];
-    const int m0 = w.mem_idx[v.bs_off[X]], m1 = w.mem_idx[v.bs_off[X + 1]];
-    int r = S.raw_cnt[s];
-    for (int m = m0; m < m1; ++m, ++r) {
-      S.key[r] = ((unsigned long long)(unsigned)w.mem_group[m] << 20) | (unsigned long long)r;
-      S.raw_src[r] = short(s);
-      S.raw_off[r] = w.mem_bsoff[m];
-    }
     if (kWithCoef) {
       const int info = S.src_info[s];
       const int k = info & 1, rel = (info >> 1) & 3;
