@@ -2773,7 +2773,7 @@ FillOut make_fill_out(int precision, int ai, int bi, long long capacity, int* co
 // chunk sizes the fill's autotuning times on a Domain's first assembly, each
 // placement and size timed kTuneReps times (the fastest launch counts)
 constexpr int kChunkCandidates[] = {128, 256, 512, 1024};
-constexpr int kTuneReps = 3;
+constexpr int kTuneReps = 5;
 
 // FEM2D_TUNE_VERBOSE=1 prints every tuning trial's time to stderr
 bool tune_verbose() {
