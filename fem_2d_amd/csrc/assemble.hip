@@ -1496,14 +1496,26 @@ __device__ __forceinline__ int segment_col(int p, int T, int q, int oq) { return
 // no branch between them, so that their loads are in flight together; the
 // second contribution (both specs of the row and of the column on the same
 // two Elems, a few entries in a hundred) follows.
-__device__ __forceinline__ void flat_entry2(const uint4* __restrict__ col_g, const uint2* __restrict__ row_g,
-                                            const double* __restrict__ coef_g, const double* __restrict__ lib,
-                                            unsigned tab, int qa, int ta, int qb, int tb, int& ca, double& cca,
+struct EntryRecs {  // the column and row records of two entries
+  uint4 ra, rb;
+  uint2 rpa, rpb;
+};
+
+__device__ __forceinline__ EntryRecs load_recs2(const uint4* __restrict__ col_g, const uint2* __restrict__ row_g,
+                                                int qa, int ta, int qb, int tb) {
+  EntryRecs r;
+  r.ra = ld32(col_g, unsigned(ta));
+  r.rb = ld32(col_g, unsigned(tb));
+  r.rpa = ld32(row_g, unsigned(qa));
+  r.rpb = ld32(row_g, unsigned(qb));
+  return r;
+}
+
+__device__ __forceinline__ void flat_entry2(const EntryRecs& rec, const double* __restrict__ coef_g,
+                                            const double* __restrict__ lib, unsigned tab, int& ca, double& cca,
                                             double& l2a, int& cb, double& ccb, double& l2b) {
-  const uint4 ra = ld32(col_g, unsigned(ta));
-  const uint4 rb = ld32(col_g, unsigned(tb));
-  const uint2 rpa = ld32(row_g, unsigned(qa));
-  const uint2 rpb = ld32(row_g, unsigned(qb));
+  const uint4 ra = rec.ra, rb = rec.rb;
+  const uint2 rpa = rec.rpa, rpb = rec.rpb;
   ca = int(ra.x);
   cb = int(rb.x);
   flat_contrib(lib, tab, coef_g, ra.w & 0xffu, ra.y, rpa, cca, l2a);
@@ -1604,34 +1616,53 @@ __global__ void __launch_bounds__((FillShape<P, kSmall>::kBlock), kFillWavesPerS
         const float bq = 2.0f * float(T) + 1.0f;
         const uint4* col_g = w.meta_col + (size_t)g * kWC;
         const uint2* row_g = w.meta_row + (size_t)g * kWRows;
+        // two entries per lane and iteration; past the end of the group's
+        // part of the chunk the second one repeats the first and is not stored.
+        // Each lane finds the row of its first entry from the root and then
+        // follows its entries (64 apart) row by row. The rows and columns are
+        // clamped into the group's records, so a lane past the part's end
+        // loads in bounds.
+        int qa, oqa;
+        segment_row(j0 + lane + off, T, nrows, bq, qa, oqa);
+        int j = j0 + lane;
+        bool second = j + kWave < j1;
+        int qb = qa, oqb = oqa;
+        {
+          const int pa = j + off, pb = (second ? j + kWave : j) + off;
+          segment_advance(pa, T, nrows, qa, oqa);
+          qb = qa;
+          oqb = oqa;
+          segment_advance(pb, T, nrows, qb, oqb);
+        }
+        // the first iteration's records are loaded before the source records
+        // are staged, so that the two loads are in flight together
+        EntryRecs rec = load_recs2(col_g, row_g, qa, segment_col(j + off, T, qa, oqa), qb,
+                                   segment_col((second ? j + kWave : j) + off, T, qb, oqb));
         double4* stage = src_lds[threadIdx.x / kWave];
         const double4* src_g = reinterpret_cast<const double4*>(w.meta_coef + (size_t)g * kWS * kCoef);
         for (int i = lane; i < hdr.z * (kCoef / 4); i += kWave) stage[i] = src_g[i];
         wave_sync();
         const double* coef_g = reinterpret_cast<const double*>(stage);
-        // two entries per lane and iteration; past the end of the group's
-        // part of the chunk the second one repeats the first and is not stored.
-        // Each lane finds the row of its first entry from the root and then
-        // follows its entries (64 apart) row by row.
-        int qa, oqa;
-        segment_row(j0 + lane + off, T, nrows, bq, qa, oqa);
-        for (int j = j0 + lane; j < j1; j += 2 * kWave) {
+        while (j < j1) {
           int c0, c1;
           double cc0, l20, cc1, l21;
-          const bool second = j + kWave < j1;
-          const int pa = j + off, pb = (second ? j + kWave : j) + off;
-          segment_advance(pa, T, nrows, qa, oqa);
-          int qb = qa, oqb = oqa;
-          segment_advance(pb, T, nrows, qb, oqb);
-          flat_entry2(col_g, row_g, coef_g, lib, tab, qa, segment_col(pa, T, qa, oqa), qb, segment_col(pb, T, qb, oqb),
-                      c0, cc0, l20, c1, cc1, l21);
-          qa = qb;
-          oqa = oqb;
+          flat_entry2(rec, coef_g, lib, tab, c0, cc0, l20, c1, cc1, l21);
           store_entry<P>(col_out, a_out, b_out, r_out, unsigned(j), c0, kACurlCurl ? cc0 : l20,
                          kBCurlCurl ? cc0 : l20);
           if (second)
             store_entry<P>(col_out, a_out, b_out, r_out, unsigned(j + kWave), c1, kACurlCurl ? cc1 : l21,
                            kBCurlCurl ? cc1 : l21);
+          j += 2 * kWave;
+          if (j >= j1) break;
+          qa = qb;
+          oqa = oqb;
+          second = j + kWave < j1;
+          const int pa = j + off, pb = (second ? j + kWave : j) + off;
+          segment_advance(pa, T, nrows, qa, oqa);
+          qb = qa;
+          oqb = oqa;
+          segment_advance(pb, T, nrows, qb, oqb);
+          rec = load_recs2(col_g, row_g, qa, segment_col(pa, T, qa, oqa), qb, segment_col(pb, T, qb, oqb));
         }
         wave_sync();  // every lane is done with the staged records
       }
