@@ -620,10 +620,12 @@ void Mesh::global_h_refinement(const HRef& r) {
 }
 
 void Mesh::h_refine_elems(const std::vector<int>& ids, const HRef& r) {
-  for (size_t a = 0; a < ids.size(); ++a)
-    for (size_t b = a + 1; b < ids.size(); ++b)
-      if (ids[a] == ids[b])
-        throw HRefError(HRefErrorKind::DuplicateElemIds, -1, "Duplicate element ids in h-Refinement; Cannot h-Refine!");
+  // any repeated id fails the batch (mesh.rs:754-760 compares every pair; a
+  // sorted copy finds the same repeats in n log n)
+  std::vector<int> sorted(ids);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    throw HRefError(HRefErrorKind::DuplicateElemIds, -1, "Duplicate element ids in h-Refinement; Cannot h-Refine!");
   std::vector<std::pair<int, HRef>> refs;
   for (int id : ids) refs.push_back({id, r});
   execute_h_refinements(refs);
@@ -1105,11 +1107,11 @@ void Mesh::global_p_refinement(const PRef& r) {
 }
 
 void Mesh::p_refine_elems(const std::vector<int>& ids, const PRef& r) {
-  for (size_t a = 0; a < ids.size(); ++a)
-    for (size_t b = a + 1; b < ids.size(); ++b)
-      if (ids[a] == ids[b])
-        throw PRefError(PRefErrorKind::DuplicateElemIds, -1,
-                        "Duplicate element ids in p-Refinement; Cannot apply p-Refinement!");
+  std::vector<int> sorted(ids);  // any repeated id fails the batch, as in h_refine_elems
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    throw PRefError(PRefErrorKind::DuplicateElemIds, -1,
+                    "Duplicate element ids in p-Refinement; Cannot apply p-Refinement!");
   std::vector<std::pair<int, PRef>> refs;
   for (int id : ids) refs.push_back({id, r});
   execute_p_refinements(refs);

@@ -77,6 +77,9 @@ def test_h_refine_elems_counts_and_errors():
     with pytest.raises(F.HRefError) as e:
         m.h_refine_elems([1, 1], F.HRef.T)
     assert e.value.kind == "DuplicateElemIds"
+    with pytest.raises(F.HRefError) as e:  # a repeat that is not adjacent
+        m.h_refine_elems([2, 1, 3, 1], F.HRef.T)
+    assert e.value.kind == "DuplicateElemIds"
     assert m.num_elems == 11
 
 
@@ -129,6 +132,9 @@ def test_p_refine_elems():
         m.p_refine_elems([15], F.PRef(1, 1))
     with pytest.raises(F.PRefError):
         m.p_refine_elems([0, 0], F.PRef(1, 1))
+    with pytest.raises(F.PRefError) as e:  # a repeat that is not adjacent
+        m.p_refine_elems([0, 1, 0], F.PRef(1, 1))
+    assert e.value.kind == "DuplicateElemIds"
 
 
 def _iso_aniso_mesh():
