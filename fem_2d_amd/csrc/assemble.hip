@@ -225,12 +225,13 @@ struct Work {
   unsigned char* is_large;
   // per row group, fixed-stride records written by the describe kernel and
   // streamed by the fill kernel (groups of the block path have T = -1)
-  int4* meta_hdr;      // [G] T (first row's length), rows, sources, unused
+  int4* meta_hdr;      // [G] T (first row's length), rows, sources, first row
   double* meta_coef;   // [G * kWS * kCoef] per source: CurlCurl (4) and L2Inner (2) prefactors, then
                        // int4 {x range base, y range base, k | rel << 1, 0} of the library
   uint4* meta_col;     // [G * kWC] per candidate column t: column, packed spec of the first and second
                        // contribution (~0 if none), their sources (bits 0-7 and 8-15)
-  uint2* meta_row;     // [G * kWRows] per row: packed specs on the group's first / second Elem
+  uint2* meta_row;     // [N] per DoF: packed specs on its row group's first / second Elem (written
+                       // by the DoF-group scan; a group's rows start at its header's w)
   int meta_cap;        // row groups the records have room for
   int64_t* seg_start;  // [G+1] first CSR entry of each row group
   int* chunk_first;    // [chunk_cap] row group holding the first entry of each chunk
@@ -829,6 +830,8 @@ struct DofGroupScan {
       w.g_e2[g1 - 1] = e2;
     }
     if (d == w.n_dofs - 1) w.g_start[g1] = int(w.n_dofs);
+    // the fill's row record of this DoF: its specs packed
+    w.meta_row[d] = make_uint2(it.sig.y >= 0 ? w.bs_pack[it.sig.y] : 0u, it.sig.w >= 0 ? w.bs_pack[it.sig.w] : 0u);
     if (e1 >= 0) mark_spec(it.sig.y, g1 - 1, it.n != 0, it.last);
     if (e2 >= 0) mark_spec(it.sig.w, g1 - 1, it.n != 0, it.last);
   }
@@ -1377,12 +1380,7 @@ __global__ void __launch_bounds__(kBlock, kLatencyWavesPerSimd) k_describe_wave(
       const int T = S.total - u_self;
       if (lane == 0) {
         w.is_large[g] = 0;
-        w.meta_hdr[g] = make_int4(T, nrows, n_src, 0);
-      }
-      for (int q = lane; q < nrows; q += kWave) {
-        const int r = r0 + q;
-        const int4 sg = w.dof_sig[r];
-        w.meta_row[(long long)g * kWRows + q] = make_uint2(w.bs_pack[sg.y], sg.w >= 0 ? w.bs_pack[sg.w] : 0u);
+        w.meta_hdr[g] = make_int4(T, nrows, n_src, r0);
       }
       for (int t = lane; t < T; t += kWave) {
         const int u = u_self + t;
@@ -1615,7 +1613,7 @@ __global__ void __launch_bounds__((FillShape<P, kSmall>::kBlock), kFillWavesPerS
         const int off = int(pbase - S);  // segment position of chunk entry 0
         const float bq = 2.0f * float(T) + 1.0f;
         const uint4* col_g = w.meta_col + (size_t)g * kWC;
-        const uint2* row_g = w.meta_row + (size_t)g * kWRows;
+        const uint2* row_g = w.meta_row + hdr.w;  // the group's rows are DoFs hdr.w, hdr.w + 1, ...
         // two entries per lane and iteration; past the end of the group's
         // part of the chunk the second one repeats the first and is not stored.
         // Each lane finds the row of its first entry from the root and then
@@ -2653,7 +2651,7 @@ int bind_work(const fem2d_device_domain* dd, int nu, int nv, int basis, Bind&& b
   const size_t G1 = size_t(std::max(1LL, dd->n_groups));
   w.meta_cap = int(G1);
   if ((rc = bind(k++, G1, &w.meta_hdr)) || (rc = bind(k++, G1 * kWS * kCoef, &w.meta_coef)) ||
-      (rc = bind(k++, G1 * kWC, &w.meta_col)) || (rc = bind(k++, G1 * kWRows, &w.meta_row)) ||
+      (rc = bind(k++, G1 * kWC, &w.meta_col)) || (rc = bind(k++, N1, &w.meta_row)) ||
       (rc = bind(k++, G1 + 1, &w.seg_start)))
     return rc;
 
